@@ -1,0 +1,194 @@
+"""Benchmark of the shallow-water step on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload urban8192|urban2048|dambreak256] [--gpus 1]
+
+A step is one SSP-RK2 step (two fused stage kernels, the inlet exchange and the dt kernel) over
+the whole synthetic grid, with the state resident in HBM. Default workload: the 8192 x 8192
+synthetic urban domain at 1 m (0.5 % plane + fBm, ~30 % building walls, paved/pervious n, rain
+50 mm/h, a triangular point inflow 0 -> 40 m^3/s at the west edge, free-outfall edges), spun up
+from dry for 600 s of simulated time, then W untimed and K timed steps.
+
+Prints ONE JSON line: cell-updates/s (open cells advanced per second), the roofline of the
+dominant kernel (second RK stage + sources) measured with HIP events over the timed region, and
+the FP64 CPU oracle timed on a bounded window of the same state as the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, vendor figure
+B_STAGE1 = 41                  # algorithmic bytes per grid cell, first stage: U^n 16 + z,n,mask 9 | U* 16
+B_STAGE2 = 65                  # second stage: U^n 16 + U* 16 + static 9 + hmax 4 | U^{n+1} 16 + hmax 4
+B_STEP = B_STAGE1 + B_STAGE2   # 106 B per cell-update (FP32 state with the FP32 depth residual)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(name, seed=2026):
+    from floodsim_amd import scenarios as sc
+    if name.startswith("urban"):
+        n = int(name[5:])
+        u = sc.urban(n, seed=seed)
+        mask = u["mask"].reshape(n, n)
+        jin = n // 2
+        mask[jin - 2:jin + 3, 0:5] = 1          # the inflow cell and its approach stay open
+        z = u["z"].reshape(n, n)
+        z[jin - 2:jin + 3, 0:5] = np.minimum(z[jin - 2:jin + 3, 0:5], z[jin, 5])
+        return dict(nx=n, ny=n, dx=1.0, z=u["z"], mask=u["mask"], manning=u["manning"],
+                    bc={k: "outflow" for k in ("west", "east", "south", "north")},
+                    inlets=[jin * n], rain=50 * sc.MMHR, hydro=True, lts_levels=4,
+                    checksum=sc.checksum(u["z"], u["mask"], u["manning"]))
+    if name == "dambreak256":
+        d = sc.dam_break(256)
+        return dict(nx=256, ny=256, dx=1.0, z=d["z"], mask=None, manning=0.03, bc={}, inlets=None, rain=0.0,
+                    hydro=False, depth=d["depth"], cfl=0.5)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(w, state, window=1024, steps=None, budget_s=20.0):
+    """The FP64 oracle (single thread) on a window of the same spun-up state."""
+    import oracle
+    oracle.set_threads(1)
+    n = w["nx"]
+    m = min(window, n)
+    j0 = i0 = (n - m) // 2
+    sl = (slice(j0, j0 + m), slice(i0, i0 + m))
+    cut = lambda a: np.ascontiguousarray(np.asarray(a).reshape(n, n)[sl]).reshape(-1)
+    man = w["manning"] if np.isscalar(w["manning"]) else cut(w["manning"])
+    c = oracle.CPUSolver(nx=m, ny=m, dx=w["dx"], z=cut(w["z"]), manning=man,
+                         mask=None if w["mask"] is None else cut(w["mask"]), bc=w["bc"], hdry=1e-5)
+    c.setState(cut(state["h"]), cut(state["hu"]), cut(state["hv"]), t=0.0)
+    c.rainRate = w["rain"]
+    open_cells = m * m if w["mask"] is None else int(cut(w["mask"]).sum())
+    # one step to size the sample, then as many as fit the budget
+    t0 = time.perf_counter()
+    c.step()
+    one = time.perf_counter() - t0
+    k = steps or max(2, min(200, int(budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        c.step()
+    el = time.perf_counter() - t0
+    return dict(value=open_cells * k / el, unit="cell-updates/s", cores=1, kind="port",
+                sample=f"FP64 oracle (C restatement of solver-cpu.js), 1 thread, {m}x{m} centre window of the "
+                       f"spun-up state, {k} global SSP-RK2 steps in {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="urban8192")
+    ap.add_argument("--spinup", type=float, default=600.0)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by "
+              f"domain strips or by instance; no multi-GPU run is part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import floodsim_amd as fsa
+    from floodsim_amd import scenarios as sc
+    from floodsim_amd.solver import copy_probe
+
+    t_setup = time.perf_counter()
+    w = build_workload(args.workload)
+    log(f"workload {args.workload}: generated in {time.perf_counter() - t_setup:.1f} s")
+    g = fsa.GPUSolver(nx=w["nx"], ny=w["ny"], dx=w["dx"], z=w["z"], manning=w["manning"], mask=w["mask"],
+                      bc=w["bc"], inlets=w["inlets"], cfl=w.get("cfl"), rows=args.rows)
+    if "depth" in w:
+        g.setDepth(w["depth"])
+    else:
+        g.setDepth(lambda X, Y, Z: 0 * X)
+    g.rainRate = w["rain"]
+    # spin-up: the hydrograph held per 60 s segment
+    t = 0.0
+    seg = 60.0
+    while t < args.spinup - 1e-9:
+        if w["hydro"]:
+            g.setInletRates([-float(sc.triangle_hydrograph(t + seg / 2))])
+        t = min(args.spinup, t + seg)
+        g.advance(t, 64)
+    log(f"spun up to t={g.t:.1f} s in {g.steps} steps ({time.perf_counter() - t_setup:.1f} s total)")
+    big = 1e9
+    from floodsim_amd.solver import lib
+    import ctypes
+    L = lib()
+    pool = L.fs_solver_pool(g._h)
+    sl = (ctypes.c_int32 * 1)(0)
+    tg = (ctypes.c_double * 1)(big)
+
+    def run_steps(k):
+        rc = L.fs_pool_step(pool, sl, tg, 1, int(k))
+        if rc != 0:
+            raise RuntimeError(L.fs_last_error(g._h).decode())
+
+    c0 = g.clock()
+    run_steps(args.warmup)
+    g.set_timing(True)
+    c1 = g.clock()                                   # synchronises the stream
+    t0 = time.perf_counter()
+    run_steps(args.steps)
+    c2 = g.clock()                                   # synchronises again
+    el = time.perf_counter() - t0
+    ms1, ms2, nt = g.kernel_ms()
+    g.set_timing(False)
+    steps = c2.steps - c1.steps
+    updates = c2.cellUpdates - c1.cellUpdates
+    cells = w["nx"] * w["ny"]
+    value = updates / el
+    ms_step = 1e3 * el / max(steps, 1)
+    achieved = cells * B_STAGE2 / (ms2 * 1e-3) / 1e9
+    s = g.readState()
+    try:
+        cp_gbs, _ = copy_probe(2 << 30, 10)
+    except Exception as e:  # noqa: BLE001
+        log("copy probe failed:", e)
+        cp_gbs = None
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, s)
+    out = {
+        "metric": "cell-updates/s + % HBM roofline",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 books)",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "nx": w["nx"], "ny": w["ny"], "dx_m": w["dx"],
+                   "open_cells": int(cells if w["mask"] is None else int(np.asarray(w["mask"]).sum())),
+                   "stepping": "global SSP-RK2", "spinup_s": args.spinup, "t_start": c1.t, "t_end": c2.t,
+                   "layers_checksum": w.get("checksum"), "rows_per_strip": args.rows or 64},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fs::sw_stage<1> (second RK stage + sources)",
+                     "bytes_per_cell": B_STAGE2, "cells_per_launch": cells, "avg_ms": ms2},
+        "cpu_baseline": cpu,
+        "extra": {"stage1_ms": ms1, "stage2_ms": ms2, "timed_launches": nt,
+                  "step_GBps_106B": cells * B_STEP / (ms_step * 1e-3) / 1e9,
+                  "step_frac_106B": cells * B_STEP / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                  "copy_probe_GBps": cp_gbs, "all_cell_updates_per_s": cells * steps / el,
+                  "mass_rel": s["mass"]["rel"], "last_dt": c2.lastDt},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

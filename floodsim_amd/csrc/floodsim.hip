@@ -1,0 +1,902 @@
+// floodsim_hip runtime: device memory, work lists, the stepping loop and the C ABI of
+// include/floodsim_hip.h. One translation unit with the kernels (sw_kernels.h).
+//
+// A pool owns any number of instances ("slots"), each a whole solver (the reference's GPUSolver,
+// src/solver-gpu.js:13). The slots advanced together form a group: their descriptors, their
+// wavefront strips (one work list) and their per-strip partials live in device arrays, so one
+// launch of each kernel steps every instance of the group, whatever its size. A single solver
+// (fs_handle) is a pool with one slot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/floodsim_hip.h"
+#include "sw_kernels.h"
+
+using namespace fs;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Slot {
+  bool used = false;
+  fs_params p{};
+  int nx = 0, ny = 0, pitch = 0;
+  double zref = 0.0;
+  size_t bytes = 0;
+  char *mem = nullptr;  // one device allocation for every per-cell array of the slot
+  float *h = nullptr, *hu = nullptr, *hv = nullptr, *w = nullptr;
+  float *h1 = nullptr, *hu1 = nullptr, *hv1 = nullptr, *w1 = nullptr;
+  float *z = nullptr, *n = nullptr, *hmax = nullptr, *infil = nullptr, *rf = nullptr;
+  uint8_t *mask = nullptr;
+  Inlet *inl = nullptr;
+  float *ispeed = nullptr;
+  int nin = 0;
+  std::vector<int> inletCells;           // interior k = j*nx + i
+  std::vector<uint8_t> hmask;            // host copy of the mask bits
+  std::vector<float> hrf;                // host copy of the rain multiplier (empty = 1)
+  int openCells = 0;
+  int rows = 64;
+  StepRec hrec{};                        // host mirror (refreshed after each launch group)
+  double volInitial = 0.0;               // vol.initial (solver-gpu.js:309)
+  double exchangedRead = 0.0;            // vol.exchanged: inlet volumes already read (solver-gpu.js:574)
+  double dtSum = 0.0;                    // for dtAvg
+  int64_t dtSteps = 0;
+};
+
+struct Group {
+  std::vector<int> slots;
+  DevInst *dinst = nullptr;
+  int4 *dwork = nullptr;
+  Part *p1 = nullptr, *p2 = nullptr;
+  int nwork = 0, capWork = 0, capInst = 0;
+  std::vector<DevInst> hinst;
+  std::vector<int> withInlets;           // group indices of instances with inlets
+  bool anyFirstOrder = false, anySecondOrder = false;
+  // captured K-step graph
+  hipGraphExec_t graph = nullptr;
+  int graphK = 0;
+};
+
+}  // namespace
+
+struct fs_pool {
+  size_t budget = 0, used = 0;
+  std::vector<Slot> slots;
+  StepRec *drec = nullptr;               // device records, one per slot
+  int capSlots = 0;
+  hipStream_t stream = nullptr;
+  Group grp;
+  double *dtarget = nullptr;
+  int capTarget = 0;
+  bool timing = false, useGraphs = true;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> tev;           // timing mode: 3 events per step (before, between, after the stages)
+  int nTev = 0;                          // events recorded since the last fs_pool_kernel_ms
+  double ms1 = 0.0, ms2 = 0.0;
+  int64_t nTimed = 0;
+  std::string err;
+};
+
+struct fs_solver {
+  fs_pool *pool = nullptr;
+  int slot = -1;
+};
+
+namespace {
+
+int fail(fs_pool *P, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (P) P->err = buf;
+  else g_create_error = buf;
+  return code;
+}
+
+#define HIPCHK(P, call)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return fail((P), FS_E_HIP, "%s: %s", #call, hipGetErrorString(e_));    \
+  } while (0)
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
+  const size_t N = (size_t)p->nx * p->ny;
+  size_t b = 0;
+  b += 8 * align256(N * 4);               // A and B
+  b += 3 * align256(N * 4);               // z, n, hmax
+  b += align256(N);                       // mask
+  if (infil) b += align256(N * 4);
+  if (rf) b += align256(N * 4);
+  b += align256((size_t)std::max(nin, 1) * sizeof(Inlet)) + align256((size_t)std::max(nin, 1) * 4);
+  return b;
+}
+
+int check_params(fs_pool *P, const fs_params *p) {
+  if (!p) return fail(P, FS_E_ARG, "params is NULL");
+  if (p->nx < 2 || p->ny < 2) return fail(P, FS_E_ARG, "nx and ny must be >= 2 (got %d x %d)", p->nx, p->ny);
+  if ((int64_t)p->nx * p->ny > (int64_t)1 << 31) return fail(P, FS_E_ARG, "grid too large");
+  if (!(p->dx > 0)) return fail(P, FS_E_ARG, "dx must be > 0");
+  if (!(p->cfl > 0) || !(p->hdry > 0) || !(p->dtDry > 0)) return fail(P, FS_E_ARG, "cfl, hdry, dtDry must be > 0");
+  for (int k = 0; k < 4; ++k)
+    if (p->bc[k] < 0 || p->bc[k] > 2) return fail(P, FS_E_ARG, "bc[%d] must be 0 wall, 1 open or 2 outflow", k);
+  if (p->lts != 0) return fail(P, FS_E_ARG, "local time stepping is not available in this build");
+  return FS_OK;
+}
+
+int ensure_rec_capacity(fs_pool *P, int n) {
+  if (n <= P->capSlots) return FS_OK;
+  int cap = std::max(16, P->capSlots * 2);
+  while (cap < n) cap *= 2;
+  StepRec *d = nullptr;
+  HIPCHK(P, hipMalloc(&d, sizeof(StepRec) * cap));
+  HIPCHK(P, hipMemsetAsync(d, 0, sizeof(StepRec) * cap, P->stream));
+  if (P->drec) {
+    HIPCHK(P, hipMemcpyAsync(d, P->drec, sizeof(StepRec) * P->capSlots, hipMemcpyDeviceToDevice, P->stream));
+    HIPCHK(P, hipStreamSynchronize(P->stream));
+    HIPCHK(P, hipFree(P->drec));
+  }
+  P->drec = d;
+  P->capSlots = cap;
+  P->grp.slots.clear();  // descriptors point into the old array
+  return FS_OK;
+}
+
+void drop_graph(Group &G) {
+  if (G.graph) { (void)hipGraphExecDestroy(G.graph); G.graph = nullptr; }
+  G.graphK = 0;
+}
+
+int upload_rec(fs_pool *P, int slot) {
+  HIPCHK(P, hipMemcpyAsync(P->drec + slot, &P->slots[slot].hrec, sizeof(StepRec), hipMemcpyHostToDevice, P->stream));
+  return FS_OK;
+}
+
+int download_recs(fs_pool *P, const std::vector<int> &slots) {
+  if (slots.empty()) return FS_OK;
+  int lo = *std::min_element(slots.begin(), slots.end()), hi = *std::max_element(slots.begin(), slots.end());
+  std::vector<StepRec> tmp(hi - lo + 1);
+  HIPCHK(P, hipMemcpyAsync(tmp.data(), P->drec + lo, sizeof(StepRec) * tmp.size(), hipMemcpyDeviceToHost, P->stream));
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  for (int s : slots) P->slots[s].hrec = tmp[s - lo];
+  return FS_OK;
+}
+
+// (re)build the device descriptors and work list of a group of slots
+int build_group(fs_pool *P, const std::vector<int> &slots) {
+  Group &G = P->grp;
+  if (G.slots == slots && G.dinst) return FS_OK;
+  drop_graph(G);
+  G.slots = slots;
+  G.hinst.clear();
+  G.withInlets.clear();
+  G.anyFirstOrder = G.anySecondOrder = false;
+  std::vector<int4> work;
+  for (size_t gi = 0; gi < slots.size(); ++gi) {
+    Slot &S = P->slots[slots[gi]];
+    DevInst d{};
+    d.h = S.h; d.hu = S.hu; d.hv = S.hv; d.w = S.w;
+    d.h1 = S.h1; d.hu1 = S.hu1; d.hv1 = S.hv1; d.w1 = S.w1;
+    d.z = S.z; d.n = S.n; d.mask = S.mask; d.hmax = S.hmax; d.infil = S.infil; d.rf = S.rf;
+    d.rec = P->drec + slots[gi];
+    d.nx = S.nx; d.ny = S.ny; d.pitch = S.pitch;
+    d.flags = (S.infil ? FLAG_INFIL : 0) | (S.rf ? FLAG_RF : 0);
+    d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;
+    d.hfront = (float)S.p.hfront; d.cfl = (float)S.p.cfl; d.dtDry = (float)S.p.dtDry;
+    d.area = (float)(S.p.dx * S.p.dx);
+    d.secondOrder = S.p.secondOrder ? 1 : 0;
+    d.limiter = S.p.limiter;
+    for (int k = 0; k < 4; ++k) d.bc[k] = S.p.bc[k];
+    d.wbeg = (int)work.size();
+    const int rows = std::max(1, S.rows);
+    for (int y0 = 0; y0 < S.ny; y0 += rows)
+      for (int x0 = 0; x0 < S.nx; x0 += kOwn) work.push_back(make_int4((int)gi, x0, y0, rows));
+    d.wend = (int)work.size();
+    d.inl = S.inl; d.ispeed = S.ispeed; d.nin = S.nin;
+    if (S.nin) G.withInlets.push_back((int)gi);
+    if (!S.p.secondOrder) G.anyFirstOrder = true;
+    else G.anySecondOrder = true;
+    G.hinst.push_back(d);
+  }
+  const int ni = (int)G.hinst.size(), nw = (int)work.size();
+  if (ni > G.capInst) {
+    if (G.dinst) HIPCHK(P, hipFree(G.dinst));
+    G.capInst = std::max(ni, G.capInst * 2);
+    HIPCHK(P, hipMalloc(&G.dinst, sizeof(DevInst) * G.capInst));
+  }
+  if (nw > G.capWork) {
+    if (G.dwork) HIPCHK(P, hipFree(G.dwork));
+    if (G.p1) HIPCHK(P, hipFree(G.p1));
+    if (G.p2) HIPCHK(P, hipFree(G.p2));
+    G.capWork = std::max(nw, G.capWork * 2);
+    HIPCHK(P, hipMalloc(&G.dwork, sizeof(int4) * G.capWork));
+    HIPCHK(P, hipMalloc(&G.p1, sizeof(Part) * G.capWork));
+    HIPCHK(P, hipMalloc(&G.p2, sizeof(Part) * G.capWork));
+  }
+  if (ni) HIPCHK(P, hipMemcpyAsync(G.dinst, G.hinst.data(), sizeof(DevInst) * ni, hipMemcpyHostToDevice, P->stream));
+  if (nw) HIPCHK(P, hipMemcpyAsync(G.dwork, work.data(), sizeof(int4) * nw, hipMemcpyHostToDevice, P->stream));
+  G.nwork = nw;
+  return FS_OK;
+}
+
+__global__ void sw_set_targets(const DevInst *__restrict__ insts, int n, const double *__restrict__ t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) insts[i].rec->tEnd = t[i];
+}
+
+hipEvent_t timing_event(fs_pool *P) {
+  if (P->nTev >= (int)P->tev.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    P->tev.push_back(e);
+  }
+  return P->tev[P->nTev++];
+}
+
+int launch_step(fs_pool *P) {
+  Group &G = P->grp;
+  const int nb = (G.nwork + 3) / 4, ni = (int)G.slots.size();
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  if (P->timing) {
+    e0 = timing_event(P); e1 = timing_event(P); e2 = timing_event(P);
+    if (!e0 || !e1 || !e2) return fail(P, FS_E_HIP, "hipEventCreate failed");
+    HIPCHK(P, hipEventRecord(e0, P->stream));
+  }
+  sw_stage<0><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1);
+  if (P->timing) HIPCHK(P, hipEventRecord(e1, P->stream));
+  // each instance runs its own scheme: the kernel of the other order returns at once for it
+  if (G.anySecondOrder) sw_stage<1><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2);
+  if (G.anyFirstOrder) sw_euler_finish<<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2);
+  if (P->timing) HIPCHK(P, hipEventRecord(e2, P->stream));
+  for (int gi : G.withInlets) {
+    const int nin = G.hinst[gi].nin;
+    sw_exchange<<<(nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi);
+  }
+  sw_dt_final<<<ni, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, 0);
+  HIPCHK(P, hipGetLastError());
+  return FS_OK;
+}
+
+// fold the recorded event triples into the stage totals
+int collect_timing(fs_pool *P) {
+  if (!P->nTev) return FS_OK;
+  HIPCHK(P, hipEventSynchronize(P->tev[P->nTev - 1]));
+  for (int i = 0; i + 2 < P->nTev; i += 3) {
+    float a = 0.f, b = 0.f;
+    HIPCHK(P, hipEventElapsedTime(&a, P->tev[i], P->tev[i + 1]));
+    HIPCHK(P, hipEventElapsedTime(&b, P->tev[i + 1], P->tev[i + 2]));
+    P->ms1 += a;
+    P->ms2 += b;
+    P->nTimed += 1;
+  }
+  P->nTev = 0;
+  return FS_OK;
+}
+
+int launch_steps(fs_pool *P, int K) {
+  Group &G = P->grp;
+  auto one = [&]() { return launch_step(P); };
+  if (P->timing || !P->useGraphs || K < 2) {
+    for (int k = 0; k < K; ++k) {
+      int rc = one();
+      if (rc) return rc;
+    }
+    return P->timing && P->nTev > 3 * 256 ? collect_timing(P) : FS_OK;
+  }
+  if (!G.graph || G.graphK != K) {
+    drop_graph(G);
+    hipGraph_t g = nullptr;
+    HIPCHK(P, hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+    int rc = FS_OK;
+    for (int k = 0; k < K && !rc; ++k) rc = one();
+    hipError_t e = hipStreamEndCapture(P->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(P, FS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&G.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { G.graph = nullptr; return fail(P, FS_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+    G.graphK = K;
+  }
+  HIPCHK(P, hipGraphLaunch(G.graph, P->stream));
+  return FS_OK;
+}
+
+// start of a launch group: targets, then a fresh dt (dirty instances) or a re-clamped one
+int begin_advance(fs_pool *P, const std::vector<int> &slots, const std::vector<double> &targets) {
+  int rc = build_group(P, slots);
+  if (rc) return rc;
+  Group &G = P->grp;
+  const int ni = (int)slots.size();
+  if (ni > P->capTarget) {
+    if (P->dtarget) HIPCHK(P, hipFree(P->dtarget));
+    P->capTarget = std::max(ni, 2 * P->capTarget);
+    HIPCHK(P, hipMalloc(&P->dtarget, sizeof(double) * P->capTarget));
+  }
+  HIPCHK(P, hipMemcpyAsync(P->dtarget, targets.data(), sizeof(double) * ni, hipMemcpyHostToDevice, P->stream));
+  sw_set_targets<<<(ni + 63) / 64, 64, 0, P->stream>>>(G.dinst, ni, P->dtarget);
+  bool dirty = false;
+  for (int s : slots) dirty = dirty || P->slots[s].hrec.dirty;
+  if (dirty) sw_speed<<<(G.nwork + 3) / 4, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2);
+  sw_dt_final<<<ni, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, 1);
+  HIPCHK(P, hipGetLastError());
+  for (int s : slots) P->slots[s].hrec.dirty = 0;
+  return FS_OK;
+}
+
+void fill_clock(const Slot &S, fs_clock *c) {
+  c->t = S.hrec.t;
+  c->lastDt = S.hrec.lastDt;
+  c->dtAvg = S.dtSteps ? S.dtSum / (double)S.dtSteps : 0.0;
+  c->steps = S.hrec.steps;
+  c->cellUpdates = (int64_t)S.hrec.cellUpdates;
+}
+
+int valid_slot(fs_pool *P, int slot) {
+  if (!P) return FS_E_ARG;
+  if (slot < 0 || slot >= (int)P->slots.size() || !P->slots[slot].used) return fail(P, FS_E_ARG, "bad slot %d", slot);
+  return FS_OK;
+}
+
+int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n, int K, bool once) {
+  if (n <= 0) return FS_OK;
+  if (K < 1) K = 1;
+  std::vector<int> sl(slots, slots + n);
+  std::vector<double> tg(targets, targets + n);
+  for (int i = 0; i < n; ++i) {
+    int rc = valid_slot(P, sl[i]);
+    if (rc) return rc;
+  }
+  // sorted unique slot order keeps the group (and its graph) stable across calls
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return sl[a] < sl[b]; });
+  std::vector<int> gs;
+  std::vector<double> gt;
+  for (int i : order) {
+    if (!gs.empty() && gs.back() == sl[i]) return fail(P, FS_E_ARG, "slot %d listed twice", sl[i]);
+    gs.push_back(sl[i]);
+    gt.push_back(tg[i]);
+  }
+  int rc = begin_advance(P, gs, gt);
+  if (rc) return rc;
+  std::vector<double> tprev(gs.size());
+  for (size_t i = 0; i < gs.size(); ++i) tprev[i] = P->slots[gs[i]].hrec.t;
+  int stalls = 0;
+  for (;;) {
+    rc = launch_steps(P, K);
+    if (rc) return rc;
+    rc = download_recs(P, gs);
+    if (rc) return rc;
+    bool done = true, progress = false;
+    for (size_t i = 0; i < gs.size(); ++i) {
+      Slot &S = P->slots[gs[i]];
+      if (S.hrec.t > tprev[i]) {
+        progress = true;
+        const int64_t ds = S.hrec.steps - S.dtSteps;
+        if (ds > 0) { S.dtSum += S.hrec.t - tprev[i]; S.dtSteps = S.hrec.steps; }
+      }
+      tprev[i] = S.hrec.t;
+      if (S.hrec.t < gt[i]) done = false;
+    }
+    if (done || once) break;
+    if (!progress && ++stalls > 3) return fail(P, FS_E_STALL, "GPU solver made no progress (device error?)");
+  }
+  return FS_OK;
+}
+
+// per-cell host preparation shared by create/add
+int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *manning, double manningScalar,
+             const uint8_t *mask, const double *infil, const double *rainField, const int32_t *inlets, int nInlets,
+             int32_t *slotOut) {
+  int rc = check_params(P, p);
+  if (rc) return rc;
+  if (!z) return fail(P, FS_E_ARG, "z is NULL");
+  if (nInlets < 0 || (nInlets > 0 && !inlets)) return fail(P, FS_E_ARG, "bad inlets");
+  const int nx = p->nx, ny = p->ny;
+  const size_t N = (size_t)nx * ny;
+  for (int m = 0; m < nInlets; ++m)
+    if (inlets[m] < 0 || (size_t)inlets[m] >= N) return fail(P, FS_E_ARG, "inlet %d out of the grid", m);
+  const size_t need = slot_bytes(p, infil != nullptr, rainField != nullptr, nInlets);
+  if (P->budget && P->used + need > P->budget)
+    return fail(P, FS_E_BUDGET, "HBM budget: %zu bytes in use, %zu needed, budget %zu", P->used, need, P->budget);
+  int slot = -1;
+  for (size_t s = 0; s < P->slots.size(); ++s)
+    if (!P->slots[s].used) { slot = (int)s; break; }
+  if (slot < 0) { slot = (int)P->slots.size(); P->slots.emplace_back(); }
+  rc = ensure_rec_capacity(P, (int)P->slots.size());
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  S = Slot();
+  S.p = *p;
+  S.nx = nx; S.ny = ny; S.pitch = nx;
+  S.rows = p->rows > 0 ? p->rows : 64;
+  S.bytes = need;
+  hipError_t e = hipMalloc(&S.mem, need);
+  if (e != hipSuccess) return fail(P, FS_E_HIP, "hipMalloc(%zu): %s", need, hipGetErrorString(e));
+  char *q = S.mem;
+  auto take = [&](size_t b) { char *r = q; q += align256(b); return r; };
+  S.h = (float *)take(N * 4); S.hu = (float *)take(N * 4); S.hv = (float *)take(N * 4); S.w = (float *)take(N * 4);
+  S.h1 = (float *)take(N * 4); S.hu1 = (float *)take(N * 4); S.hv1 = (float *)take(N * 4); S.w1 = (float *)take(N * 4);
+  S.z = (float *)take(N * 4); S.n = (float *)take(N * 4); S.hmax = (float *)take(N * 4);
+  S.mask = (uint8_t *)take(N);
+  if (infil) S.infil = (float *)take(N * 4);
+  if (rainField) S.rf = (float *)take(N * 4);
+  S.nin = nInlets;
+  S.inl = (Inlet *)take((size_t)std::max(nInlets, 1) * sizeof(Inlet));
+  S.ispeed = (float *)take((size_t)std::max(nInlets, 1) * 4);
+  S.used = true;
+  P->used += need;
+
+  double zmin = INFINITY;
+  for (size_t k = 0; k < N; ++k) zmin = std::min(zmin, z[k]);
+  S.zref = zmin;
+  std::vector<float> tmp(N);
+  for (size_t k = 0; k < N; ++k) tmp[k] = (float)(z[k] - zmin);
+  HIPCHK(P, hipMemcpy(S.z, tmp.data(), N * 4, hipMemcpyHostToDevice));
+  for (size_t k = 0; k < N; ++k) tmp[k] = (float)(manning ? manning[k] : manningScalar);
+  HIPCHK(P, hipMemcpy(S.n, tmp.data(), N * 4, hipMemcpyHostToDevice));
+  S.hmask.assign(N, 0);
+  int open = 0;
+  for (size_t k = 0; k < N; ++k) {
+    const bool o = mask ? mask[k] != 0 : true;
+    S.hmask[k] = o ? MASK_OPEN : 0;
+    open += o;
+  }
+  S.inletCells.assign(inlets, inlets + nInlets);
+  for (int m = 0; m < nInlets; ++m) S.hmask[inlets[m]] |= MASK_INLET;
+  S.openCells = open;
+  HIPCHK(P, hipMemcpy(S.mask, S.hmask.data(), N, hipMemcpyHostToDevice));
+  if (infil) {
+    for (size_t k = 0; k < N; ++k) tmp[k] = (float)infil[k];
+    HIPCHK(P, hipMemcpy(S.infil, tmp.data(), N * 4, hipMemcpyHostToDevice));
+  }
+  double rainArea = 0.0;
+  if (rainField) {
+    S.hrf.resize(N);
+    for (size_t k = 0; k < N; ++k) { S.hrf[k] = (float)rainField[k]; if (S.hmask[k] & MASK_OPEN) rainArea += S.hrf[k]; }
+    HIPCHK(P, hipMemcpy(S.rf, S.hrf.data(), N * 4, hipMemcpyHostToDevice));
+  } else rainArea = open;
+  // zero state (A and B, contiguous) and max depth; the static layers sit between them
+  HIPCHK(P, hipMemset(S.h, 0, (size_t)((char *)S.z - (char *)S.h)));
+  HIPCHK(P, hipMemset(S.hmax, 0, N * 4));
+  HIPCHK(P, hipDeviceSynchronize());
+  std::vector<Inlet> hi(std::max(nInlets, 1));
+  for (int m = 0; m < nInlets; ++m) {
+    const int k = inlets[m], i = k % nx, j = k / nx;
+    hi[m].cell = j * S.pitch + i;
+    hi[m].q = 0.f;
+    hi[m].vol = 0.0;
+  }
+  HIPCHK(P, hipMemcpy(S.inl, hi.data(), sizeof(Inlet) * hi.size(), hipMemcpyHostToDevice));
+  HIPCHK(P, hipMemset(S.ispeed, 0, (size_t)std::max(nInlets, 1) * 4));
+  StepRec &r = S.hrec;
+  r = StepRec{};
+  r.rainArea = rainArea * p->dx * p->dx;
+  r.rainCells = rainField ? -1 : open;
+  r.openCells = open;
+  r.levels = p->lts;
+  r.dirty = 1;
+  rc = upload_rec(P, slot);
+  if (rc) return rc;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  P->grp.slots.clear();  // force a rebuild
+  *slotOut = slot;
+  return FS_OK;
+}
+
+int set_state(fs_pool *P, int slot, const double *h, const double *hu, const double *hv, int bookInitial) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  const size_t N = (size_t)S.nx * S.ny;
+  std::vector<float> th(N), tu(N), tv(N);
+  double vol = 0.0;
+  for (size_t k = 0; k < N; ++k) {
+    const bool open = S.hmask[k] & MASK_OPEN;
+    const double d = h ? h[k] : 0.0;
+    th[k] = open ? (float)std::max(0.0, d) : 0.f;  // setDepth: mask ? max(0, fn) : 0
+    tu[k] = hu ? (float)hu[k] : 0.f;
+    tv[k] = hv ? (float)hv[k] : 0.f;
+    if (open) vol += (double)th[k];
+  }
+  HIPCHK(P, hipMemcpyAsync(S.h, th.data(), N * 4, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(P, hipMemcpyAsync(S.hu, tu.data(), N * 4, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(P, hipMemcpyAsync(S.hv, tv.data(), N * 4, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(P, hipMemsetAsync(S.w, 0, N * 4, P->stream));
+  rc = download_recs(P, {slot});
+  if (rc) return rc;
+  if (bookInitial) {
+    S.hrec.rain = S.hrec.infil = S.hrec.outflow = S.hrec.clipped = S.hrec.exchanged = 0.0;
+    S.volInitial = vol * S.p.dx * S.p.dx;
+  }
+  S.hrec.dirty = 1;
+  rc = upload_rec(P, slot);
+  if (rc) return rc;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  return FS_OK;
+}
+
+}  // namespace
+
+// plain copy of n float4 (16 B) elements: the achievable-HBM reference and the byte-counter
+// calibration for the profiles
+__global__ __launch_bounds__(256) void fs_copy_kernel(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+// ============================================================================ C ABI
+extern "C" {
+
+const char *fs_version(void) { return "floodsim_hip 0.1 gfx950"; }
+
+int fs_copy_probe(size_t bytes, int32_t iters, double *gbps, double *ms) {
+  if (bytes < 1024 || iters < 1) return fail(nullptr, FS_E_ARG, "bad copy probe size");
+  const size_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  HIPCHK(nullptr, hipMalloc(&a, n * 16));
+  HIPCHK(nullptr, hipMalloc(&b, n * 16));
+  HIPCHK(nullptr, hipMemset(a, 1, n * 16));
+  HIPCHK(nullptr, hipEventCreate(&e0));
+  HIPCHK(nullptr, hipEventCreate(&e1));
+  const int blocks = 256 * 8 * 4;
+  fs_copy_kernel<<<blocks, 256>>>(a, b, n);
+  HIPCHK(nullptr, hipEventRecord(e0, 0));
+  for (int i = 0; i < iters; ++i) fs_copy_kernel<<<blocks, 256>>>(a, b, n);
+  HIPCHK(nullptr, hipEventRecord(e1, 0));
+  HIPCHK(nullptr, hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(nullptr, hipEventElapsedTime(&t, e0, e1));
+  const double per = t / iters;
+  if (ms) *ms = per;
+  if (gbps) *gbps = 2.0 * (double)(n * 16) / (per * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return FS_OK;
+}
+
+int fs_pool_create(size_t hbmBudgetBytes, fs_pool_t *out) {
+  if (!out) return fail(nullptr, FS_E_ARG, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1)
+    return fail(nullptr, FS_E_HIP, "no HIP device: %s", e != hipSuccess ? hipGetErrorString(e) : "count 0");
+  fs_pool *P = new fs_pool();
+  P->budget = hbmBudgetBytes;
+  // FS_NO_GRAPHS=1: plain launches instead of captured K-step graphs (profilers that trace
+  // kernels do not follow graph launches here)
+  const char *ng = getenv("FS_NO_GRAPHS");
+  if (ng && ng[0] && ng[0] != '0') P->useGraphs = false;
+  e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete P; return fail(nullptr, FS_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  for (auto &ev : P->ev) {
+    e = hipEventCreate(&ev);
+    if (e != hipSuccess) { delete P; return fail(nullptr, FS_E_HIP, "hipEventCreate: %s", hipGetErrorString(e)); }
+  }
+  *out = P;
+  return FS_OK;
+}
+
+void fs_pool_destroy(fs_pool_t P) {
+  if (!P) return;
+  (void)hipStreamSynchronize(P->stream);
+  drop_graph(P->grp);
+  for (auto &S : P->slots)
+    if (S.used && S.mem) (void)hipFree(S.mem);
+  if (P->grp.dinst) (void)hipFree(P->grp.dinst);
+  if (P->grp.dwork) (void)hipFree(P->grp.dwork);
+  if (P->grp.p1) (void)hipFree(P->grp.p1);
+  if (P->grp.p2) (void)hipFree(P->grp.p2);
+  if (P->drec) (void)hipFree(P->drec);
+  if (P->dtarget) (void)hipFree(P->dtarget);
+  for (auto &ev : P->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : P->tev) (void)hipEventDestroy(ev);
+  if (P->stream) (void)hipStreamDestroy(P->stream);
+  delete P;
+}
+
+const char *fs_pool_last_error(fs_pool_t P) { return P ? P->err.c_str() : g_create_error.c_str(); }
+
+size_t fs_pool_bytes_needed(const fs_params *p, int32_t withInfil, int32_t withRainField) {
+  return p ? slot_bytes(p, withInfil != 0, withRainField != 0, 0) : 0;
+}
+
+size_t fs_pool_bytes_used(fs_pool_t P) { return P ? P->used : 0; }
+
+int fs_pool_add(fs_pool_t P, const fs_params *p, const double *z, const double *manning, double manningScalar,
+                const uint8_t *mask, const double *infil, const double *rainField, const int32_t *inlets,
+                int32_t nInlets, int32_t *slot) {
+  if (!P || !slot) return fail(P, FS_E_ARG, "pool or slot is NULL");
+  return add_slot(P, p, z, manning, manningScalar, mask, infil, rainField, inlets, nInlets, slot);
+}
+
+int fs_pool_remove(fs_pool_t P, int32_t slot) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  HIPCHK(P, hipFree(S.mem));
+  P->used -= S.bytes;
+  S = Slot();
+  P->grp.slots.clear();
+  drop_graph(P->grp);
+  return FS_OK;
+}
+
+int fs_pool_set_state(fs_pool_t P, int32_t slot, const double *h, const double *hu, const double *hv,
+                      int32_t bookInitial) {
+  return set_state(P, slot, h, hu, hv, bookInitial);
+}
+
+int fs_pool_set_rain(fs_pool_t P, int32_t slot, double rate) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  const float r = (float)rate;
+  HIPCHK(P, hipMemcpyAsync(&P->drec[slot].rainRate, &r, sizeof r, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  S.hrec.rainRate = r;
+  return FS_OK;
+}
+
+int fs_pool_set_inlet_rates(fs_pool_t P, int32_t slot, const double *q, int32_t n) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (n != S.nin || (n && !q)) return fail(P, FS_E_ARG, "expected %d inlet rates, got %d", S.nin, n);
+  if (!n) return FS_OK;
+  std::vector<Inlet> hi(n);
+  HIPCHK(P, hipMemcpyAsync(hi.data(), S.inl, sizeof(Inlet) * n, hipMemcpyDeviceToHost, P->stream));
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  for (int m = 0; m < n; ++m) hi[m].q = (float)q[m];
+  HIPCHK(P, hipMemcpyAsync(S.inl, hi.data(), sizeof(Inlet) * n, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  return FS_OK;
+}
+
+static int read_inlets(fs_pool *P, int slot, double *vol, double *h, int n) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (n != S.nin) return fail(P, FS_E_ARG, "expected %d inlets, got %d", S.nin, n);
+  if (!n) return FS_OK;
+  std::vector<Inlet> hi(n);
+  HIPCHK(P, hipMemcpyAsync(hi.data(), S.inl, sizeof(Inlet) * n, hipMemcpyDeviceToHost, P->stream));
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  for (int m = 0; m < n; ++m) {
+    if (vol) vol[m] = hi[m].vol;
+    S.exchangedRead += hi[m].vol;
+    hi[m].vol = 0.0;
+    if (h) {
+      float v = 0.f;
+      HIPCHK(P, hipMemcpy(&v, S.h + hi[m].cell, 4, hipMemcpyDeviceToHost));
+      h[m] = v;
+    }
+  }
+  HIPCHK(P, hipMemcpyAsync(S.inl, hi.data(), sizeof(Inlet) * n, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  return FS_OK;
+}
+
+int fs_pool_advance(fs_pool_t P, const int32_t *slots, const double *tTargets, int32_t n, int32_t K,
+                    fs_clock *clks) {
+  if (!P || (n > 0 && (!slots || !tTargets))) return fail(P, FS_E_ARG, "bad arguments");
+  int rc = advance_group(P, slots, tTargets, n, K, false);
+  if (rc) return rc;
+  if (clks)
+    for (int i = 0; i < n; ++i) fill_clock(P->slots[slots[i]], &clks[i]);
+  return FS_OK;
+}
+
+int fs_pool_step(fs_pool_t P, const int32_t *slots, const double *tTargets, int32_t n, int32_t K) {
+  if (!P || (n > 0 && (!slots || !tTargets))) return fail(P, FS_E_ARG, "bad arguments");
+  return advance_group(P, slots, tTargets, n, K, true);
+}
+
+int fs_pool_clock(fs_pool_t P, int32_t slot, fs_clock *clk) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  rc = download_recs(P, {slot});
+  if (rc) return rc;
+  if (clk) fill_clock(P->slots[slot], clk);
+  return FS_OK;
+}
+
+int fs_pool_read_state(fs_pool_t P, int32_t slot, double *h, double *hu, double *hv, double *hmax, fs_mass *mass) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  const size_t N = (size_t)S.nx * S.ny;
+  std::vector<float> th(N), tw(N), tmp(N);
+  HIPCHK(P, hipMemcpyAsync(th.data(), S.h, N * 4, hipMemcpyDeviceToHost, P->stream));
+  HIPCHK(P, hipMemcpyAsync(tw.data(), S.w, N * 4, hipMemcpyDeviceToHost, P->stream));
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  if (h) for (size_t k = 0; k < N; ++k) h[k] = th[k];
+  auto fetch = [&](const float *src, double *dst) -> int {
+    if (!dst) return FS_OK;
+    HIPCHK(P, hipMemcpy(tmp.data(), src, N * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < N; ++k) dst[k] = tmp[k];
+    return FS_OK;
+  };
+  if ((rc = fetch(S.hu, hu))) return rc;
+  if ((rc = fetch(S.hv, hv))) return rc;
+  if ((rc = fetch(S.hmax, hmax))) return rc;
+  rc = download_recs(P, {slot});
+  if (rc) return rc;
+  if (mass) {
+    double vol = 0.0;
+    for (size_t k = 0; k < N; ++k)
+      if (S.hmask[k] & MASK_OPEN) vol += (double)th[k] - (double)tw[k];  // depth less its residual
+    const double A = S.p.dx * S.p.dx;
+    double pending = 0.0;
+    if (S.nin) {
+      std::vector<Inlet> hi(S.nin);
+      HIPCHK(P, hipMemcpy(hi.data(), S.inl, sizeof(Inlet) * S.nin, hipMemcpyDeviceToHost));
+      for (auto &q : hi) pending += q.vol;
+    }
+    const StepRec &r = S.hrec;
+    fs_mass &m = *mass;
+    m.volume = vol * A;
+    m.rain = r.rain;
+    m.infil = r.infil;
+    m.outflow = r.outflow;
+    m.exchanged = S.exchangedRead + pending;
+    m.clipped = r.clipped;
+    m.initial = S.volInitial;
+    m.expected = S.volInitial + r.rain - r.infil - r.outflow - m.exchanged;
+    // scale as readState (solver-gpu.js:685)
+    const double scale = std::max(1e-30, std::max(std::fabs(m.expected),
+                                                   S.volInitial + r.rain + std::max(0.0, -m.exchanged) +
+                                                       std::max(0.0, -r.outflow)));
+    m.rel = (m.volume - m.expected) / scale;
+    m.relExClip = (m.volume - m.expected - m.clipped) / scale;
+  }
+  return FS_OK;
+}
+
+int fs_pool_set_timing(fs_pool_t P, int32_t on) {
+  if (!P) return FS_E_ARG;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  P->nTev = 0;
+  P->timing = on != 0;
+  P->ms1 = P->ms2 = 0.0;
+  P->nTimed = 0;
+  return FS_OK;
+}
+
+int fs_pool_kernel_ms(fs_pool_t P, double *s1, double *s2, int64_t *launches) {
+  if (!P) return FS_E_ARG;
+  int rc = collect_timing(P);
+  if (rc) return rc;
+  if (s1) *s1 = P->nTimed ? P->ms1 / P->nTimed : -1.0;
+  if (s2) *s2 = P->nTimed ? P->ms2 / P->nTimed : -1.0;
+  if (launches) *launches = P->nTimed;
+  P->ms1 = P->ms2 = 0.0;
+  P->nTimed = 0;
+  return FS_OK;
+}
+
+// ---------------------------------------------------------------- single solver
+int fs_create(const fs_params *p, const double *z, const double *manning, double manningScalar, const uint8_t *mask,
+              const double *infil, const double *rainField, const int32_t *inlets, int32_t nInlets, fs_handle *out) {
+  if (!out) return fail(nullptr, FS_E_ARG, "out is NULL");
+  *out = nullptr;
+  fs_pool *P = nullptr;
+  int rc = fs_pool_create(0, &P);
+  if (rc) return rc;
+  int32_t slot = -1;
+  rc = add_slot(P, p, z, manning, manningScalar, mask, infil, rainField, inlets, nInlets, &slot);
+  if (rc) {
+    g_create_error = P->err;
+    fs_pool_destroy(P);
+    return rc;
+  }
+  fs_solver *s = new fs_solver();
+  s->pool = P;
+  s->slot = slot;
+  *out = s;
+  return FS_OK;
+}
+
+void fs_destroy(fs_handle s) {
+  if (!s) return;
+  fs_pool_destroy(s->pool);
+  delete s;
+}
+
+const char *fs_last_error(fs_handle s) { return s ? s->pool->err.c_str() : g_create_error.c_str(); }
+
+fs_pool_t fs_solver_pool(fs_handle s) { return s ? s->pool : nullptr; }
+
+int fs_set_state(fs_handle s, const double *h, const double *hu, const double *hv, int32_t bookInitial) {
+  if (!s) return FS_E_ARG;
+  return set_state(s->pool, s->slot, h, hu, hv, bookInitial);
+}
+
+int fs_set_rain(fs_handle s, double rate) { return s ? fs_pool_set_rain(s->pool, s->slot, rate) : FS_E_ARG; }
+
+int fs_set_rain_field(fs_handle s, const double *field) {
+  if (!s || !field) return FS_E_ARG;
+  fs_pool *P = s->pool;
+  Slot &S = P->slots[s->slot];
+  if (!S.rf) return fail(P, FS_E_STATE, "setRainField needs a solver created with a rainField");
+  const size_t N = (size_t)S.nx * S.ny;
+  double area = 0.0;
+  for (size_t k = 0; k < N; ++k) { S.hrf[k] = (float)field[k]; if (S.hmask[k] & MASK_OPEN) area += S.hrf[k]; }
+  HIPCHK(P, hipMemcpy(S.rf, S.hrf.data(), N * 4, hipMemcpyHostToDevice));
+  int rc = download_recs(P, {s->slot});
+  if (rc) return rc;
+  S.hrec.rainArea = area * S.p.dx * S.p.dx;
+  return upload_rec(P, s->slot) || hipStreamSynchronize(P->stream);
+}
+
+int fs_set_inlet_rates(fs_handle s, const double *q, int32_t n) {
+  return s ? fs_pool_set_inlet_rates(s->pool, s->slot, q, n) : FS_E_ARG;
+}
+
+int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n) {
+  return s ? read_inlets(s->pool, s->slot, vol, h, n) : FS_E_ARG;
+}
+
+int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk) {
+  if (!s) return FS_E_ARG;
+  int32_t sl = s->slot;
+  return fs_pool_advance(s->pool, &sl, &tTarget, 1, K, clk);
+}
+
+int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk) {
+  if (!s) return FS_E_ARG;
+  fs_pool *P = s->pool;
+  int32_t sl = s->slot;
+  int rc = advance_group(P, &sl, &tEnd, 1, K, true);
+  if (rc) return rc;
+  Slot &S = P->slots[sl];
+  if (out) {
+    const size_t N = (size_t)S.nx * S.ny;
+    std::vector<float> th(N), tu(N), tv(N);
+    HIPCHK(P, hipMemcpy(th.data(), S.h, N * 4, hipMemcpyDeviceToHost));
+    HIPCHK(P, hipMemcpy(tu.data(), S.hu, N * 4, hipMemcpyDeviceToHost));
+    HIPCHK(P, hipMemcpy(tv.data(), S.hv, N * 4, hipMemcpyDeviceToHost));
+    // pack (kernels.wgsl.js:1348-1355): depth and speed per interior cell
+    for (size_t k = 0; k < N; ++k) {
+      const float d = th[k];
+      out[2 * k] = d;
+      out[2 * k + 1] = d > (float)S.p.hdry ? std::sqrt(tu[k] * tu[k] + tv[k] * tv[k]) / d : 0.f;
+    }
+  }
+  if (clk) fill_clock(S, clk);
+  return FS_OK;
+}
+
+int fs_read_state(fs_handle s, double *h, double *hu, double *hv, double *hmax, fs_mass *mass) {
+  return s ? fs_pool_read_state(s->pool, s->slot, h, hu, hv, hmax, mass) : FS_E_ARG;
+}
+
+int fs_clock_get(fs_handle s, fs_clock *clk) { return s ? fs_pool_clock(s->pool, s->slot, clk) : FS_E_ARG; }
+
+int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total) {
+  if (!s) return FS_E_ARG;
+  Slot &S = s->pool->slots[s->slot];
+  if (active) *active = S.openCells;
+  if (total) *total = (int64_t)S.nx * S.ny;
+  return FS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Device-side data of the shallow-water step: instance descriptors, step records, partials.
+// Shared by the kernels (sw_kernels.hip) and the host runtime (floodsim.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fs {
+
+constexpr int kLanes = 64;                   // wavefront width on CDNA
+constexpr int kHalo = 3;                     // theta limiter + MUSCL need 3 cells each side
+constexpr int kOwn = kLanes - 2 * kHalo;     // columns a wavefront strip owns (58)
+constexpr int kLtsTile = 8;                  // LTS level tiles (solver-cpu.js:14)
+
+enum { BC_WALL = 0, BC_OPEN = 1, BC_OUTFLOW = 2 };
+enum { MASK_OPEN = 1, MASK_INLET = 2 };      // mask byte bits: open (not a wall) cell, inlet cell
+enum { FLAG_INFIL = 1, FLAG_RF = 2 };
+
+// Per-instance mutable record: clock, dt, FP64 books. Written by dt_final only (and the host
+// between launches); the stage kernels read dt and the rain rate from it.
+struct StepRec {
+  double t;        // time at the start of the pending step
+  double tAfter;   // time once the pending step is done (exactly tEnd when clamped)
+  double tEnd;     // target of the current advance
+  double rain;     // FP64 books, m^3
+  double infil;
+  double outflow;
+  double clipped;
+  double exchanged;
+  double rainArea; // m^2: sum of the rain multiplier over open cells times dx^2
+  double cellUpdates;
+  float dt;        // pending step (0 = no-op)
+  float smax;      // last max wave speed
+  float lastDt;
+  float rainRate;  // m/s
+  int64_t steps;
+  int32_t rainCells;  // open cells with a uniform multiplier of 1 (no rain field), else -1
+  int32_t openCells;
+  int32_t levels;     // LTS levels (0 = global)
+  int32_t dirty;      // the host changed the state: the next dt comes from a fresh speed reduction
+};
+
+// one inlet: cell index (y*pitch + x), rate (m^3/s, + into the network, - a point inflow)
+struct Inlet {
+  int cell;
+  float q;
+  double vol;   // exchanged since the last read, m^3
+};
+
+struct DevInst {
+  float *h, *hu, *hv, *w;       // A: U^n, and U^{n+1} after the second stage (in place)
+  float *h1, *hu1, *hv1, *w1;   // B: U* (first-stage result)
+  const float *z;               // bed, m, relative to the instance minimum
+  const float *n;               // Manning n
+  const uint8_t *mask;          // MASK_* bits
+  float *hmax;
+  const float *infil;           // m/s, or null
+  const float *rf;              // rain multiplier, or null
+  StepRec *rec;
+  int nx, ny, pitch, flags;
+  float dx, inv_dx, g, hdry, hfront, cfl, dtDry, area;
+  int secondOrder, limiter;
+  int bc[4];
+  int wbeg, wend;               // this instance's range in the work list
+  Inlet *inl;            // this instance's inlets
+  float *ispeed;                // wave speed of each inlet cell after the exchange
+  int nin, pad;
+};
+
+// per wavefront strip and stage: boundary outflow (m^3), clipped (m^3), infiltration (m^3), max speed
+struct Part {
+  double out, clip, infil;
+  float smax, pad;
+};
+
+
+}  // namespace fs

@@ -1,0 +1,576 @@
+// HIP kernels of the shallow-water step for gfx950 (CDNA4).
+//
+// One RK stage is one kernel. Each 64-lane wavefront owns a strip of 58 columns (lanes 3..60;
+// 3 halo lanes each side) and walks its rows top to bottom with a register window:
+//   row r loaded -> x reconstruction and x faces of row r (x neighbours by DPP lane shifts)
+//                -> y reconstruction of row r-1, y face (r-2 | r-1)
+//                -> draining-time limiter theta of row r-2
+//                -> conservative update (and, in the second stage, sources) of row r-3
+// so every face, theta and update is computed from values held in registers: nothing but
+// the state, the static layers and the outputs crosses HBM, and no LDS or barrier is used.
+//
+// Physics follow the reference GPU kernels (src/kernels.wgsl.js) and CPU solver
+// (src/solver-cpu.js) cited per function. Elevations enter only as differences of
+// neighbouring FP32 beds (exact by Sterbenz for neighbours within a factor of two) plus
+// depth-relative terms, never as absolute eta = h + z, so thin films on high ground keep
+// their digits.
+#pragma once
+#include "sw_device.h"
+
+namespace fs {
+
+// ---------------------------------------------------------------- lane shifts (DPP)
+__device__ __forceinline__ float from_left(float v) {   // lane l receives lane l-1
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {  // lane l receives lane l+1
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ int from_left_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int from_right_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsq(float x) { return __builtin_amdgcn_sqrtf(x); }
+// x^p for x > 0 through v_log_f32 / v_exp_f32 (base 2)
+__device__ __forceinline__ float fpow(float x, float p) {
+  return __builtin_amdgcn_exp2f(p * __builtin_amdgcn_logf(x));
+}
+
+// slope limiter: minmod or van Leer (solver-cpu.js:17-24; kernels.wgsl.js:83-87)
+__device__ __forceinline__ float limit(float a, float b, int vanleer) {
+  float p = a * b;
+  float mm = fabsf(a) < fabsf(b) ? a : b;
+  float v = vanleer ? 2.f * p * frcp(a + b) : mm;
+  return p <= 0.f ? 0.f : v;
+}
+
+// HLLC in the face-normal frame with dry-bed speed estimates (solver-cpu.js:579-602;
+// kernels.wgsl.js:183-210). Branch-free: every path is evaluated and selected.
+__device__ __forceinline__ void hllc(float hL, float uL, float vL, float hR, float uR, float vR, float g,
+                                     float hdry, float &F0, float &F1, float &F2) {
+  const bool wetL = hL > hdry, wetR = hR > hdry;
+  uL = wetL ? uL : 0.f; vL = wetL ? vL : 0.f;
+  uR = wetR ? uR : 0.f; vR = wetR ? vR : 0.f;
+  const float cL = fsq(g * hL), cR = fsq(g * hR);
+  const float us = 0.5f * (uL + uR) + cL - cR;
+  const float cs = 0.5f * (cL + cR) + 0.25f * (uL - uR);
+  float SL = fminf(uL - cL, us - cs), SR = fmaxf(uR + cR, us + cs);
+  SL = wetR ? SL : uL - cL;
+  SR = wetR ? SR : uL + 2.f * cL;
+  SL = wetL ? SL : uR - 2.f * cR;
+  SR = wetL ? SR : uR + cR;
+  const float qL = hL * uL, qR = hR * uR;
+  const float FL1 = qL * uL + 0.5f * g * hL * hL;
+  const float FR1 = qR * uR + 0.5f * g * hR * hR;
+  const float inv = frcp(SR - SL);
+  const float H0 = (SR * qL - SL * qR + SL * SR * (hR - hL)) * inv;
+  const float H1 = (SR * FL1 - SL * FR1 + SL * SR * (qR - qL)) * inv;
+  // contact speed SM >= 0 <=> numerator and denominator share a sign (no division needed)
+  const float dSM = hR * (uR - SR) - hL * (uL - SL);
+  const float nSM = SL * hR * (uR - SR) - SR * hL * (uL - SL);
+  const bool left = !(fabsf(dSM) > 1e-30f) || (nSM * dSM >= 0.f);
+  const float H2 = H0 * (left ? vL : vR);
+  float f0 = H0, f1 = H1, f2 = H2;
+  if (SL >= 0.f) { f0 = qL; f1 = FL1; f2 = qL * vL; }
+  if (SR <= 0.f && !(SL >= 0.f)) { f0 = qR; f1 = FR1; f2 = qR * vR; }
+  const bool dry = !wetL && !wetR;
+  F0 = dry ? 0.f : f0;
+  F1 = dry ? 0.f : f1;
+  F2 = dry ? 0.f : f2;
+}
+
+// Face with hydrostatic reconstruction (Audusse) and wall mirroring (solver-cpu.js:237-267;
+// kernels.wgsl.js:233-263). L and R are the reconstructed (h, eta - z_cell, un, ut) of the two
+// cells; dz = z_R - z_L of the cell beds. The face beds are z_cell + (e - h); only their
+// difference delta enters: h*_L = max(0, hL - max(0, delta)), h*_R = max(0, hR - max(0, -delta)).
+struct Face { float F0, F1, F2, cL, cR; };
+
+__device__ __forceinline__ Face face(float hL, float eL, float uL, float vL, bool ML, float hR, float eR, float uR,
+                                     float vR, bool MR, float dz, float g, float hdry) {
+  if (!MR) { hR = hL; eR = eL; uR = -uL; vR = vL; }
+  if (!ML) { hL = hR; eL = eR; uL = -uR; vL = vR; }
+  const float delta = (ML && MR) ? (dz + (eR - hR)) - (eL - hL) : 0.f;
+  const float hLs = fmaxf(0.f, hL - fmaxf(0.f, delta));
+  const float hRs = fmaxf(0.f, hR - fmaxf(0.f, -delta));
+  Face f;
+  hllc(hLs, uL, vL, hRs, uR, vR, g, hdry, f.F0, f.F1, f.F2);
+  const float hg = 0.5f * g;
+  f.cL = ML ? hg * (hL * hL - hLs * hLs) : 0.f;
+  f.cR = MR ? hg * (hR * hR - hRs * hRs) : 0.f;
+  if (!ML && !MR) { f.F0 = f.F1 = f.F2 = f.cL = f.cR = 0.f; }
+  return f;
+}
+
+// Kahan-compensated depth: stored h with residual w, true depth h - w (kernels.wgsl.js:265-272)
+__device__ __forceinline__ void kadd(float &h, float &w, float d) {
+  const float y = d - w;
+  const float t = h + y;
+  w = (t - h) - y;
+  h = t;
+}
+
+// free-outfall edge velocity: Manning on the surface slope toward the edge beyond 1e-4, at most
+// critical (solver-cpu.js:179-183; kernels.wgsl.js:92-101). e = edge cell, i = next cell inward.
+__device__ __forceinline__ float out_vel(float he, float ze, float ne, float hi, float zi, bool Mi, float inv_dx,
+                                         float g, float hdry) {
+  if (he <= hdry || hi <= hdry || !Mi) return 0.f;
+  const float s = ((hi - he) + (zi - ze)) * inv_dx - 1e-4f;
+  if (s <= 0.f) return 0.f;
+  const float crit = fsq(g * he);
+  if (ne <= 0.f) return crit;
+  return fminf(crit, fpow(he, 2.f / 3.f) * fsq(s) * frcp(ne));
+}
+
+// Primitive of cell (X, Y) of the evaluated state, ghosts included (solver-cpu.js:165-213;
+// kernels.wgsl.js:122-156): walls mirror across the edge and flip the normal velocity, open
+// and outflow edges copy the edge cell, outflow sets the outward Manning velocity. Ghost beds
+// are the nearest interior bed (zero gradient), ghost cells are open. Corner ghosts never
+// enter an interior cell's stencil and only need to be finite.
+struct Prim { float h, u, v, z, hu, hv; bool M, in; };
+
+__device__ __forceinline__ Prim load_prim(const DevInst &I, const float *__restrict__ H, const float *__restrict__ HU,
+                                          const float *__restrict__ HV, int X, int Y) {
+  const int nx = I.nx, ny = I.ny;
+  int sx = X, sy = Y;
+  float fu = 1.f, fv = 1.f;
+  const bool gx = X < 0 || X >= nx, gy = Y < 0 || Y >= ny;
+  if (X < 0) { sx = I.bc[0] == BC_WALL ? -1 - X : 0; if (I.bc[0] == BC_WALL) fu = -1.f; }
+  else if (X >= nx) { sx = I.bc[1] == BC_WALL ? 2 * nx - 1 - X : nx - 1; if (I.bc[1] == BC_WALL) fu = -1.f; }
+  if (Y < 0) { sy = I.bc[2] == BC_WALL ? -1 - Y : 0; if (I.bc[2] == BC_WALL) fv = -1.f; }
+  else if (Y >= ny) { sy = I.bc[3] == BC_WALL ? 2 * ny - 1 - Y : ny - 1; if (I.bc[3] == BC_WALL) fv = -1.f; }
+  sx = min(max(sx, 0), nx - 1);
+  sy = min(max(sy, 0), ny - 1);
+  const int zx = min(max(X, 0), nx - 1), zy = min(max(Y, 0), ny - 1);
+  const size_t k = (size_t)sy * I.pitch + sx;
+  Prim p;
+  p.h = H[k];
+  p.hu = HU[k] * fu;
+  p.hv = HV[k] * fv;
+  p.z = I.z[(size_t)zy * I.pitch + zx];
+  p.in = !gx && !gy;
+  p.M = p.in ? (I.mask[k] & MASK_OPEN) != 0 : true;
+  const float ih = frcp(p.h);
+  const bool wet = p.h > I.hdry;
+  p.u = wet ? p.hu * ih : 0.f;
+  p.v = wet ? p.hv * ih : 0.f;
+  if (gx != gy) {  // edge ghost (not a corner): free outfall
+    int e = -1, i = -1, side = -1;
+    if (X < 0 && I.bc[0] == BC_OUTFLOW) { e = 0; i = 1; side = 0; }
+    else if (X >= nx && I.bc[1] == BC_OUTFLOW) { e = nx - 1; i = nx - 2; side = 1; }
+    else if (Y < 0 && I.bc[2] == BC_OUTFLOW) { e = 0; i = 1; side = 2; }
+    else if (Y >= ny && I.bc[3] == BC_OUTFLOW) { e = ny - 1; i = ny - 2; side = 3; }
+    if (side >= 0) {
+      i = max(i, 0);
+      size_t ke, ki;
+      if (side < 2) { ke = (size_t)zy * I.pitch + e; ki = (size_t)zy * I.pitch + min(i, nx - 1); }
+      else { ke = (size_t)e * I.pitch + zx; ki = (size_t)min(i, ny - 1) * I.pitch + zx; }
+      const float ov = out_vel(H[ke], I.z[ke], I.n[ke], H[ki], I.z[ki], (I.mask[ki] & MASK_OPEN) != 0, I.inv_dx,
+                               I.g, I.hdry);
+      if (side == 0) p.u = -ov;
+      else if (side == 1) p.u = ov;
+      else if (side == 2) p.v = -ov;
+      else p.v = ov;
+    }
+  }
+  return p;
+}
+
+// MUSCL slopes of (h, eta, un, ut) from the two neighbours along one axis (solver-cpu.js:216-233;
+// kernels.wgsl.js:159-180): first order when the cell or an open neighbour is thinner than hfront;
+// wall neighbours are left out of the slope. dza = z_c - z_a, dzb = z_b - z_c.
+struct Slopes { float sh, se, sn, st; };
+
+__device__ __forceinline__ Slopes muscl(float hc, float nc, float tc, float ha, float na, float ta, bool Ma,
+                                        float dza, float hb, float nb, float tb, bool Mb, float dzb, int secondOrder,
+                                        float hfront, int vanleer) {
+  Slopes s = {0.f, 0.f, 0.f, 0.f};
+  const bool fo = !secondOrder || hc < hfront || (Ma && ha < hfront) || (Mb && hb < hfront);
+  if (fo) return s;
+  const float dhm = Ma ? hc - ha : 0.f, dhp = Mb ? hb - hc : 0.f;
+  s.sh = limit(dhm, dhp, vanleer);
+  s.se = limit(Ma ? dhm + dza : 0.f, Mb ? dhp + dzb : 0.f, vanleer);
+  s.sn = limit(Ma ? nc - na : 0.f, Mb ? nb - nc : 0.f, vanleer);
+  s.st = limit(Ma ? tc - ta : 0.f, Mb ? tb - tc : 0.f, vanleer);
+  return s;
+}
+
+// wavefront reductions (fixed tree: deterministic)
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// wavefront index with blocks of one XCD (blockIdx % 8) mapped to one contiguous run of work,
+// so strips that share halo rows and columns share an L2
+__device__ __forceinline__ int xcd_wave_index(int nwaves_per_block) {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+  const int lb = x * q + min(x, r) + k;
+  return lb * nwaves_per_block + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
+// |u| + |v| + 2 sqrt(g h): the sum of the directional wave speeds (computeDt, solver-cpu.js:383-394)
+__device__ __noinline__ float cell_speed(float h, float hu, float hv, float g) {
+  const float ih = frcp(h);
+  return fabsf(hu * ih) + fabsf(hv * ih) + 2.f * fsq(g * h);
+}
+
+// Friction, rain, infiltration and max depth of one cell after its update, then the cell's wave
+// speed for the next dt (sourcesAt, kernels.wgsl.js:544-590; _sources, solver-cpu.js:456-483).
+// Semi-implicit Manning: (hu, hv) /= 1 + dt g n^2 |u| / h^(4/3).
+__device__ __forceinline__ void sources_cell(const DevInst &I, size_t k, float dt, float rain, float areaf, float &qh,
+                                             float &qw, float &qhu, float &qhv, double &acc_inf, float &smax) {
+  const float g = I.g, hdry = I.hdry;
+  const float d = qh;
+  const float nn = I.n[k];
+  if (d > hdry && nn > 0.f) {
+    const float id = frcp(d);
+    const float u = qhu * id, v = qhv * id;
+    const float af = dt * g * nn * nn * frcp(fpow(d, 4.f / 3.f));
+    const float sp = fsq(u * u + v * v);
+    const float kf = frcp(1.f + af * sp);
+    qhu *= kf;
+    qhv *= kf;
+  }
+  const float rr = (I.flags & FLAG_RF) ? rain * I.rf[k] * dt : rain * dt;
+  float inf = 0.f;
+  if (I.flags & FLAG_INFIL) inf = fminf(I.infil[k] * dt, d + rr);
+  float nh = qh, nw = qw;
+  kadd(nh, nw, rr - inf);
+  if (nh < 0.f) { inf = d - qw + rr; qh = 0.f; qw = 0.f; }
+  else { qh = nh; qw = nw; }
+  acc_inf += (double)inf * areaf;
+  if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
+  const float hm = I.hmax[k];
+  if (qh > hm) I.hmax[k] = qh;
+  if (qh > hdry && !(I.mask[k] & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
+}
+
+struct FaceRow { float F0, F1, F2, cL, cR; };
+
+// ------------------------------------------------------------------ one RK stage
+// STAGE 0: U* = U^n + dt L(U^n)                   (updateAt STAGE 0, kernels.wgsl.js:489-492)
+// STAGE 1: U^{n+1} = (U^n + U* + dt L(U*)) / 2, then friction, rain, infiltration, max depth,
+//          and the wave speed of the result for the next step's dt (kernels.wgsl.js:493-498, 544-590)
+// work[k] = (instance, first owned column, first owned row, rows)
+template <int STAGE>
+__global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
+                                                int nwork, Part *__restrict__ part) {
+  const int wid = xcd_wave_index(4);
+  if (wid >= nwork) return;
+  const int lane = threadIdx.x & 63;
+  const int4 wk = work[wid];
+  const DevInst &I = insts[wk.x];
+  if (STAGE == 1 && !I.secondOrder) return;  // first order: sw_euler_finish writes this strip's partial
+  const float dt = I.rec->dt;
+  if (!(dt > 0.f)) {
+    if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0.f};
+    return;
+  }
+  const float rain = I.rec->rainRate;
+  const float g = I.g, hdry = I.hdry, hfront = I.hfront, inv_dx = I.inv_dx, dx = I.dx;
+  const int so = I.secondOrder, vl = I.limiter;
+  const int nx = I.nx, ny = I.ny, pitch = I.pitch;
+  const int X = wk.y - kHalo + lane;
+  const bool own_col = lane >= kHalo && lane < kLanes - kHalo && X < nx;
+  const int y0 = wk.z, y1 = min(wk.z + wk.w, ny);
+
+  // evaluated state: U^n in stage 0, U* in stage 1
+  const float *__restrict__ Hs = STAGE == 0 ? I.h : I.h1;
+  const float *__restrict__ HUs = STAGE == 0 ? I.hu : I.hu1;
+  const float *__restrict__ HVs = STAGE == 0 ? I.hv : I.hv1;
+  const float *__restrict__ Ws = STAGE == 0 ? I.w : I.w1;
+  float *__restrict__ Ho = STAGE == 0 ? I.h1 : I.h;
+  float *__restrict__ HUo = STAGE == 0 ? I.hu1 : I.hu;
+  float *__restrict__ HVo = STAGE == 0 ? I.hv1 : I.hv;
+  float *__restrict__ Wo = STAGE == 0 ? I.w1 : I.w;
+
+  const float hg = 0.5f * g;
+  const float wgt = so ? 0.5f : 1.f;
+  const float areaf = dx * dx;
+
+  // ---- register window
+  // prims of rows r-1, r-2
+  float h1r = 0, u1r = 0, v1r = 0, z1r = 0, h2r = 0, u2r = 0, v2r = 0, z2r = 0;
+  bool M1r = false, M2r = false;
+  // raw state of rows r-1, r-2, r-3 (for the update of row r-3)
+  float hu1 = 0, hv1 = 0, hu2 = 0, hv2 = 0, hu3 = 0, hv3 = 0, hh3 = 0;
+  bool M3 = false, in1 = false, in2 = false, in3 = false;
+  // y reconstruction of row r-2 toward north (+)
+  float yp_h = 0, yp_e = 0, yp_n = 0, yp_t = 0;
+  // x faces (l | l+1) of rows r-1, r-2, r-3
+  FaceRow xf1 = {0, 0, 0, 0, 0}, xf2 = {0, 0, 0, 0, 0}, xf3 = {0, 0, 0, 0, 0};
+  // y faces (r-3 | r-2) and (r-4 | r-3)
+  FaceRow yf3 = {0, 0, 0, 0, 0}, yf4 = {0, 0, 0, 0, 0};
+  // theta of rows r-3, r-4
+  float th3 = 1.f, th4 = 1.f;
+  // bed-slope source terms: x for rows r-1, r-2, r-3; y for rows r-2, r-3
+  float sx1 = 0, sx2 = 0, sx3 = 0, sy2 = 0, sy3 = 0;
+  bool M1 = false, M2 = false;
+  (void)M1; (void)M2;
+
+  double acc_out = 0.0, acc_clip = 0.0, acc_inf = 0.0;
+  float smax = 0.f;
+
+  for (int r = y0 - kHalo; r < y1 + kHalo; ++r) {
+    // ---------------- row r: primitives
+    const Prim p = load_prim(I, Hs, HUs, HVs, X, r);
+
+    // ---------------- x reconstruction of row r (neighbours by lane shifts)
+    const float hl = from_left(p.h), ul = from_left(p.u), vlf = from_left(p.v), zl = from_left(p.z);
+    const float hr = from_right(p.h), ur = from_right(p.u), vr = from_right(p.v), zr = from_right(p.z);
+    const bool Ml = from_left_i(p.M) != 0, Mr = from_right_i(p.M) != 0;
+    const Slopes sxs = muscl(p.h, p.u, p.v, hl, ul, vlf, Ml, p.z - zl, hr, ur, vr, Mr, zr - p.z, so, hfront, vl);
+    const float xh_p = p.h + 0.5f * sxs.sh, xe_p = p.h + 0.5f * sxs.se;
+    const float xh_m = p.h - 0.5f * sxs.sh, xe_m = p.h - 0.5f * sxs.se;
+    const float xu_p = p.u + 0.5f * sxs.sn, xv_p = p.v + 0.5f * sxs.st;
+    const float xu_m = p.u - 0.5f * sxs.sn, xv_m = p.v - 0.5f * sxs.st;
+    const float sx0 = hg * (xh_m + xh_p) * ((xe_p - xh_p) - (xe_m - xh_m));
+    // x face (l | l+1): right side is lane l+1's minus side
+    FaceRow xf0;
+    {
+      const float Rh = from_right(xh_m), Re = from_right(xe_m), Ru = from_right(xu_m), Rv = from_right(xv_m);
+      const Face f = face(xh_p, xe_p, xu_p, xv_p, p.M, Rh, Re, Ru, Rv, Mr, zr - p.z, g, hdry);
+      xf0 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR};
+    }
+
+    // ---------------- y reconstruction of row r-1 (neighbours: rows r-2 and r in registers)
+    const Slopes sys = muscl(h1r, v1r, u1r, h2r, v2r, u2r, M2r, z1r - z2r, p.h, p.v, p.u, p.M, p.z - z1r, so, hfront, vl);
+    const float yh_p = h1r + 0.5f * sys.sh, ye_p = h1r + 0.5f * sys.se;
+    const float yh_m = h1r - 0.5f * sys.sh, ye_m = h1r - 0.5f * sys.se;
+    const float yn_p = v1r + 0.5f * sys.sn, yt_p = u1r + 0.5f * sys.st;
+    const float yn_m = v1r - 0.5f * sys.sn, yt_m = u1r - 0.5f * sys.st;
+    const float sy1 = hg * (yh_m + yh_p) * ((ye_p - yh_p) - (ye_m - yh_m));
+    // y face (r-2 | r-1): L = row r-2 toward north, R = row r-1 toward south
+    FaceRow yf2;
+    {
+      const Face f = face(yp_h, yp_e, yp_n, yp_t, M2r, yh_m, ye_m, yn_m, yt_m, M1r, z1r - z2r, g, hdry);
+      yf2 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR};
+    }
+
+    // ---------------- theta of row r-2 (solver-cpu.js:307-318; kernels.wgsl.js:349-363)
+    float th2 = 1.f;
+    {
+      const float fe = xf2.F0, fw = from_left(xf2.F0), fs = yf3.F0, fn = yf2.F0;
+      const float out = ((fe > 0.f ? fe : 0.f) - (fw < 0.f ? fw : 0.f) + (fn > 0.f ? fn : 0.f) - (fs < 0.f ? fs : 0.f)) * dx * dt;
+      const float vol = h2r * areaf;
+      if (in2 && M2r && out > vol) th2 = vol / out;
+    }
+
+    // ---------------- update of row r-3
+    {
+      const int ry = r - kHalo;
+      // west face (l-1 | l) from lane l-1; east face (l | l+1) own
+      const float Fw0 = from_left(xf3.F0), Fw1 = from_left(xf3.F1), Fw2 = from_left(xf3.F2), cRw = from_left(xf3.cR);
+      const float thW = from_left(th3), thE = from_right(th3);
+      float Fe0 = xf3.F0, Fe1 = xf3.F1, Fe2 = xf3.F2;
+      const float cLe = xf3.cL;
+      float Fs0 = yf4.F0, Fs1 = yf4.F1, Fs2 = yf4.F2;
+      const float cRs = yf4.cR;
+      float Fn0 = yf3.F0, Fn1 = yf3.F1, Fn2 = yf3.F2;
+      const float cLn = yf3.cL;
+      // each face carries its donor's theta (solver-cpu.js:319-326; kernels.wgsl.js:364-365)
+      const float tw = Fw0 > 0.f ? thW : th3;
+      const float te = Fe0 > 0.f ? th3 : thE;
+      const float ts = Fs0 > 0.f ? th4 : th3;
+      const float tn = Fn0 > 0.f ? th3 : th2;
+      const float w0 = Fw0 * tw, w1 = Fw1 * tw, w2 = Fw2 * tw;
+      Fe0 *= te; Fe1 *= te; Fe2 *= te;
+      Fs0 *= ts; Fs1 *= ts; Fs2 *= ts;
+      Fn0 *= tn; Fn1 *= tn; Fn2 *= tn;
+      if (own_col && ry >= y0 && ry < y1 && in3 && M3) {
+        const float dh = -inv_dx * (Fe0 - w0 + Fn0 - Fs0);
+        const float dhu = -inv_dx * ((Fe1 + cLe) - (w1 + cRw) + sx3 + Fn2 - Fs2);
+        const float dhv = -inv_dx * ((Fn1 + cLn) - (Fs1 + cRs) + sy3 + Fe2 - w2);
+        const size_t k = (size_t)ry * pitch + X;
+        float qh = hh3, qhu = hu3, qhv = hv3, qw = Ws[k];
+        if (STAGE == 0) {
+          qhu = qhu + dt * dhu;
+          qhv = qhv + dt * dhv;
+          kadd(qh, qw, dt * dh);
+        } else {
+          const float h0 = I.h[k], hu0 = I.hu[k], hv0 = I.hv[k], w0n = I.w[k];
+          qhu = 0.5f * (hu0 + qhu + dt * dhu);
+          qhv = 0.5f * (hv0 + qhv + dt * dhv);
+          kadd(qh, qw, 0.5f * ((h0 - w0n) - (qh - qw) + dt * dh));
+        }
+        if (qh < 0.f) { acc_clip += -(double)(qh - qw) * areaf; qh = 0.f; qw = 0.f; }
+        if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
+        // boundary outflow through the domain edge faces (edgeOutAt, kernels.wgsl.js:1020-1030)
+        {
+          double o = 0.0;
+          if (X == 0) o -= (double)w0;
+          if (X == nx - 1) o += (double)Fe0;
+          if (ry == 0) o -= (double)Fs0;
+          if (ry == ny - 1) o += (double)Fn0;
+          acc_out += o * ((double)dx * (double)dt * (double)wgt);
+        }
+        if (STAGE == 1) sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+        Ho[k] = qh; HUo[k] = qhu; HVo[k] = qhv; Wo[k] = qw;
+      }
+    }
+
+    // ---------------- rotate the window
+    yf4 = yf3; yf3 = yf2;
+    th4 = th3; th3 = th2;
+    xf3 = xf2; xf2 = xf1; xf1 = xf0;
+    sx3 = sx2; sx2 = sx1; sx1 = sx0;
+    sy3 = sy2; sy2 = sy1;
+    yp_h = yh_p; yp_e = ye_p; yp_n = yn_p; yp_t = yt_p;
+    hh3 = h2r; hu3 = hu2; hv3 = hv2; M3 = M2r; in3 = in2;
+    h2r = h1r; u2r = u1r; v2r = v1r; z2r = z1r; M2r = M1r; hu2 = hu1; hv2 = hv1; in2 = in1;
+    h1r = p.h; u1r = p.u; v1r = p.v; z1r = p.z; M1r = p.M; hu1 = p.hu; hv1 = p.hv; in1 = p.in;
+  }
+
+  const double o = wave_sum(acc_out), c = wave_sum(acc_clip), f = wave_sum(acc_inf);
+  const float s = wave_max(smax);
+  if (lane == 0) part[wid] = Part{o, c, f, s, 0.f};
+}
+
+// max wave speed |u| + |v| + 2 sqrt(g h) over wet open cells (dtReduceAt, kernels.wgsl.js:767-780);
+// used before the first step and after the host changes the state
+__global__ __launch_bounds__(256) void sw_speed(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
+                                                int nwork, Part *__restrict__ part) {
+  const int wid = xcd_wave_index(4);
+  if (wid >= nwork) return;
+  const int lane = threadIdx.x & 63;
+  const int4 wk = work[wid];
+  const DevInst &I = insts[wk.x];
+  const int X = wk.y - kHalo + lane;
+  float smax = 0.f;
+  if (lane >= kHalo && lane < kLanes - kHalo && X < I.nx) {
+    for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
+      const size_t k = (size_t)y * I.pitch + X;
+      const float h = I.h[k];
+      if ((I.mask[k] & MASK_OPEN) && h > I.hdry) smax = fmaxf(smax, cell_speed(h, I.hu[k], I.hv[k], I.g));
+    }
+  }
+  const float s = wave_max(smax);
+  if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, s, 0.f};
+}
+
+// first order (secondOrder = false): the Euler stage's U* plus the sources becomes U^{n+1}
+// (step, solver-cpu.js:432-438: one stage, then _sources)
+__global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict__ insts,
+                                                       const int4 *__restrict__ work, int nwork,
+                                                       Part *__restrict__ part) {
+  const int wid = xcd_wave_index(4);
+  if (wid >= nwork) return;
+  const int lane = threadIdx.x & 63;
+  const int4 wk = work[wid];
+  const DevInst &I = insts[wk.x];
+  if (I.secondOrder) return;  // second order: sw_stage<1> writes this strip's partial
+  const float dt = I.rec->dt;
+  double acc_inf = 0.0;
+  float smax = 0.f;
+  const int X = wk.y - kHalo + lane;
+  if (dt > 0.f && lane >= kHalo && lane < kLanes - kHalo && X < I.nx) {
+    const float rain = I.rec->rainRate, areaf = I.dx * I.dx;
+    for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
+      const size_t k = (size_t)y * I.pitch + X;
+      if (!(I.mask[k] & MASK_OPEN)) continue;
+      float qh = I.h1[k], qw = I.w1[k], qhu = I.hu1[k], qhv = I.hv1[k];
+      sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+      I.h[k] = qh; I.w[k] = qw; I.hu[k] = qhu; I.hv[k] = qhv;
+    }
+  }
+  const double f = wave_sum(acc_inf);
+  const float s = wave_max(smax);
+  if (lane == 0) part[wid] = Part{0.0, 0.0, f, s, 0.f};
+}
+
+// inlet exchange after the sources (exchangeAt, kernels.wgsl.js:597-618): the held rate over the
+// step, never more water than the cell holds; negative rates are point inflows. Writes the wave
+// speed of the inlet cell for the next dt (the second stage leaves inlet cells out of its maximum).
+__global__ void sw_exchange(const DevInst *__restrict__ insts, int inst) {
+  const DevInst &I = insts[inst];
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= I.nin) return;
+  Inlet &q = I.inl[m];
+  const float dt = I.rec->dt;
+  const size_t k = (size_t)q.cell;
+  float h = I.h[k], w = I.w[k], hu = I.hu[k], hv = I.hv[k];
+  if (dt > 0.f) {
+    const float A = I.dx * I.dx;
+    const float dd = fminf(q.q * dt / A, fmaxf(h, 0.f));
+    kadd(h, w, -dd);
+    if (h <= 0.f) { h = 0.f; w = 0.f; }
+    if (h <= I.hdry) { hu = 0.f; hv = 0.f; }
+    I.h[k] = h; I.w[k] = w; I.hu[k] = hu; I.hv[k] = hv;
+    q.vol += (double)dd * (double)A;
+  }
+  I.ispeed[m] = h > I.hdry ? cell_speed(h, hu, hv, I.g) : 0.f;
+}
+
+// End of a step, one block per instance (dtFinal, kernels.wgsl.js:783-799): fold the stage
+// partials into the FP64 books in a fixed order, advance the clock, and set the next dt from the
+// max wave speed: dt = cfl dx / smax (dtDry when dry), clamped to land exactly on tEnd.
+// mode 0: end of a step; 1: start of an advance: instances the host marked dirty take their max
+// speed from the sw_speed partials (p2), the others re-clamp their last one to the new tEnd.
+__global__ __launch_bounds__(256) void sw_dt_final(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
+                                                   const Part *__restrict__ p2, int mode) {
+  const DevInst &I = insts[blockIdx.x];
+  StepRec *rec = I.rec;
+  const int t = threadIdx.x;
+  __shared__ double s_o[4], s_c[4], s_f[4];
+  __shared__ float s_m[4];
+  double o = 0.0, c = 0.0, f = 0.0;
+  float m = 0.f;
+  const bool fresh = mode == 0 || rec->dirty;
+  if (fresh) {
+    for (int k = I.wbeg + t; k < I.wend; k += blockDim.x) {
+      const Part b = p2[k];
+      m = fmaxf(m, b.smax);
+      if (mode == 0) {
+        const Part a = p1[k];
+        o += a.out + b.out;
+        c += a.clip + b.clip;
+        f += b.infil;
+      }
+    }
+    if (mode == 0)
+      for (int k = t; k < I.nin; k += blockDim.x) m = fmaxf(m, I.ispeed[k]);
+  }
+  o = wave_sum(o); c = wave_sum(c); f = wave_sum(f); m = wave_max(m);
+  if ((t & 63) == 0) { s_o[t >> 6] = o; s_c[t >> 6] = c; s_f[t >> 6] = f; s_m[t >> 6] = m; }
+  __syncthreads();
+  if (t != 0) return;
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) { o += s_o[k]; c += s_c[k]; f += s_f[k]; m = fmaxf(m, s_m[k]); }
+  // a no-op step (dt = 0, past tEnd) measured nothing: keep the last real max speed
+  if (!fresh || (mode == 0 && !(rec->dt > 0.f))) m = rec->smax;
+  rec->dirty = 0;
+  if (mode == 0 && rec->dt > 0.f) {
+    const float dtp = rec->dt;
+    rec->outflow += o;
+    rec->clipped += c;
+    rec->infil += f;
+    if (rec->rainCells >= 0) rec->rain += (double)(rec->rainRate * dtp) * (double)rec->rainCells * (double)I.area;
+    else rec->rain += (double)rec->rainRate * (double)dtp * rec->rainArea;
+    rec->t = rec->tAfter;
+    rec->steps += 1;
+    rec->lastDt = dtp;
+    rec->cellUpdates += (double)rec->openCells;
+  }
+  rec->smax = m;
+  const double rem = rec->tEnd - rec->t;
+  float dt = 0.f;
+  double after = rec->t;
+  if (rem > 0.0) {
+    const float cand = m > 0.f ? I.cfl * I.dx / m : I.dtDry;
+    if ((double)cand >= rem) { dt = (float)rem; after = rec->tEnd; }
+    else { dt = cand; after = rec->t + (double)cand; }
+    if (!(dt > 0.f)) { dt = 0.f; after = rec->t; }
+  }
+  rec->dt = dt;
+  rec->tAfter = after;
+}
+
+}  // namespace fs

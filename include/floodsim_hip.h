@@ -1,0 +1,135 @@
+/*
+ * floodsim_hip: MI355X (gfx950) shallow-water step behind a C ABI.
+ *
+ * The surface solver of the reference (src/solver-gpu.js, class GPUSolver) is a
+ * host object that owns its device buffers, is built from the model layers,
+ * advanced in time and read back. This header is the C-ABI replacement of that
+ * object; the N-API addon (floodsim_amd/csrc/napi_addon.c) and the JavaScript
+ * class (floodsim_amd/js/solver-hip.js) bind it one to one.
+ *
+ * Conventions: every array is interior-only, nx*ny, row-major (k = j*nx + i), in
+ * SI units (m, m/s, m^3); the bed is absolute (the library stores it relative to
+ * its minimum, as solver-gpu.js:52-53, 65 does). Every call returns 0 on success
+ * or a negative FS_E_* code; fs_last_error() gives the message. Calls on one
+ * handle must not overlap (the JS side serialises them).
+ */
+#ifndef FLOODSIM_HIP_H
+#define FLOODSIM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_OK 0
+#define FS_E_ARG (-1)      /* bad argument */
+#define FS_E_HIP (-2)      /* a HIP runtime call failed */
+#define FS_E_BUDGET (-3)   /* admission refused: the pool's HBM budget is spent */
+#define FS_E_STALL (-4)    /* advance made no progress (solver-gpu.js:601) */
+#define FS_E_STATE (-5)    /* call not valid in this state */
+
+enum { FS_BC_WALL = 0, FS_BC_OPEN = 1, FS_BC_OUTFLOW = 2 };
+
+/* Constructor options of GPUSolver (solver-gpu.js:35-158; defaults at :39-47). */
+typedef struct fs_params {
+  int32_t nx, ny;      /* interior cells */
+  double dx;           /* cell size, m */
+  double g;            /* 9.81 */
+  double cfl;          /* 0.48 second order, 0.9 first order */
+  double hdry;         /* 1e-5 (GPU default, solver-gpu.js:42) */
+  double hfront;       /* 1e-3 */
+  double dtDry;        /* step while all is dry, s (1.0) */
+  int32_t secondOrder; /* 1 = MUSCL + SSP-RK2 */
+  int32_t limiter;     /* 0 minmod, 1 van Leer */
+  int32_t bc[4];       /* west, east, south, north: FS_BC_* */
+  int32_t lts;         /* local time stepping levels 0..4 (0 = global steps) */
+  int32_t rows;        /* rows per wavefront strip (0 = autotuned) */
+} fs_params;
+
+/* clock fields of GPUSolver: t, steps, lastDt, dtAvg (solver-gpu.js:578-586) */
+typedef struct fs_clock {
+  double t;
+  double lastDt;
+  double dtAvg;
+  int64_t steps;
+  int64_t cellUpdates; /* interior open cells advanced, summed over steps (LTS: per own-level step) */
+} fs_clock;
+
+/* readState().mass of GPUSolver (solver-gpu.js:683-689), FP64 books */
+typedef struct fs_mass {
+  double volume, rain, infil, outflow, exchanged, clipped, expected, rel, relExClip, initial;
+} fs_mass;
+
+typedef struct fs_pool *fs_pool_t;
+
+/* ---- one solver: new GPUSolver(device, o) (solver-gpu.js:35) ----
+ * z, mask required (mask may be NULL = all open); manning NULL uses manningScalar;
+ * infil (m/s) and rainField NULL = none / 1; inlets are interior indices j*nx+i. */
+typedef struct fs_solver *fs_handle;
+int fs_create(const fs_params *p, const double *z, const double *manning, double manningScalar,
+              const uint8_t *mask, const double *infil, const double *rainField, const int32_t *inlets,
+              int32_t nInlets, fs_handle *out);
+void fs_destroy(fs_handle s);                              /* destroy() (solver-gpu.js:692) */
+const char *fs_last_error(fs_handle s);                    /* NULL handle: last create error */
+/* setDepth + setVelocity (solver-gpu.js:301-318): host-evaluated h, hu, hv; bookInitial=1 books
+ * vol.initial = sum of the stored FP32 depths over open cells, as setDepth does */
+int fs_set_state(fs_handle s, const double *h, const double *hu, const double *hv, int32_t bookInitial);
+int fs_set_rain(fs_handle s, double rate_ms);              /* rainRate property (solver-gpu.js:48, 603) */
+int fs_set_rain_field(fs_handle s, const double *field);   /* setRainField (solver-gpu.js:470-478) */
+int fs_set_inlet_rates(fs_handle s, const double *q, int32_t n);           /* setInletRates (:557) */
+int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n);        /* readInlets (:564-576) */
+int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk);     /* advance (:589-604) */
+/* frame (:607-622): at most K steps toward tEnd; out (nx*ny*2: depth, speed) may be NULL */
+int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk);
+/* readState (:651-690): interior FP64 copies (any may be NULL) and the FP64 books */
+int fs_read_state(fs_handle s, double *h, double *hu, double *hv, double *hmax, fs_mass *mass);
+int fs_clock_get(fs_handle s, fs_clock *clk);
+/* tileStats (:625-629): open cells advanced by the last step and the interior cell total */
+int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total);
+
+/* ---- batched / served instances (no counterpart in the reference, which runs one
+ * solve at a time: src/batch.js:1-4). Many uneven-size instances share one launch
+ * sequence; admission is against an HBM budget. ---- */
+int fs_pool_create(size_t hbmBudgetBytes, fs_pool_t *out);
+void fs_pool_destroy(fs_pool_t pool);
+const char *fs_pool_last_error(fs_pool_t pool);
+size_t fs_pool_bytes_needed(const fs_params *p, int32_t withInfil, int32_t withRainField);
+int fs_pool_add(fs_pool_t pool, const fs_params *p, const double *z, const double *manning, double manningScalar,
+                const uint8_t *mask, const double *infil, const double *rainField, const int32_t *inlets,
+                int32_t nInlets, int32_t *slot);
+int fs_pool_remove(fs_pool_t pool, int32_t slot);
+int fs_pool_set_state(fs_pool_t pool, int32_t slot, const double *h, const double *hu, const double *hv,
+                      int32_t bookInitial);
+int fs_pool_set_rain(fs_pool_t pool, int32_t slot, double rate_ms);
+int fs_pool_set_inlet_rates(fs_pool_t pool, int32_t slot, const double *q, int32_t n);
+/* advance the listed slots together until each reaches its own target (K steps per launch group);
+ * clks (n entries) may be NULL */
+int fs_pool_advance(fs_pool_t pool, const int32_t *slots, const double *tTargets, int32_t n, int32_t K,
+                    fs_clock *clks);
+/* exactly K steps of every listed slot (stopping at its target), one device sync */
+int fs_pool_step(fs_pool_t pool, const int32_t *slots, const double *tTargets, int32_t n, int32_t K);
+int fs_pool_read_state(fs_pool_t pool, int32_t slot, double *h, double *hu, double *hv, double *hmax,
+                       fs_mass *mass);
+int fs_pool_clock(fs_pool_t pool, int32_t slot, fs_clock *clk);
+size_t fs_pool_bytes_used(fs_pool_t pool);
+/* wall time in ms of the dominant (second-stage) kernel, averaged over the launches since the
+ * last call, measured with HIP events on the pool's stream; -1 when not measured */
+int fs_pool_kernel_ms(fs_pool_t pool, double *stage1_ms, double *stage2_ms, int64_t *launches);
+int fs_pool_set_timing(fs_pool_t pool, int32_t on);
+/* the underlying pool of a single solver (for timing and tuning calls) */
+fs_pool_t fs_solver_pool(fs_handle s);
+
+/* HBM reference: copies `bytes` device-to-device `iters` times with a plain vector copy kernel
+ * and reports read+write GB/s and ms per copy (no counterpart in the reference: it measures the
+ * achievable bandwidth the roofline is quoted against, and calibrates byte counters) */
+int fs_copy_probe(size_t bytes, int32_t iters, double *gbps, double *ms);
+
+/* library version string */
+const char *fs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -2,11 +2,12 @@
 
     python bench.py [--steps K] [--warmup W] [--workload urban8192|urban2048|dambreak256] [--gpus 1]
 
-A step is one SSP-RK2 step (two fused stage kernels, the inlet exchange and the dt kernel) over
-the whole synthetic grid, with the state resident in HBM. Default workload: the 8192 x 8192
-synthetic urban domain at 1 m (0.5 % plane + fBm, ~30 % building walls, paved/pervious n, rain
-50 mm/h, a triangular point inflow 0 -> 40 m^3/s at the west edge, free-outfall edges), spun up
-from dry for 600 s of simulated time, then W untimed and K timed steps.
+Default workload: the 8192 x 8192 synthetic urban domain at 1 m (0.5 % plane + fBm, ~30 % building
+walls, paved/pervious n, rain 50 mm/h, a triangular point inflow 0 -> 40 m^3/s at the west edge,
+free-outfall edges) with local time stepping on 4 levels, spun up from dry for 600 s of simulated
+time in 10 s frames, then W untimed and K timed steps. With local time stepping a step is one
+cycle (2^4 base steps, 31 level records); with --stepping global it is one SSP-RK2 step. The
+state stays resident in HBM throughout.
 
 Prints ONE JSON line: cell-updates/s (open cells advanced per second), the roofline of the
 dominant kernel (second RK stage + sources) measured with HIP events over the timed region, and
@@ -45,17 +46,17 @@ def build_workload(name, seed=2026):
         z[jin - 2:jin + 3, 0:5] = np.minimum(z[jin - 2:jin + 3, 0:5], z[jin, 5])
         return dict(nx=n, ny=n, dx=1.0, z=u["z"], mask=u["mask"], manning=u["manning"],
                     bc={k: "outflow" for k in ("west", "east", "south", "north")},
-                    inlets=[jin * n], rain=50 * sc.MMHR, hydro=True, lts_levels=4,
+                    inlets=[jin * n], rain=50 * sc.MMHR, hydro=True, lts=4,
                     checksum=sc.checksum(u["z"], u["mask"], u["manning"]))
     if name == "dambreak256":
         d = sc.dam_break(256)
         return dict(nx=256, ny=256, dx=1.0, z=d["z"], mask=None, manning=0.03, bc={}, inlets=None, rain=0.0,
-                    hydro=False, depth=d["depth"], cfl=0.5)
+                    hydro=False, depth=d["depth"], cfl=0.5, lts=0)
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(w, state, window=1024, steps=None, budget_s=20.0):
-    """The FP64 oracle (single thread) on a window of the same spun-up state."""
+def cpu_baseline(w, state, lts=0, window=1024, steps=None, budget_s=20.0):
+    """The FP64 oracle (single thread, same stepping) on a window of the same spun-up state."""
     import oracle
     oracle.set_threads(1)
     n = w["nx"]
@@ -65,7 +66,7 @@ def cpu_baseline(w, state, window=1024, steps=None, budget_s=20.0):
     cut = lambda a: np.ascontiguousarray(np.asarray(a).reshape(n, n)[sl]).reshape(-1)
     man = w["manning"] if np.isscalar(w["manning"]) else cut(w["manning"])
     c = oracle.CPUSolver(nx=m, ny=m, dx=w["dx"], z=cut(w["z"]), manning=man,
-                         mask=None if w["mask"] is None else cut(w["mask"]), bc=w["bc"], hdry=1e-5)
+                         mask=None if w["mask"] is None else cut(w["mask"]), bc=w["bc"], hdry=1e-5, lts=lts)
     c.setState(cut(state["h"]), cut(state["hu"]), cut(state["hv"]), t=0.0)
     c.rainRate = w["rain"]
     open_cells = m * m if w["mask"] is None else int(cut(w["mask"]).sum())
@@ -73,23 +74,27 @@ def cpu_baseline(w, state, window=1024, steps=None, budget_s=20.0):
     t0 = time.perf_counter()
     c.step()
     one = time.perf_counter() - t0
-    k = steps or max(2, min(200, int(budget_s / max(one, 1e-6))))
+    k = steps or max(1, min(200, int(budget_s / max(one, 1e-6))))
+    u0 = c.cellSteps
     t0 = time.perf_counter()
     for _ in range(k):
         c.step()
     el = time.perf_counter() - t0
-    return dict(value=open_cells * k / el, unit="cell-updates/s", cores=1, kind="port",
+    upd = (c.cellSteps - u0) if lts else open_cells * k
+    kind = "LTS cycles (%d levels)" % lts if lts else "global SSP-RK2 steps"
+    return dict(value=upd / el, unit="cell-updates/s", cores=1, kind="port",
                 sample=f"FP64 oracle (C restatement of solver-cpu.js), 1 thread, {m}x{m} centre window of the "
-                       f"spun-up state, {k} global SSP-RK2 steps in {el:.1f} s")
+                       f"spun-up state, {k} {kind} in {el:.1f} s")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (LTS: cycles); default 40 cycles / 300 steps")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first; default 4 cycles / 20 steps")
     ap.add_argument("--workload", default="urban8192")
     ap.add_argument("--spinup", type=float, default=600.0)
+    ap.add_argument("--stepping", choices=("lts", "global"), default=None)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -105,21 +110,28 @@ def main():
     t_setup = time.perf_counter()
     w = build_workload(args.workload)
     log(f"workload {args.workload}: generated in {time.perf_counter() - t_setup:.1f} s")
+    stepping = args.stepping or ("lts" if w.get("lts") else "global")
+    lts = (w.get("lts") or 4) if stepping == "lts" else 0
+    if args.steps is None:
+        args.steps = 40 if lts else 300
+    if args.warmup is None:
+        args.warmup = 4 if lts else 20
     g = fsa.GPUSolver(nx=w["nx"], ny=w["ny"], dx=w["dx"], z=w["z"], manning=w["manning"], mask=w["mask"],
-                      bc=w["bc"], inlets=w["inlets"], cfl=w.get("cfl"), rows=args.rows)
+                      bc=w["bc"], inlets=w["inlets"], cfl=w.get("cfl"), rows=args.rows, lts=lts)
     if "depth" in w:
         g.setDepth(w["depth"])
     else:
         g.setDepth(lambda X, Y, Z: 0 * X)
     g.rainRate = w["rain"]
-    # spin-up: the hydrograph held per 60 s segment
+    # spin-up in 10 s frames (the reference app's quanta, app/map.js:2281); the hydrograph is held
+    # per frame at its mid-frame value
     t = 0.0
-    seg = 60.0
+    seg = 10.0
     while t < args.spinup - 1e-9:
         if w["hydro"]:
             g.setInletRates([-float(sc.triangle_hydrograph(t + seg / 2))])
         t = min(args.spinup, t + seg)
-        g.advance(t, 64)
+        g.advance(t, 8 if lts else 64)
     log(f"spun up to t={g.t:.1f} s in {g.steps} steps ({time.perf_counter() - t_setup:.1f} s total)")
     big = 1e9
     from floodsim_amd.solver import lib
@@ -143,13 +155,15 @@ def main():
     c2 = g.clock()                                   # synchronises again
     el = time.perf_counter() - t0
     ms1, ms2, nt = g.kernel_ms()
+    dom_ms, dom_cells, dom_n = g.dominant_kernel()
     g.set_timing(False)
     steps = c2.steps - c1.steps
     updates = c2.cellUpdates - c1.cellUpdates
     cells = w["nx"] * w["ny"]
     value = updates / el
     ms_step = 1e3 * el / max(steps, 1)
-    achieved = cells * B_STAGE2 / (ms2 * 1e-3) / 1e9
+    # dominant kernel: the second RK stage (+ sources); algorithmic bytes = 65 B per open cell it advances
+    achieved = dom_cells * B_STAGE2 / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     s = g.readState()
     try:
         cp_gbs, _ = copy_probe(2 << 30, 10)
@@ -158,7 +172,7 @@ def main():
         cp_gbs = None
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(w, s)
+        cpu = cpu_baseline(w, s, lts=lts)
     out = {
         "metric": "cell-updates/s + % HBM roofline",
         "value": value,
@@ -174,16 +188,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "nx": w["nx"], "ny": w["ny"], "dx_m": w["dx"],
                    "open_cells": int(cells if w["mask"] is None else int(np.asarray(w["mask"]).sum())),
-                   "stepping": "global SSP-RK2", "spinup_s": args.spinup, "t_start": c1.t, "t_end": c2.t,
+                   "stepping": ("LTS, %d levels (a step is one cycle)" % lts) if lts else "global SSP-RK2",
+                   "spinup_s": args.spinup, "t_start": c1.t, "t_end": c2.t,
                    "layers_checksum": w.get("checksum"), "rows_per_strip": args.rows or 64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fs::sw_stage<1> (second RK stage + sources)",
-                     "bytes_per_cell": B_STAGE2, "cells_per_launch": cells, "avg_ms": ms2},
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": "fs::sw_stage<1,%s> (second RK stage + sources)" % ("true" if lts else "false"),
+                     "bytes_per_cell_update": B_STAGE2, "launches": dom_n,
+                     "cells_per_launch": dom_cells / max(dom_n, 1), "avg_ms": dom_ms / max(dom_n, 1)},
         "cpu_baseline": cpu,
         "extra": {"stage1_ms": ms1, "stage2_ms": ms2, "timed_launches": nt,
-                  "step_GBps_106B": cells * B_STEP / (ms_step * 1e-3) / 1e9,
-                  "step_frac_106B": cells * B_STEP / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                  "step_GBps_106B": value * B_STEP / 1e9,
+                  "step_frac_106B": value * B_STEP / 1e9 / HBM_PEAK_GBS,
                   "copy_probe_GBps": cp_gbs, "all_cell_updates_per_s": cells * steps / el,
                   "mass_rel": s["mass"]["rel"], "last_dt": c2.lastDt},
     }

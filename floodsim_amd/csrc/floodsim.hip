@@ -45,6 +45,14 @@ struct Slot {
   std::vector<float> hrf;                // host copy of the rain multiplier (empty = 1)
   int openCells = 0;
   int rows = 64;
+  // local time stepping
+  int L = 0, tx = 0, ty = 0, nunits = 0;
+  uint8_t *lv = nullptr, *lv0 = nullptr, *gflag = nullptr, *umask = nullptr;
+  float *tsp = nullptr, *tileRf = nullptr;
+  int32_t *tileOpen = nullptr, *ucells = nullptr, *ulist = nullptr;
+  float *sh = nullptr, *shu = nullptr, *shv = nullptr, *sw = nullptr;
+  float *g0h = nullptr, *g0hu = nullptr, *g0hv = nullptr, *g1h = nullptr, *g1hu = nullptr, *g1hv = nullptr;
+  float4 *bsx = nullptr, *bsy = nullptr;
   StepRec hrec{};                        // host mirror (refreshed after each launch group)
   double volInitial = 0.0;               // vol.initial (solver-gpu.js:309)
   double exchangedRead = 0.0;            // vol.exchanged: inlet volumes already read (solver-gpu.js:574)
@@ -61,6 +69,7 @@ struct Group {
   std::vector<DevInst> hinst;
   std::vector<int> withInlets;           // group indices of instances with inlets
   bool anyFirstOrder = false, anySecondOrder = false;
+  bool lts = false;                      // a local-time-stepping instance (alone in its group)
   // captured K-step graph
   hipGraphExec_t graph = nullptr;
   int graphK = 0;
@@ -83,6 +92,15 @@ struct fs_pool {
   int nTev = 0;                          // events recorded since the last fs_pool_kernel_ms
   double ms1 = 0.0, ms2 = 0.0;
   int64_t nTimed = 0;
+  // timing mode, dominant kernel (second stage): launches, total ms, open cells it advanced
+  struct Span { hipEvent_t a, b; int cycle, level; };
+  std::vector<Span> spans;
+  std::vector<hipEvent_t> spanEv;
+  int nSpanEv = 0, nCycles = 0;
+  int32_t *dhist = nullptr;              // per timed LTS cycle: own open cells per level
+  int capHist = 0;
+  double domMs = 0.0, domCells = 0.0;
+  int64_t domLaunches = 0;
   std::string err;
 };
 
@@ -112,9 +130,28 @@ int fail(fs_pool *P, int code, const char *fmt, ...) {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// strip geometry: global stepping uses 58 owned columns (3 halo lanes each side); local time
+// stepping uses 56 (4 halo lanes on the left) and rows in multiples of 8 so that a strip holds
+// whole 8x8 level tiles
+int strip_own(const fs_params *p) { return p->lts ? 56 : kOwn; }
+int strip_rows(const fs_params *p) {
+  int r = p->rows > 0 ? p->rows : 64;
+  if (p->lts) r = std::max(8, (r + 7) / 8 * 8);
+  return r;
+}
+int units_of(const fs_params *p) {
+  return ((p->nx + strip_own(p) - 1) / strip_own(p)) * ((p->ny + strip_rows(p) - 1) / strip_rows(p));
+}
+
 size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
   const size_t N = (size_t)p->nx * p->ny;
   size_t b = 0;
+  if (p->lts) {
+    const size_t nt = (size_t)((p->nx + 7) / 8) * ((p->ny + 7) / 8), nu = (size_t)units_of(p);
+    b += 2 * align256(nt) + align256(nt * 4) * 3;           // lv, lv0, tsp, tileOpen, tileRf
+    b += 10 * align256(N * 4) + align256(N) + 2 * align256(N * 16);  // store, gathers, flag, bsx, bsy
+    b += align256(nu) + 2 * align256(nu * 4 * (kMaxLevels + 1));     // umask, ucells, ulist
+  }
   b += 8 * align256(N * 4);               // A and B
   b += 3 * align256(N * 4);               // z, n, hmax
   b += align256(N);                       // mask
@@ -132,7 +169,7 @@ int check_params(fs_pool *P, const fs_params *p) {
   if (!(p->cfl > 0) || !(p->hdry > 0) || !(p->dtDry > 0)) return fail(P, FS_E_ARG, "cfl, hdry, dtDry must be > 0");
   for (int k = 0; k < 4; ++k)
     if (p->bc[k] < 0 || p->bc[k] > 2) return fail(P, FS_E_ARG, "bc[%d] must be 0 wall, 1 open or 2 outflow", k);
-  if (p->lts != 0) return fail(P, FS_E_ARG, "local time stepping is not available in this build");
+  if (p->lts < 0 || p->lts > kMaxLevels) return fail(P, FS_E_ARG, "lts must be 0..%d", kMaxLevels);
   return FS_OK;
 }
 
@@ -183,6 +220,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
   G.hinst.clear();
   G.withInlets.clear();
   G.anyFirstOrder = G.anySecondOrder = false;
+  G.lts = false;
   std::vector<int4> work;
   for (size_t gi = 0; gi < slots.size(); ++gi) {
     Slot &S = P->slots[slots[gi]];
@@ -200,17 +238,27 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.limiter = S.p.limiter;
     for (int k = 0; k < 4; ++k) d.bc[k] = S.p.bc[k];
     d.wbeg = (int)work.size();
-    const int rows = std::max(1, S.rows);
+    const int rows = std::max(1, S.rows), own = strip_own(&S.p);
     for (int y0 = 0; y0 < S.ny; y0 += rows)
-      for (int x0 = 0; x0 < S.nx; x0 += kOwn) work.push_back(make_int4((int)gi, x0, y0, rows));
+      for (int x0 = 0; x0 < S.nx; x0 += own) work.push_back(make_int4((int)gi, x0, y0, rows));
     d.wend = (int)work.size();
     d.inl = S.inl; d.ispeed = S.ispeed; d.nin = S.nin;
+    d.hl = S.L ? 4 : kHalo;
+    d.own = own;
+    d.L = S.L; d.tx = S.tx; d.ty = S.ty;
+    d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.tileOpen = S.tileOpen; d.tileRf = S.rf ? S.tileRf : nullptr;
+    d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
+    d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.g1h = S.g1h; d.g1hu = S.g1hu; d.g1hv = S.g1hv;
+    d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
+    d.umask = S.umask; d.ucells = S.ucells; d.ulist = S.ulist;
+    if (S.L) G.lts = true;
     if (S.nin) G.withInlets.push_back((int)gi);
     if (!S.p.secondOrder) G.anyFirstOrder = true;
     else G.anySecondOrder = true;
     G.hinst.push_back(d);
   }
   const int ni = (int)G.hinst.size(), nw = (int)work.size();
+  if (G.lts && ni > 1) { G.slots.clear(); return fail(P, FS_E_ARG, "an instance with local time stepping advances alone"); }
   if (ni > G.capInst) {
     if (G.dinst) HIPCHK(P, hipFree(G.dinst));
     G.capInst = std::max(ni, G.capInst * 2);
@@ -236,6 +284,15 @@ __global__ void sw_set_targets(const DevInst *__restrict__ insts, int n, const d
   if (i < n) insts[i].rec->tEnd = t[i];
 }
 
+hipEvent_t span_event(fs_pool *P) {
+  if (P->nSpanEv >= (int)P->spanEv.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    P->spanEv.push_back(e);
+  }
+  return P->spanEv[P->nSpanEv++];
+}
+
 hipEvent_t timing_event(fs_pool *P) {
   if (P->nTev >= (int)P->tev.size()) {
     hipEvent_t e = nullptr;
@@ -254,15 +311,16 @@ int launch_step(fs_pool *P) {
     if (!e0 || !e1 || !e2) return fail(P, FS_E_HIP, "hipEventCreate failed");
     HIPCHK(P, hipEventRecord(e0, P->stream));
   }
-  sw_stage<0><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1);
+  sw_stage<0, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
   if (P->timing) HIPCHK(P, hipEventRecord(e1, P->stream));
   // each instance runs its own scheme: the kernel of the other order returns at once for it
-  if (G.anySecondOrder) sw_stage<1><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2);
-  if (G.anyFirstOrder) sw_euler_finish<<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2);
-  if (P->timing) HIPCHK(P, hipEventRecord(e2, P->stream));
+  if (G.anySecondOrder) sw_stage<1, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+  if (P->timing) P->spans.push_back({e1, nullptr, -1, -1});  // completed below with e2
+  if (G.anyFirstOrder) sw_euler_finish<false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+  if (P->timing) { HIPCHK(P, hipEventRecord(e2, P->stream)); P->spans.back().b = e2; }
   for (int gi : G.withInlets) {
     const int nin = G.hinst[gi].nin;
-    sw_exchange<<<(nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi);
+    sw_exchange<false><<<(nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
   }
   sw_dt_final<<<ni, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, 0);
   HIPCHK(P, hipGetLastError());
@@ -282,18 +340,103 @@ int collect_timing(fs_pool *P) {
     P->nTimed += 1;
   }
   P->nTev = 0;
+  if (!P->spans.empty()) {
+    HIPCHK(P, hipEventSynchronize(P->spans.back().b));
+    std::vector<int32_t> hist;
+    if (P->nCycles) {
+      hist.resize((size_t)(kMaxLevels + 1) * P->nCycles);
+      HIPCHK(P, hipMemcpy(hist.data(), P->dhist, sizeof(int32_t) * hist.size(), hipMemcpyDeviceToHost));
+    }
+    double open = 0.0;
+    for (int s : P->grp.slots) open += P->slots[s].openCells;
+    for (auto &sp : P->spans) {
+      float ms = 0.f;
+      HIPCHK(P, hipEventElapsedTime(&ms, sp.a, sp.b));
+      P->domMs += ms;
+      P->domLaunches += 1;
+      P->domCells += sp.cycle < 0 ? open : (double)hist[(size_t)(kMaxLevels + 1) * sp.cycle + sp.level];
+    }
+    P->spans.clear();
+    P->nSpanEv = 0;
+    P->nCycles = 0;
+  }
+  return FS_OK;
+}
+
+// one local-time-stepping cycle of the group's (single) instance: levels and lists, then the
+// records finest first (solver-cpu.js:504-527; solver-gpu.js:367-396)
+int launch_cycle(fs_pool *P) {
+  Group &G = P->grp;
+  const DevInst &d = G.hinst[0];
+  const int nt = d.tx * d.ty, nu = d.wend - d.wbeg, L = d.L;
+  const int nbp = std::min((nu + 3) / 4, 2048);
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  if (P->timing) {
+    e0 = timing_event(P); e1 = timing_event(P); e2 = timing_event(P);
+    if (!e0 || !e1 || !e2) return fail(P, FS_E_HIP, "hipEventCreate failed");
+    HIPCHK(P, hipEventRecord(e0, P->stream));
+  }
+  sw_lts_speed<<<(nt + 3) / 4, 256, 0, P->stream>>>(G.dinst);
+  sw_lts_cycle<<<1, 1024, 0, P->stream>>>(G.dinst);
+  sw_lts_level<<<(nt + 255) / 256, 256, 0, P->stream>>>(G.dinst);
+  sw_lts_smooth<<<(nt + 255) / 256, 256, 0, P->stream>>>(G.dinst);
+  sw_lts_units<<<(nu + 255) / 256, 256, 0, P->stream>>>(G.dinst, G.dwork);
+  sw_lts_lists<<<1, 1024, 0, P->stream>>>(G.dinst);
+  const int cyc = P->nCycles;
+  if (P->timing) {
+    HIPCHK(P, hipEventRecord(e1, P->stream));
+    if (cyc >= P->capHist) {
+      const int cap = std::max(64, 2 * P->capHist);
+      int32_t *h = nullptr;
+      HIPCHK(P, hipStreamSynchronize(P->stream));
+      HIPCHK(P, hipMalloc(&h, sizeof(int32_t) * (kMaxLevels + 1) * cap));
+      if (P->dhist) {
+        HIPCHK(P, hipMemcpy(h, P->dhist, sizeof(int32_t) * (kMaxLevels + 1) * P->capHist, hipMemcpyDeviceToDevice));
+        HIPCHK(P, hipFree(P->dhist));
+      }
+      P->dhist = h;
+      P->capHist = cap;
+    }
+    HIPCHK(P, hipMemcpyAsync(P->dhist + (size_t)(kMaxLevels + 1) * cyc, &P->drec[G.slots[0]].lcells,
+                             sizeof(int32_t) * (kMaxLevels + 1), hipMemcpyDeviceToDevice, P->stream));
+    P->nCycles += 1;
+  }
+  const int Pn = 1 << L;
+  for (int k = 0; k < Pn; ++k)
+    for (int m = 0; m <= L; ++m) {
+      if ((k + 1) % (1 << m)) continue;
+      sw_stage<0, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
+      hipEvent_t sa = nullptr, sb = nullptr;
+      if (P->timing) {
+        sa = span_event(P); sb = span_event(P);
+        if (!sa || !sb) return fail(P, FS_E_HIP, "hipEventCreate failed");
+        HIPCHK(P, hipEventRecord(sa, P->stream));
+      }
+      if (d.secondOrder) sw_stage<1, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, m);
+      else sw_euler_finish<true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, m);
+      if (P->timing) {
+        HIPCHK(P, hipEventRecord(sb, P->stream));
+        P->spans.push_back({sa, sb, cyc, m});
+      }
+      sw_lts_fold<<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, m);
+      if (d.nin) sw_exchange<true><<<(d.nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
+      sw_lts_record_end<<<1, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, m);
+    }
+  sw_lts_cycle_end<<<1, 1, 0, P->stream>>>(G.dinst);
+  if (P->timing) HIPCHK(P, hipEventRecord(e2, P->stream));
+  HIPCHK(P, hipGetLastError());
   return FS_OK;
 }
 
 int launch_steps(fs_pool *P, int K) {
   Group &G = P->grp;
-  auto one = [&]() { return launch_step(P); };
+  auto one = [&]() { return G.lts ? launch_cycle(P) : launch_step(P); };
   if (P->timing || !P->useGraphs || K < 2) {
     for (int k = 0; k < K; ++k) {
       int rc = one();
       if (rc) return rc;
     }
-    return P->timing && P->nTev > 3 * 256 ? collect_timing(P) : FS_OK;
+    return P->timing && (P->nTev > 3 * 256 || P->spans.size() > 4096) ? collect_timing(P) : FS_OK;
   }
   if (!G.graph || G.graphK != K) {
     drop_graph(G);
@@ -328,8 +471,10 @@ int begin_advance(fs_pool *P, const std::vector<int> &slots, const std::vector<d
   sw_set_targets<<<(ni + 63) / 64, 64, 0, P->stream>>>(G.dinst, ni, P->dtarget);
   bool dirty = false;
   for (int s : slots) dirty = dirty || P->slots[s].hrec.dirty;
-  if (dirty) sw_speed<<<(G.nwork + 3) / 4, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2);
-  sw_dt_final<<<ni, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, 1);
+  if (!G.lts) {
+    if (dirty) sw_speed<<<(G.nwork + 3) / 4, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2);
+    sw_dt_final<<<ni, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, 1);
+  }
   HIPCHK(P, hipGetLastError());
   for (int s : slots) P->slots[s].hrec.dirty = 0;
   return FS_OK;
@@ -436,6 +581,24 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   S.nin = nInlets;
   S.inl = (Inlet *)take((size_t)std::max(nInlets, 1) * sizeof(Inlet));
   S.ispeed = (float *)take((size_t)std::max(nInlets, 1) * 4);
+  S.rows = strip_rows(p);
+  if (p->lts) {
+    S.L = p->lts;
+    S.tx = (nx + 7) / 8;
+    S.ty = (ny + 7) / 8;
+    const size_t nt = (size_t)S.tx * S.ty;
+    S.nunits = units_of(p);
+    S.lv = (uint8_t *)take(nt); S.lv0 = (uint8_t *)take(nt);
+    S.tsp = (float *)take(nt * 4); S.tileOpen = (int32_t *)take(nt * 4); S.tileRf = (float *)take(nt * 4);
+    S.sh = (float *)take(N * 4); S.shu = (float *)take(N * 4); S.shv = (float *)take(N * 4); S.sw = (float *)take(N * 4);
+    S.g0h = (float *)take(N * 4); S.g0hu = (float *)take(N * 4); S.g0hv = (float *)take(N * 4);
+    S.g1h = (float *)take(N * 4); S.g1hu = (float *)take(N * 4); S.g1hv = (float *)take(N * 4);
+    S.gflag = (uint8_t *)take(N);
+    S.bsx = (float4 *)take(N * 16); S.bsy = (float4 *)take(N * 16);
+    S.umask = (uint8_t *)take((size_t)S.nunits);
+    S.ucells = (int32_t *)take((size_t)S.nunits * 4 * (kMaxLevels + 1));
+    S.ulist = (int32_t *)take((size_t)S.nunits * 4 * (kMaxLevels + 1));
+  }
   S.used = true;
   P->used += need;
 
@@ -461,6 +624,23 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   if (infil) {
     for (size_t k = 0; k < N; ++k) tmp[k] = (float)infil[k];
     HIPCHK(P, hipMemcpy(S.infil, tmp.data(), N * 4, hipMemcpyHostToDevice));
+  }
+  if (S.L) {
+    const size_t nt = (size_t)S.tx * S.ty;
+    std::vector<int32_t> to(nt, 0);
+    std::vector<float> tr(nt, 0.f);
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        if (S.hmask[(size_t)j * nx + i] & MASK_OPEN) {
+          to[(size_t)(j / 8) * S.tx + i / 8] += 1;
+          tr[(size_t)(j / 8) * S.tx + i / 8] += rainField ? (float)rainField[(size_t)j * nx + i] : 1.f;
+        }
+    HIPCHK(P, hipMemcpy(S.tileOpen, to.data(), nt * 4, hipMemcpyHostToDevice));
+    HIPCHK(P, hipMemcpy(S.tileRf, tr.data(), nt * 4, hipMemcpyHostToDevice));
+    HIPCHK(P, hipMemset(S.lv, 0, nt));
+    HIPCHK(P, hipMemset(S.sh, 0, (size_t)((char *)S.bsx - (char *)S.sh)));      // store, gathers, flags
+    HIPCHK(P, hipMemset(S.bsx, 0, N * 32));
+    HIPCHK(P, hipDeviceSynchronize());
   }
   double rainArea = 0.0;
   if (rainField) {
@@ -608,6 +788,8 @@ void fs_pool_destroy(fs_pool_t P) {
   for (auto &ev : P->ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : P->tev) (void)hipEventDestroy(ev);
+  for (auto &ev : P->spanEv) (void)hipEventDestroy(ev);
+  if (P->dhist) (void)hipFree(P->dhist);
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
 }
@@ -771,10 +953,27 @@ int fs_pool_read_state(fs_pool_t P, int32_t slot, double *h, double *hu, double 
   return FS_OK;
 }
 
+int fs_pool_dominant_kernel(fs_pool_t P, double *ms_total, double *cells_total, int64_t *launches) {
+  if (!P) return FS_E_ARG;
+  int rc = collect_timing(P);
+  if (rc) return rc;
+  if (ms_total) *ms_total = P->domMs;
+  if (cells_total) *cells_total = P->domCells;
+  if (launches) *launches = P->domLaunches;
+  P->domMs = P->domCells = 0.0;
+  P->domLaunches = 0;
+  return FS_OK;
+}
+
 int fs_pool_set_timing(fs_pool_t P, int32_t on) {
   if (!P) return FS_E_ARG;
   HIPCHK(P, hipStreamSynchronize(P->stream));
   P->nTev = 0;
+  P->spans.clear();
+  P->nSpanEv = 0;
+  P->nCycles = 0;
+  P->domMs = P->domCells = 0.0;
+  P->domLaunches = 0;
   P->timing = on != 0;
   P->ms1 = P->ms2 = 0.0;
   P->nTimed = 0;

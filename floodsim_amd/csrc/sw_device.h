@@ -10,6 +10,8 @@ constexpr int kLanes = 64;                   // wavefront width on CDNA
 constexpr int kHalo = 3;                     // theta limiter + MUSCL need 3 cells each side
 constexpr int kOwn = kLanes - 2 * kHalo;     // columns a wavefront strip owns (58)
 constexpr int kLtsTile = 8;                  // LTS level tiles (solver-cpu.js:14)
+constexpr int kMaxLevels = 4;                // LTS levels 1..4 (solver-cpu.js:121)
+constexpr double kLtsSafe = 0.9;             // share of its CFL step a tile may take (solver-cpu.js:15)
 
 enum { BC_WALL = 0, BC_OPEN = 1, BC_OUTFLOW = 2 };
 enum { MASK_OPEN = 1, MASK_INLET = 2 };      // mask byte bits: open (not a wall) cell, inlet cell
@@ -37,6 +39,15 @@ struct StepRec {
   int32_t openCells;
   int32_t levels;     // LTS levels (0 = global)
   int32_t dirty;      // the host changed the state: the next dt comes from a fresh speed reduction
+  // local time stepping: the current cycle (one "step" of an LTS instance is one cycle)
+  double cycT0;       // cycle start
+  double cycD;        // cycle length
+  double base;        // base step delta = D / 2^L
+  double lrf[kMaxLevels + 1];     // sum of rain multipliers over own open cells per level
+  float ldt[kMaxLevels + 1];      // record step per level: delta * 2^m (0 = no-op)
+  int32_t lcells[kMaxLevels + 1]; // own open cells per level
+  int32_t lcount[kMaxLevels + 1]; // work-list length per level
+  int32_t records;                // records run so far (diagnostic)
 };
 
 // one inlet: cell index (y*pitch + x), rate (m^3/s, + into the network, - a point inflow)
@@ -61,9 +72,25 @@ struct DevInst {
   int secondOrder, limiter;
   int bc[4];
   int wbeg, wend;               // this instance's range in the work list
-  Inlet *inl;            // this instance's inlets
+  Inlet *inl;                   // this instance's inlets
   float *ispeed;                // wave speed of each inlet cell after the exchange
-  int nin, pad;
+  int nin;
+  int hl, own;                  // strip geometry: halo lanes on the left, owned columns
+  // local time stepping (L > 0)
+  int L, tx, ty;                // levels; LTS tiles across and down (8x8 cells)
+  uint8_t *lv, *lv0;            // tile levels (smoothed, raw)
+  float *tsp;                   // tile max wave speed at the cycle start
+  const int32_t *tileOpen;      // open cells per tile
+  const float *tileRf;          // rain multiplier summed over the open cells of a tile (rain field only)
+  float *sh, *shu, *shv, *sw;   // flux a waiting cell has gathered (Kahan residual sw)
+  float *g0h, *g0hu, *g0hv;     // a record's first-stage gather of a waiting cell (folded into sh.. after)
+  float *g1h, *g1hu, *g1hv;     // the same, second stage
+  uint8_t *gflag;               // 1: the cell gathered in the current record
+  float4 *bsx, *bsy;            // a cell's reconstruction toward its faces at their last evaluation:
+                                // x (h west, dz west, h east, dz east), y (south, north); dz = face bed - cell bed
+  uint8_t *umask;               // per work unit: levels it takes part in
+  int32_t *ucells;              // per work unit and level: own open cells
+  int32_t *ulist;               // per level (stride wend - wbeg): work units of the level's records
 };
 
 // per wavefront strip and stage: boundary outflow (m^3), clipped (m^3), infiltration (m^3), max speed

@@ -130,10 +130,26 @@ __device__ __forceinline__ float out_vel(float he, float ze, float ne, float hi,
 // and outflow edges copy the edge cell, outflow sets the outward Manning velocity. Ghost beds
 // are the nearest interior bed (zero gradient), ghost cells are open. Corner ghosts never
 // enter an interior cell's stencil and only need to be finite.
-struct Prim { float h, u, v, z, hu, hv; bool M, in; };
+struct Prim { float h, u, v, z, hu, hv; bool M, in; int lv; };
 
-__device__ __forceinline__ Prim load_prim(const DevInst &I, const float *__restrict__ H, const float *__restrict__ HU,
-                                          const float *__restrict__ HV, int X, int Y) {
+// level of the LTS tile holding interior cell (x, y)
+__device__ __forceinline__ int tile_level(const DevInst &I, int x, int y) {
+  return I.lv[(y >> 3) * I.tx + (x >> 3)];
+}
+
+// state source of cell k: the stage-1 stencil of a local-time-stepping record reads U* (B) for the
+// cells of the stepping level and the frozen state (A) for every other cell
+template <int STAGE, bool LTS>
+__device__ __forceinline__ void cell_state(const DevInst &I, int x, int y, int m, float &h, float &hu, float &hv) {
+  const size_t k = (size_t)y * I.pitch + x;
+  bool useB = STAGE == 1;
+  if constexpr (LTS) useB = STAGE == 1 && tile_level(I, x, y) == m;
+  if (useB) { h = I.h1[k]; hu = I.hu1[k]; hv = I.hv1[k]; }
+  else { h = I.h[k]; hu = I.hu[k]; hv = I.hv[k]; }
+}
+
+template <int STAGE, bool LTS>
+__device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m) {
   const int nx = I.nx, ny = I.ny;
   int sx = X, sy = Y;
   float fu = 1.f, fv = 1.f;
@@ -147,12 +163,15 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, const float *__restr
   const int zx = min(max(X, 0), nx - 1), zy = min(max(Y, 0), ny - 1);
   const size_t k = (size_t)sy * I.pitch + sx;
   Prim p;
-  p.h = H[k];
-  p.hu = HU[k] * fu;
-  p.hv = HV[k] * fv;
+  float hu, hv;
+  cell_state<STAGE, LTS>(I, sx, sy, m, p.h, hu, hv);
+  p.hu = hu * fu;
+  p.hv = hv * fv;
   p.z = I.z[(size_t)zy * I.pitch + zx];
   p.in = !gx && !gy;
   p.M = p.in ? (I.mask[k] & MASK_OPEN) != 0 : true;
+  p.lv = 0;
+  if constexpr (LTS) p.lv = tile_level(I, zx, zy);  // a ghost takes its edge cell's level (solver-cpu.js:125-128)
   const float ih = frcp(p.h);
   const bool wet = p.h > I.hdry;
   p.u = wet ? p.hu * ih : 0.f;
@@ -165,11 +184,14 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, const float *__restr
     else if (Y >= ny && I.bc[3] == BC_OUTFLOW) { e = ny - 1; i = ny - 2; side = 3; }
     if (side >= 0) {
       i = max(i, 0);
-      size_t ke, ki;
-      if (side < 2) { ke = (size_t)zy * I.pitch + e; ki = (size_t)zy * I.pitch + min(i, nx - 1); }
-      else { ke = (size_t)e * I.pitch + zx; ki = (size_t)min(i, ny - 1) * I.pitch + zx; }
-      const float ov = out_vel(H[ke], I.z[ke], I.n[ke], H[ki], I.z[ki], (I.mask[ki] & MASK_OPEN) != 0, I.inv_dx,
-                               I.g, I.hdry);
+      int ex, ey, ix, iy;
+      if (side < 2) { ex = e; ey = zy; ix = min(i, nx - 1); iy = zy; }
+      else { ex = zx; ey = e; ix = zx; iy = min(i, ny - 1); }
+      const size_t ke = (size_t)ey * I.pitch + ex, ki = (size_t)iy * I.pitch + ix;
+      float he, hi, t0, t1;
+      cell_state<STAGE, LTS>(I, ex, ey, m, he, t0, t1);
+      cell_state<STAGE, LTS>(I, ix, iy, m, hi, t0, t1);
+      const float ov = out_vel(he, I.z[ke], I.n[ke], hi, I.z[ki], (I.mask[ki] & MASK_OPEN) != 0, I.inv_dx, I.g, I.hdry);
       if (side == 0) p.u = -ov;
       else if (side == 1) p.u = ov;
       else if (side == 2) p.v = -ov;
@@ -254,39 +276,30 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, size_t k, float d
   if (qh > hdry && !(I.mask[k] & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
 }
 
-struct FaceRow { float F0, F1, F2, cL, cR; };
+struct FaceRow { float F0, F1, F2, cL, cR, on; };
 
-// ------------------------------------------------------------------ one RK stage
+// One strip of one RK stage (see the file comment for the row pipeline).
 // STAGE 0: U* = U^n + dt L(U^n)                   (updateAt STAGE 0, kernels.wgsl.js:489-492)
 // STAGE 1: U^{n+1} = (U^n + U* + dt L(U*)) / 2, then friction, rain, infiltration, max depth,
 //          and the wave speed of the result for the next step's dt (kernels.wgsl.js:493-498, 544-590)
-// work[k] = (instance, first owned column, first owned row, rows)
-template <int STAGE>
-__global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
-                                                int nwork, Part *__restrict__ part) {
-  const int wid = xcd_wave_index(4);
-  if (wid >= nwork) return;
-  const int lane = threadIdx.x & 63;
-  const int4 wk = work[wid];
-  const DevInst &I = insts[wk.x];
-  if (STAGE == 1 && !I.secondOrder) return;  // first order: sw_euler_finish writes this strip's partial
-  const float dt = I.rec->dt;
-  if (!(dt > 0.f)) {
-    if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0.f};
-    return;
-  }
+// LTS: a record of level m (solver-cpu.js:529-558; kernels.wgsl.js:313-324, 504-537): only faces
+// whose lower-level cell is at m pass flux; cells at m step from their state plus their gathered
+// store; cells at m + 1 wait and gather the RK-weighted flux of those faces; the bed-slope term of a
+// stepping cell takes, for a face whose neighbour is finer, its reconstruction from that face's last
+// evaluation (kept in bsx/bsy by the finer record's last stage).
+template <int STAGE, bool LTS>
+__device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
+                                          Part *__restrict__ part) {
   const float rain = I.rec->rainRate;
   const float g = I.g, hdry = I.hdry, hfront = I.hfront, inv_dx = I.inv_dx, dx = I.dx;
   const int so = I.secondOrder, vl = I.limiter;
   const int nx = I.nx, ny = I.ny, pitch = I.pitch;
-  const int X = wk.y - kHalo + lane;
-  const bool own_col = lane >= kHalo && lane < kLanes - kHalo && X < nx;
+  const int X = wk.y - I.hl + lane;
+  const bool own_col = lane >= I.hl && lane < I.hl + I.own && X < nx;
   const int y0 = wk.z, y1 = min(wk.z + wk.w, ny);
+  const bool last = STAGE == 1 || !so;   // the stage that ends the step (or the record)
 
-  // evaluated state: U^n in stage 0, U* in stage 1
-  const float *__restrict__ Hs = STAGE == 0 ? I.h : I.h1;
-  const float *__restrict__ HUs = STAGE == 0 ? I.hu : I.hu1;
-  const float *__restrict__ HVs = STAGE == 0 ? I.hv : I.hv1;
+  // writes go to B in stage 0 and to A in stage 1; Ws is the residual of the evaluated state
   const float *__restrict__ Ws = STAGE == 0 ? I.w : I.w1;
   float *__restrict__ Ho = STAGE == 0 ? I.h1 : I.h;
   float *__restrict__ HUo = STAGE == 0 ? I.hu1 : I.hu;
@@ -298,31 +311,25 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
   const float areaf = dx * dx;
 
   // ---- register window
-  // prims of rows r-1, r-2
   float h1r = 0, u1r = 0, v1r = 0, z1r = 0, h2r = 0, u2r = 0, v2r = 0, z2r = 0;
   bool M1r = false, M2r = false;
-  // raw state of rows r-1, r-2, r-3 (for the update of row r-3)
+  int lv1 = 0, lv2 = 0, lv3 = 0;
   float hu1 = 0, hv1 = 0, hu2 = 0, hv2 = 0, hu3 = 0, hv3 = 0, hh3 = 0;
   bool M3 = false, in1 = false, in2 = false, in3 = false;
-  // y reconstruction of row r-2 toward north (+)
   float yp_h = 0, yp_e = 0, yp_n = 0, yp_t = 0;
-  // x faces (l | l+1) of rows r-1, r-2, r-3
-  FaceRow xf1 = {0, 0, 0, 0, 0}, xf2 = {0, 0, 0, 0, 0}, xf3 = {0, 0, 0, 0, 0};
-  // y faces (r-3 | r-2) and (r-4 | r-3)
-  FaceRow yf3 = {0, 0, 0, 0, 0}, yf4 = {0, 0, 0, 0, 0};
-  // theta of rows r-3, r-4
+  FaceRow xf1 = {0, 0, 0, 0, 0, 1}, xf2 = {0, 0, 0, 0, 0, 1}, xf3 = {0, 0, 0, 0, 0, 1};
+  FaceRow yf3 = {0, 0, 0, 0, 0, 1}, yf4 = {0, 0, 0, 0, 0, 1};
   float th3 = 1.f, th4 = 1.f;
-  // bed-slope source terms: x for rows r-1, r-2, r-3; y for rows r-2, r-3
   float sx1 = 0, sx2 = 0, sx3 = 0, sy2 = 0, sy3 = 0;
-  bool M1 = false, M2 = false;
-  (void)M1; (void)M2;
+  // LTS: the cell's own reconstruction toward each face (h, dz) for rows r-1..r-3 (x) and r-2..r-3 (y)
+  float4 rx1 = {0, 0, 0, 0}, rx2 = {0, 0, 0, 0}, rx3 = {0, 0, 0, 0}, ry2 = {0, 0, 0, 0}, ry3 = {0, 0, 0, 0};
 
   double acc_out = 0.0, acc_clip = 0.0, acc_inf = 0.0;
   float smax = 0.f;
 
   for (int r = y0 - kHalo; r < y1 + kHalo; ++r) {
     // ---------------- row r: primitives
-    const Prim p = load_prim(I, Hs, HUs, HVs, X, r);
+    const Prim p = load_prim<STAGE, LTS>(I, X, r, m);
 
     // ---------------- x reconstruction of row r (neighbours by lane shifts)
     const float hl = from_left(p.h), ul = from_left(p.u), vlf = from_left(p.v), zl = from_left(p.z);
@@ -334,12 +341,28 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     const float xu_p = p.u + 0.5f * sxs.sn, xv_p = p.v + 0.5f * sxs.st;
     const float xu_m = p.u - 0.5f * sxs.sn, xv_m = p.v - 0.5f * sxs.st;
     const float sx0 = hg * (xh_m + xh_p) * ((xe_p - xh_p) - (xe_m - xh_m));
+    const float4 rx0 = {xh_m, xe_m - xh_m, xh_p, xe_p - xh_p};
     // x face (l | l+1): right side is lane l+1's minus side
     FaceRow xf0;
+    int lvr = 0;
     {
       const float Rh = from_right(xh_m), Re = from_right(xe_m), Ru = from_right(xu_m), Rv = from_right(xv_m);
       const Face f = face(xh_p, xe_p, xu_p, xv_p, p.M, Rh, Re, Ru, Rv, Mr, zr - p.z, g, hdry);
-      xf0 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR};
+      xf0 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
+      if constexpr (LTS) {
+        lvr = from_right_i(p.lv);
+        xf0.on = min(p.lv, lvr) == m ? 1.f : 0.f;
+      }
+    }
+    if constexpr (LTS) {
+      // a waiting cell keeps its reconstruction toward the faces this record evaluates (last stage)
+      const float onW = from_left(xf0.on);
+      if (last && own_col && p.in && p.M && p.lv == m + 1 && r >= y0 && r < y1) {
+        const size_t k = (size_t)r * pitch + X;
+        float2 *b = reinterpret_cast<float2 *>(&I.bsx[k]);
+        if (onW > 0.f) b[0] = make_float2(rx0.x, rx0.y);
+        if (xf0.on > 0.f) b[1] = make_float2(rx0.z, rx0.w);
+      }
     }
 
     // ---------------- y reconstruction of row r-1 (neighbours: rows r-2 and r in registers)
@@ -349,20 +372,45 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     const float yn_p = v1r + 0.5f * sys.sn, yt_p = u1r + 0.5f * sys.st;
     const float yn_m = v1r - 0.5f * sys.sn, yt_m = u1r - 0.5f * sys.st;
     const float sy1 = hg * (yh_m + yh_p) * ((ye_p - yh_p) - (ye_m - yh_m));
+    const float4 ry1 = {yh_m, ye_m - yh_m, yh_p, ye_p - yh_p};
     // y face (r-2 | r-1): L = row r-2 toward north, R = row r-1 toward south
     FaceRow yf2;
     {
       const Face f = face(yp_h, yp_e, yp_n, yp_t, M2r, yh_m, ye_m, yn_m, yt_m, M1r, z1r - z2r, g, hdry);
-      yf2 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR};
+      yf2 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
+      if constexpr (LTS) yf2.on = min(lv2, lv1) == m ? 1.f : 0.f;
+    }
+    if constexpr (LTS) {
+      const int ry = r - 1;
+      const bool onN = min(lv1, p.lv) == m;
+      if (last && own_col && in1 && M1r && lv1 == m + 1 && ry >= y0 && ry < y1) {
+        const size_t k = (size_t)ry * pitch + X;
+        float2 *b = reinterpret_cast<float2 *>(&I.bsy[k]);
+        if (yf2.on > 0.f) b[0] = make_float2(ry1.x, ry1.y);
+        if (onN) b[1] = make_float2(ry1.z, ry1.w);
+      }
     }
 
     // ---------------- theta of row r-2 (solver-cpu.js:307-318; kernels.wgsl.js:349-363)
     float th2 = 1.f;
     {
-      const float fe = xf2.F0, fw = from_left(xf2.F0), fs = yf3.F0, fn = yf2.F0;
+      float fe = xf2.F0, fw = from_left(xf2.F0 * xf2.on), fs = yf3.F0, fn = yf2.F0;
+      bool take = in2 && M2r;
+      float hv_ = h2r;
+      if constexpr (LTS) {
+        fe *= xf2.on; fs *= yf3.on; fn *= yf2.on;
+        take = take && (lv2 == m || lv2 == m + 1);
+        // a waiting cell, or a stepping one in its first stage, holds its water plus its store
+        if (take && (lv2 == m + 1 || STAGE == 0)) {
+          const size_t k2 = (size_t)(r - 2) * pitch + X;
+          float st = I.sh[k2] - I.sw[k2];
+          if (STAGE == 1 && I.gflag[k2]) st += I.g0h[k2];  // the first stage's gather, not yet folded
+          hv_ = fmaxf(0.f, h2r + st);
+        }
+      }
       const float out = ((fe > 0.f ? fe : 0.f) - (fw < 0.f ? fw : 0.f) + (fn > 0.f ? fn : 0.f) - (fs < 0.f ? fs : 0.f)) * dx * dt;
-      const float vol = h2r * areaf;
-      if (in2 && M2r && out > vol) th2 = vol / out;
+      const float vol = hv_ * areaf;
+      if (take && out > vol) th2 = vol / out;
     }
 
     // ---------------- update of row r-3
@@ -371,6 +419,8 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
       // west face (l-1 | l) from lane l-1; east face (l | l+1) own
       const float Fw0 = from_left(xf3.F0), Fw1 = from_left(xf3.F1), Fw2 = from_left(xf3.F2), cRw = from_left(xf3.cR);
       const float thW = from_left(th3), thE = from_right(th3);
+      float kw = 1.f, ke = 1.f, ks = 1.f, kn = 1.f;
+      if constexpr (LTS) { kw = from_left(xf3.on); ke = xf3.on; ks = yf4.on; kn = yf3.on; }
       float Fe0 = xf3.F0, Fe1 = xf3.F1, Fe2 = xf3.F2;
       const float cLe = xf3.cL;
       float Fs0 = yf4.F0, Fs1 = yf4.F1, Fs2 = yf4.F2;
@@ -386,35 +436,90 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
       Fe0 *= te; Fe1 *= te; Fe2 *= te;
       Fs0 *= ts; Fs1 *= ts; Fs2 *= ts;
       Fn0 *= tn; Fn1 *= tn; Fn2 *= tn;
-      if (own_col && ry >= y0 && ry < y1 && in3 && M3) {
-        const float dh = -inv_dx * (Fe0 - w0 + Fn0 - Fs0);
-        const float dhu = -inv_dx * ((Fe1 + cLe) - (w1 + cRw) + sx3 + Fn2 - Fs2);
-        const float dhv = -inv_dx * ((Fn1 + cLn) - (Fs1 + cRs) + sy3 + Fe2 - w2);
+      bool upd = own_col && ry >= y0 && ry < y1 && in3 && M3;
+      bool own = true, coarse = false;
+      if constexpr (LTS) {
+        own = lv3 == m;
+        coarse = lv3 == m + 1;
+        upd = upd && (own || coarse);
+      }
+      if (upd) {
         const size_t k = (size_t)ry * pitch + X;
-        float qh = hh3, qhu = hu3, qhv = hv3, qw = Ws[k];
-        if (STAGE == 0) {
-          qhu = qhu + dt * dhu;
-          qhv = qhv + dt * dhv;
-          kadd(qh, qw, dt * dh);
+        float bx = sx3, by = sy3;
+        if constexpr (LTS) {
+          if (own) {
+            // bed slope from the face reconstructions (solver-cpu.js:363-368): a face with a finer
+            // neighbour contributes the values of its last evaluation
+            float4 xs = rx3, ys = ry3;
+            if (kw == 0.f || ke == 0.f) {
+              const float4 b = I.bsx[k];
+              if (kw == 0.f) { xs.x = b.x; xs.y = b.y; }
+              if (ke == 0.f) { xs.z = b.z; xs.w = b.w; }
+            }
+            if (ks == 0.f || kn == 0.f) {
+              const float4 b = I.bsy[k];
+              if (ks == 0.f) { ys.x = b.x; ys.y = b.y; }
+              if (kn == 0.f) { ys.z = b.z; ys.w = b.w; }
+            }
+            bx = hg * (xs.x + xs.z) * (xs.w - xs.y);
+            by = hg * (ys.x + ys.z) * (ys.w - ys.y);
+          } else {
+            bx = 0.f;
+            by = 0.f;
+          }
+        }
+        const float dh = -inv_dx * (ke * Fe0 - kw * w0 + kn * Fn0 - ks * Fs0);
+        const float dhu = -inv_dx * (ke * (Fe1 + cLe) - kw * (w1 + cRw) + bx + kn * Fn2 - ks * Fs2);
+        const float dhv = -inv_dx * (kn * (Fn1 + cLn) - ks * (Fs1 + cRs) + by + ke * Fe2 - kw * w2);
+        if (LTS && coarse) {
+          // waiting cell: gather the RK-weighted change (ltsUpdate mode 2, kernels.wgsl.js:515-519)
+          // (into per-stage deltas: a neighbouring strip may read this cell's store for theta
+          // during this kernel; sw_lts_fold adds the deltas once the record's stages are done)
+          if (kw + ke + ks + kn > 0.f) {
+            const float wd = so ? 0.5f * dt : dt;
+            if (STAGE == 0) { I.g0h[k] = wd * dh; I.g0hu[k] = wd * dhu; I.g0hv[k] = wd * dhv; I.gflag[k] = 1; }
+            else { I.g1h[k] = wd * dh; I.g1hu[k] = wd * dhu; I.g1hv[k] = wd * dhv; }
+          }
         } else {
-          const float h0 = I.h[k], hu0 = I.hu[k], hv0 = I.hv[k], w0n = I.w[k];
-          qhu = 0.5f * (hu0 + qhu + dt * dhu);
-          qhv = 0.5f * (hv0 + qhv + dt * dhv);
-          kadd(qh, qw, 0.5f * ((h0 - w0n) - (qh - qw) + dt * dh));
+          float qh = hh3, qhu = hu3, qhv = hv3, qw = Ws[k];
+          // RK start state: U^n, plus (LTS) what finer neighbours passed while the cell waited
+          float h0 = 0.f, w0n = 0.f, hu0 = 0.f, hv0 = 0.f;
+          if (STAGE == 1 || LTS) {
+            h0 = I.h[k]; w0n = I.w[k]; hu0 = I.hu[k]; hv0 = I.hv[k];
+            if constexpr (LTS) {
+              const float sh = I.sh[k], sw = I.sw[k];
+              hu0 += I.shu[k];
+              hv0 += I.shv[k];
+              kadd(h0, w0n, sh - sw);
+            }
+          }
+          if (STAGE == 0) {
+            if constexpr (LTS) { qh = h0; qw = w0n; qhu = hu0; qhv = hv0; }
+            qhu = qhu + dt * dhu;
+            qhv = qhv + dt * dhv;
+            kadd(qh, qw, dt * dh);
+          } else {
+            qhu = 0.5f * (hu0 + qhu + dt * dhu);
+            qhv = 0.5f * (hv0 + qhv + dt * dhv);
+            kadd(qh, qw, 0.5f * ((h0 - w0n) - (qh - qw) + dt * dh));
+          }
+          if (qh < 0.f) { acc_clip += -(double)(qh - qw) * areaf; qh = 0.f; qw = 0.f; }
+          if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
+          // boundary outflow through the domain edge faces (edgeOutAt, kernels.wgsl.js:1020-1030)
+          {
+            double o = 0.0;
+            if (X == 0) o -= (double)w0;
+            if (X == nx - 1) o += (double)Fe0;
+            if (ry == 0) o -= (double)Fs0;
+            if (ry == ny - 1) o += (double)Fn0;
+            acc_out += o * ((double)dx * (double)dt * (double)wgt);
+          }
+          if (STAGE == 1) {
+            sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+            if constexpr (LTS) { I.sh[k] = 0.f; I.shu[k] = 0.f; I.shv[k] = 0.f; I.sw[k] = 0.f; }
+          }
+          Ho[k] = qh; HUo[k] = qhu; HVo[k] = qhv; Wo[k] = qw;
         }
-        if (qh < 0.f) { acc_clip += -(double)(qh - qw) * areaf; qh = 0.f; qw = 0.f; }
-        if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
-        // boundary outflow through the domain edge faces (edgeOutAt, kernels.wgsl.js:1020-1030)
-        {
-          double o = 0.0;
-          if (X == 0) o -= (double)w0;
-          if (X == nx - 1) o += (double)Fe0;
-          if (ry == 0) o -= (double)Fs0;
-          if (ry == ny - 1) o += (double)Fn0;
-          acc_out += o * ((double)dx * (double)dt * (double)wgt);
-        }
-        if (STAGE == 1) sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
-        Ho[k] = qh; HUo[k] = qhu; HVo[k] = qhv; Wo[k] = qw;
       }
     }
 
@@ -424,6 +529,7 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     xf3 = xf2; xf2 = xf1; xf1 = xf0;
     sx3 = sx2; sx2 = sx1; sx1 = sx0;
     sy3 = sy2; sy2 = sy1;
+    if constexpr (LTS) { rx3 = rx2; rx2 = rx1; rx1 = rx0; ry3 = ry2; ry2 = ry1; lv3 = lv2; lv2 = lv1; lv1 = p.lv; }
     yp_h = yh_p; yp_e = ye_p; yp_n = yn_p; yp_t = yt_p;
     hh3 = h2r; hu3 = hu2; hv3 = hv2; M3 = M2r; in3 = in2;
     h2r = h1r; u2r = u1r; v2r = v1r; z2r = z1r; M2r = M1r; hu2 = hu1; hv2 = hv1; in2 = in1;
@@ -432,7 +538,286 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
 
   const double o = wave_sum(acc_out), c = wave_sum(acc_clip), f = wave_sum(acc_inf);
   const float s = wave_max(smax);
-  if (lane == 0) part[wid] = Part{o, c, f, s, 0.f};
+  if (lane == 0) *part = Part{o, c, f, s, 0.f};
+}
+
+// One RK stage over a work list. Global stepping: one wavefront per strip of `work`.
+// LTS record of level m (single-instance groups): persistent wavefronts walk the level's list,
+// each a contiguous chunk (neighbouring strips stay on one XCD).
+template <int STAGE, bool LTS>
+__global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
+                                                int nwork, Part *__restrict__ part, int m) {
+  const int wid = xcd_wave_index(4);
+  const int lane = threadIdx.x & 63;
+  if constexpr (!LTS) {
+    if (wid >= nwork) return;
+    const int4 wk = work[wid];
+    const DevInst &I = insts[wk.x];
+    if (STAGE == 1 && !I.secondOrder) return;  // first order: sw_euler_finish writes this strip's partial
+    const float dt = I.rec->dt;
+    if (!(dt > 0.f)) {
+      if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0.f};
+      return;
+    }
+    run_strip<STAGE, false>(I, wk, lane, 0, dt, part + wid);
+  } else {
+    const DevInst &I = insts[0];
+    if (STAGE == 1 && !I.secondOrder) return;
+    const float dt = I.rec->ldt[m];
+    const int n = I.rec->lcount[m];
+    const int nw = gridDim.x * 4;
+    const int chunk = (n + nw - 1) / nw;
+    const int i0 = wid * chunk, i1 = min(n, i0 + chunk);
+    const int nu = I.wend - I.wbeg;
+    for (int i = i0; i < i1; ++i) {
+      if (!(dt > 0.f)) {
+        if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0.f};
+        continue;
+      }
+      const int u = I.ulist[m * nu + i];
+      run_strip<STAGE, true>(I, work[I.wbeg + u], lane, m, dt, part + i);
+    }
+  }
+}
+
+// ------------------------------------------------------------ local time stepping, per record
+// after the stages of a level-m record: a waiting cell's store takes the record's gathered flux
+// (compensated, as kadd on U0 in ltsUpdate, kernels.wgsl.js:518)
+__global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
+                                                   int m) {
+  const DevInst &I = insts[0];
+  const int wid = xcd_wave_index(4), lane = threadIdx.x & 63;
+  const int n = I.rec->lcount[m], nw = gridDim.x * 4, chunk = (n + nw - 1) / nw;
+  const int nu = I.wend - I.wbeg;
+  const bool two = I.secondOrder;
+  for (int i = wid * chunk; i < min(n, wid * chunk + chunk); ++i) {
+    const int4 wk = work[I.wbeg + I.ulist[m * nu + i]];
+    const int X = wk.y - I.hl + lane;
+    if (lane < I.hl || lane >= I.hl + I.own || X >= I.nx) continue;
+    for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
+      const size_t k = (size_t)y * I.pitch + X;
+      if (!I.gflag[k]) continue;
+      float sh = I.sh[k], sw = I.sw[k];
+      kadd(sh, sw, I.g0h[k]);
+      float shu = I.shu[k] + I.g0hu[k], shv = I.shv[k] + I.g0hv[k];
+      if (two) {
+        kadd(sh, sw, I.g1h[k]);
+        shu += I.g1hu[k];
+        shv += I.g1hv[k];
+      }
+      I.sh[k] = sh; I.sw[k] = sw; I.shu[k] = shu; I.shv[k] = shv;
+      I.gflag[k] = 0;
+    }
+  }
+}
+
+// ------------------------------------------------------------ local time stepping, per cycle
+// (_cycle, solver-cpu.js:504-527; ltsCycle/ltsLevel/ltsSmooth/ltsList, kernels.wgsl.js:677-741)
+
+// max wave speed per 8x8 tile at the cycle start: one wavefront per tile
+__global__ __launch_bounds__(256) void sw_lts_speed(const DevInst *__restrict__ insts) {
+  const DevInst &I = insts[0];
+  const int t = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  if (t >= I.tx * I.ty) return;
+  const int lane = threadIdx.x & 63;
+  const int x = (t % I.tx) * kLtsTile + (lane & 7), y = (t / I.tx) * kLtsTile + (lane >> 3);
+  float s = 0.f;
+  if (x < I.nx && y < I.ny) {
+    const size_t k = (size_t)y * I.pitch + x;
+    const float h = I.h[k];
+    if ((I.mask[k] & MASK_OPEN) && h > I.hdry) s = cell_speed(h, I.hu[k], I.hv[k], I.g);
+  }
+  s = wave_max(s);
+  if (lane == 0) I.tsp[t] = s;
+}
+
+// cycle length D = min(2^L 0.9 cfl dx / smax (dtDry when dry), tEnd - t), base step D / 2^L
+__global__ __launch_bounds__(1024) void sw_lts_cycle(const DevInst *__restrict__ insts) {
+  const DevInst &I = insts[0];
+  StepRec *rec = I.rec;
+  __shared__ float s_m[16];
+  float m = 0.f;
+  const int nt = I.tx * I.ty;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) m = fmaxf(m, I.tsp[t]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fmaxf(m, s_m[k]);
+  const int P = 1 << I.L;
+  const double t0 = rec->t, rem = rec->tEnd - t0;
+  double D = 0.0, after = t0;
+  if (rem > 0.0) {
+    const double cand = m > 0.f ? (double)P * kLtsSafe * (double)I.cfl * (double)I.dx / (double)m : (double)I.dtDry;
+    if (cand >= rem) { D = rem; after = rec->tEnd; }
+    else { D = cand; after = t0 + cand; }
+  }
+  rec->smax = m;
+  rec->cycT0 = t0;
+  rec->cycD = D;
+  rec->base = D / P;
+  rec->tAfter = after;
+  for (int l = 0; l <= kMaxLevels; ++l) rec->ldt[l] = l <= I.L ? (float)(rec->base * (double)(1 << l)) : 0.f;
+}
+
+// tile level: the largest m <= L whose step 2^m delta is within 0.9 of the CFL step of the fastest
+// water in the tile and its 8 neighbours (all dry: L)
+__global__ __launch_bounds__(256) void sw_lts_level(const DevInst *__restrict__ insts) {
+  const DevInst &I = insts[0];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= I.tx * I.ty) return;
+  const int x = t % I.tx, y = t / I.tx;
+  float s = 0.f;
+  for (int b = max(0, y - 1); b <= min(I.ty - 1, y + 1); ++b)
+    for (int a = max(0, x - 1); a <= min(I.tx - 1, x + 1); ++a) s = fmaxf(s, I.tsp[b * I.tx + a]);
+  const double dl = I.rec->base;
+  int lv = I.L;
+  if (s > 0.f && dl > 0.0) {
+    const double q = floor(log2(kLtsSafe * (double)I.cfl * (double)I.dx / (double)s / dl));
+    lv = (int)fmax(0.0, fmin((double)I.L, q));
+  }
+  I.lv0[t] = (uint8_t)lv;
+}
+
+// no tile more than one level above a face neighbour: the reference applies L sweeps of
+// lv = min(lv, neighbour + 1); with levels in [0, L] that is exactly min over tiles within
+// Manhattan distance L of (raw level + distance)
+__global__ __launch_bounds__(256) void sw_lts_smooth(const DevInst *__restrict__ insts) {
+  const DevInst &I = insts[0];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= I.tx * I.ty) return;
+  const int x = t % I.tx, y = t / I.tx, L = I.L;
+  int lv = I.lv0[t];
+  for (int dy = -L; dy <= L; ++dy) {
+    const int b = y + dy;
+    if (b < 0 || b >= I.ty) continue;
+    const int r = L - abs(dy);
+    for (int dx = -r; dx <= r; ++dx) {
+      const int a = x + dx;
+      if (a < 0 || a >= I.tx) continue;
+      lv = min(lv, (int)I.lv0[b * I.tx + a] + abs(dx) + abs(dy));
+    }
+  }
+  I.lv[t] = (uint8_t)lv;
+}
+
+// per work unit (whole tiles): the levels whose records it takes part in -- its own tiles' levels,
+// and level l - 1 for a tile at l with a face neighbour at l - 1 (it waits and gathers) -- and its
+// own open cells per level
+__global__ __launch_bounds__(256) void sw_lts_units(const DevInst *__restrict__ insts, const int4 *__restrict__ work) {
+  const DevInst &I = insts[0];
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nu = I.wend - I.wbeg;
+  if (u >= nu) return;
+  const int4 wk = work[I.wbeg + u];
+  const int tx0 = wk.y / kLtsTile, tx1 = min(I.tx, (wk.y + I.own) / kLtsTile);
+  const int ty0 = wk.z / kLtsTile, ty1 = min(I.ty, (wk.z + wk.w + kLtsTile - 1) / kLtsTile);
+  int cells[kMaxLevels + 1] = {0, 0, 0, 0, 0};
+  unsigned mask = 0;
+  for (int b = ty0; b < ty1; ++b)
+    for (int a = tx0; a < tx1; ++a) {
+      const int l = I.lv[b * I.tx + a];
+      mask |= 1u << l;
+      cells[l] += I.tileOpen[b * I.tx + a];
+      if (l > 0) {
+        const bool fine = (a > 0 && I.lv[b * I.tx + a - 1] == l - 1) || (a + 1 < I.tx && I.lv[b * I.tx + a + 1] == l - 1) ||
+                          (b > 0 && I.lv[(b - 1) * I.tx + a] == l - 1) || (b + 1 < I.ty && I.lv[(b + 1) * I.tx + a] == l - 1);
+        if (fine) mask |= 1u << (l - 1);
+      }
+    }
+  I.umask[u] = (uint8_t)mask;
+  for (int l = 0; l <= kMaxLevels; ++l) I.ucells[u * (kMaxLevels + 1) + l] = cells[l];
+}
+
+// per level: the list of units in unit order (a block-wide scan: deterministic), the own open
+// cells, and (rain field) the rain multiplier summed over them
+__global__ __launch_bounds__(1024) void sw_lts_lists(const DevInst *__restrict__ insts) {
+  const DevInst &I = insts[0];
+  const int nu = I.wend - I.wbeg, T = blockDim.x, tid = threadIdx.x;
+  __shared__ int s_scan[1024];
+  __shared__ long long s_cells[kMaxLevels + 1];
+  const int chunk = (nu + T - 1) / T, u0 = tid * chunk, u1 = min(nu, u0 + chunk);
+  if (tid <= kMaxLevels) s_cells[tid] = 0;
+  __syncthreads();
+  for (int l = 0; l <= I.L; ++l) {
+    int c = 0;
+    long long cl = 0;
+    for (int u = u0; u < u1; ++u) {
+      c += (I.umask[u] >> l) & 1;
+      cl += I.ucells[u * (kMaxLevels + 1) + l];
+    }
+    s_scan[tid] = c;
+    __syncthreads();
+    for (int o = 1; o < T; o <<= 1) {  // inclusive scan
+      const int v = tid >= o ? s_scan[tid - o] : 0;
+      __syncthreads();
+      s_scan[tid] += v;
+      __syncthreads();
+    }
+    int pos = s_scan[tid] - c;
+    for (int u = u0; u < u1; ++u)
+      if ((I.umask[u] >> l) & 1) I.ulist[l * nu + pos++] = u;
+    atomicAdd((unsigned long long *)&s_cells[l], (unsigned long long)cl);
+    __syncthreads();
+    if (tid == 0) {
+      I.rec->lcount[l] = s_scan[T - 1];
+      I.rec->lcells[l] = (int)s_cells[l];
+    }
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int l = I.L + 1; l <= kMaxLevels; ++l) { I.rec->lcount[l] = 0; I.rec->lcells[l] = 0; }
+  if (I.tileRf && tid <= I.L) {  // rain multiplier per level (only with a rain field)
+    double acc = 0.0;
+    for (int t = 0; t < I.tx * I.ty; ++t)
+      if (I.lv[t] == tid) acc += (double)I.tileRf[t];
+    I.rec->lrf[tid] = acc;
+  }
+}
+
+// end of a level-m record: books in a fixed order (as sw_dt_final)
+__global__ __launch_bounds__(256) void sw_lts_record_end(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
+                                                        const Part *__restrict__ p2, int m) {
+  const DevInst &I = insts[0];
+  StepRec *rec = I.rec;
+  const float dt = rec->ldt[m];
+  if (!(dt > 0.f)) return;
+  const int n = rec->lcount[m];
+  __shared__ double s_o[4], s_c[4], s_f[4];
+  double o = 0.0, c = 0.0, f = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const Part a = p1[k];
+    o += a.out;
+    c += a.clip;
+    if (I.secondOrder) {
+      const Part b = p2[k];
+      o += b.out;
+      c += b.clip;
+      f += b.infil;
+    } else f += p2[k].infil;
+  }
+  o = wave_sum(o); c = wave_sum(c); f = wave_sum(f);
+  if ((threadIdx.x & 63) == 0) { s_o[threadIdx.x >> 6] = o; s_c[threadIdx.x >> 6] = c; s_f[threadIdx.x >> 6] = f; }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) { o += s_o[k]; c += s_c[k]; f += s_f[k]; }
+  rec->outflow += o;
+  rec->clipped += c;
+  rec->infil += f;
+  if (rec->rainCells >= 0) rec->rain += (double)(rec->rainRate * dt) * (double)rec->lcells[m] * (double)I.area;
+  else rec->rain += (double)rec->rainRate * (double)dt * rec->lrf[m] * (double)I.area;
+  rec->cellUpdates += (double)rec->lcells[m];
+  rec->records += 1;
+  rec->lastDt = dt;
+}
+
+// end of a cycle: the clock (solver-cpu.js:525)
+__global__ void sw_lts_cycle_end(const DevInst *__restrict__ insts) {
+  StepRec *rec = insts[0].rec;
+  if (rec->cycD > 0.0) {
+    rec->t = rec->tAfter;
+    rec->steps += 1;
+  }
 }
 
 // max wave speed |u| + |v| + 2 sqrt(g h) over wet open cells (dtReduceAt, kernels.wgsl.js:767-780);
@@ -444,9 +829,9 @@ __global__ __launch_bounds__(256) void sw_speed(const DevInst *__restrict__ inst
   const int lane = threadIdx.x & 63;
   const int4 wk = work[wid];
   const DevInst &I = insts[wk.x];
-  const int X = wk.y - kHalo + lane;
+  const int X = wk.y - I.hl + lane;
   float smax = 0.f;
-  if (lane >= kHalo && lane < kLanes - kHalo && X < I.nx) {
+  if (lane >= I.hl && lane < I.hl + I.own && X < I.nx) {
     for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
       const size_t k = (size_t)y * I.pitch + X;
       const float h = I.h[k];
@@ -458,45 +843,68 @@ __global__ __launch_bounds__(256) void sw_speed(const DevInst *__restrict__ inst
 }
 
 // first order (secondOrder = false): the Euler stage's U* plus the sources becomes U^{n+1}
-// (step, solver-cpu.js:432-438: one stage, then _sources)
-__global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict__ insts,
-                                                       const int4 *__restrict__ work, int nwork,
-                                                       Part *__restrict__ part) {
-  const int wid = xcd_wave_index(4);
-  if (wid >= nwork) return;
-  const int lane = threadIdx.x & 63;
-  const int4 wk = work[wid];
-  const DevInst &I = insts[wk.x];
-  if (I.secondOrder) return;  // second order: sw_stage<1> writes this strip's partial
-  const float dt = I.rec->dt;
+// (step, solver-cpu.js:432-438: one stage, then _sources). LTS: the level-m cells of a record,
+// whose store is then spent (solver-cpu.js:550-557).
+template <bool LTS>
+__device__ __forceinline__ void euler_finish_strip(const DevInst &I, const int4 wk, int lane, int m, float dt, Part *part) {
   double acc_inf = 0.0;
   float smax = 0.f;
-  const int X = wk.y - kHalo + lane;
-  if (dt > 0.f && lane >= kHalo && lane < kLanes - kHalo && X < I.nx) {
+  const int X = wk.y - I.hl + lane;
+  if (dt > 0.f && lane >= I.hl && lane < I.hl + I.own && X < I.nx) {
     const float rain = I.rec->rainRate, areaf = I.dx * I.dx;
     for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
       const size_t k = (size_t)y * I.pitch + X;
       if (!(I.mask[k] & MASK_OPEN)) continue;
+      if (LTS && tile_level(I, X, y) != m) continue;
       float qh = I.h1[k], qw = I.w1[k], qhu = I.hu1[k], qhv = I.hv1[k];
       sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
       I.h[k] = qh; I.w[k] = qw; I.hu[k] = qhu; I.hv[k] = qhv;
+      if (LTS) { I.sh[k] = 0.f; I.shu[k] = 0.f; I.shv[k] = 0.f; I.sw[k] = 0.f; }
     }
   }
   const double f = wave_sum(acc_inf);
   const float s = wave_max(smax);
-  if (lane == 0) part[wid] = Part{0.0, 0.0, f, s, 0.f};
+  if (lane == 0) *part = Part{0.0, 0.0, f, s, 0.f};
+}
+
+template <bool LTS>
+__global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict__ insts,
+                                                       const int4 *__restrict__ work, int nwork,
+                                                       Part *__restrict__ part, int m) {
+  const int wid = xcd_wave_index(4);
+  const int lane = threadIdx.x & 63;
+  if constexpr (!LTS) {
+    if (wid >= nwork) return;
+    const int4 wk = work[wid];
+    const DevInst &I = insts[wk.x];
+    if (I.secondOrder) return;  // second order: sw_stage<1> writes this strip's partial
+    euler_finish_strip<false>(I, wk, lane, 0, I.rec->dt, part + wid);
+  } else {
+    const DevInst &I = insts[0];
+    if (I.secondOrder) return;
+    const float dt = I.rec->ldt[m];
+    const int n = I.rec->lcount[m], nw = gridDim.x * 4, chunk = (n + nw - 1) / nw, nu = I.wend - I.wbeg;
+    for (int i = wid * chunk; i < min(n, wid * chunk + chunk); ++i)
+      euler_finish_strip<true>(I, work[I.wbeg + I.ulist[m * nu + i]], lane, m, dt, part + i);
+  }
 }
 
 // inlet exchange after the sources (exchangeAt, kernels.wgsl.js:597-618): the held rate over the
 // step, never more water than the cell holds; negative rates are point inflows. Writes the wave
 // speed of the inlet cell for the next dt (the second stage leaves inlet cells out of its maximum).
-__global__ void sw_exchange(const DevInst *__restrict__ insts, int inst) {
+// LTS: inlets whose cell is at the record's level, with its step (exchangeL, kernels.wgsl.js:596).
+template <bool LTS>
+__global__ void sw_exchange(const DevInst *__restrict__ insts, int inst, int m) {
   const DevInst &I = insts[inst];
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= I.nin) return;
-  Inlet &q = I.inl[m];
-  const float dt = I.rec->dt;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= I.nin) return;
+  Inlet &q = I.inl[j];
+  float dt = I.rec->dt;
   const size_t k = (size_t)q.cell;
+  if constexpr (LTS) {
+    if (tile_level(I, (int)(k % I.pitch), (int)(k / I.pitch)) != m) return;
+    dt = I.rec->ldt[m];
+  }
   float h = I.h[k], w = I.w[k], hu = I.hu[k], hv = I.hv[k];
   if (dt > 0.f) {
     const float A = I.dx * I.dx;
@@ -507,7 +915,7 @@ __global__ void sw_exchange(const DevInst *__restrict__ insts, int inst) {
     I.h[k] = h; I.w[k] = w; I.hu[k] = hu; I.hv[k] = hv;
     q.vol += (double)dd * (double)A;
   }
-  I.ispeed[m] = h > I.hdry ? cell_speed(h, hu, hv, I.g) : 0.f;
+  I.ispeed[j] = h > I.hdry ? cell_speed(h, hu, hv, I.g) : 0.f;
 }
 
 // End of a step, one block per instance (dtFinal, kernels.wgsl.js:783-799): fold the stage

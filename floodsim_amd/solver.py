@@ -53,6 +53,7 @@ EXPORTS = [
     "fs_pool_add", "fs_pool_remove", "fs_pool_set_state", "fs_pool_set_rain", "fs_pool_set_inlet_rates",
     "fs_pool_advance", "fs_pool_step", "fs_pool_read_state", "fs_pool_clock", "fs_pool_bytes_used",
     "fs_pool_kernel_ms", "fs_pool_set_timing", "fs_solver_pool", "fs_version", "fs_copy_probe",
+    "fs_pool_dominant_kernel",
 ]
 
 
@@ -107,6 +108,7 @@ def lib(path=None):
     L.fs_pool_kernel_ms.argtypes = [P, dp, dp, ctypes.POINTER(ctypes.c_int64)]
     L.fs_pool_set_timing.argtypes = [P, I32]
     L.fs_copy_probe.argtypes = [SZ, I32, dp, dp]
+    L.fs_pool_dominant_kernel.argtypes = [P, dp, dp, ctypes.POINTER(ctypes.c_int64)]
     L.fs_solver_pool.restype = P
     L.fs_solver_pool.argtypes = [P]
     _LIB = L
@@ -308,6 +310,13 @@ class GPUSolver:
     def kernel_ms(self):
         a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
         self._chk(lib().fs_pool_kernel_ms(lib().fs_solver_pool(self._h), ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
+
+    def dominant_kernel(self):
+        """(total ms, open cells advanced, launches) of the second-stage kernel since the last call."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        self._chk(lib().fs_pool_dominant_kernel(lib().fs_solver_pool(self._h), ctypes.byref(a), ctypes.byref(b),
+                                                ctypes.byref(n)))
         return a.value, b.value, n.value
 
 

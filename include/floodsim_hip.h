@@ -118,6 +118,10 @@ size_t fs_pool_bytes_used(fs_pool_t pool);
  * last call, measured with HIP events on the pool's stream; -1 when not measured */
 int fs_pool_kernel_ms(fs_pool_t pool, double *stage1_ms, double *stage2_ms, int64_t *launches);
 int fs_pool_set_timing(fs_pool_t pool, int32_t on);
+/* timing mode: total ms of the dominant kernel's launches (the second RK stage with the sources;
+ * under LTS one launch per record), the open cells those launches advanced, and the launch count,
+ * since the last call (no counterpart in the reference: the roofline measurement) */
+int fs_pool_dominant_kernel(fs_pool_t pool, double *ms_total, double *cells_total, int64_t *launches);
 /* the underlying pool of a single solver (for timing and tuning calls) */
 fs_pool_t fs_solver_pool(fs_handle s);
 

@@ -1,0 +1,93 @@
+"""GPU local time stepping against the FP64 oracle's LTS (a C restatement of solver-cpu.js
+_cycle/_record, itself pinned by tests/test_oracle.py). Cases follow tests/run-lts-tests.mjs and the
+reference GPU suite's LTS section (results-gpu.json `lts`: cycles, vsCPU 2.9e-7 m on the dam breaks,
+3.9e-5 m on the 256x192 channel)."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import refcases as rc
+
+pytestmark = pytest.mark.gpu
+TOL_H = 1e-3
+TOL_MASS = 1e-6
+
+
+@pytest.fixture(scope="module")
+def fsa():
+    import floodsim_amd
+    floodsim_amd.lib()
+    return floodsim_amd
+
+
+def pair(fsa, opts, init, tEnd, rain=0.0, lts=3, frame=None, **gopts):
+    g = fsa.GPUSolver(**opts, lts=lts, **gopts)
+    c = oracle.CPUSolver(**opts, lts=lts, hdry=g.hdry)
+    init(g)
+    init(c)
+    if rain:
+        g.rainRate = rain
+        c.rainRate = rain
+    # frames cut the cycles as the reference app does (10 s quanta, app/map.js:2281)
+    for t in (np.arange(frame, tEnd + 1e-9, frame) if frame else [tEnd]):
+        g.advance(float(t), 8)
+        c.run(float(t))
+    s = g.readState()
+    d = float(np.max(np.abs(s["h"] - c.field("h"))))
+    clk = g.clock()
+    print(f"\nLTS: max|dh| {d:.2e} m, cycles gpu/cpu {clk.steps}/{c.steps}, cell updates gpu/cpu "
+          f"{clk.cellUpdates}/{c.cellSteps}, mass rel {s['mass']['rel']:.2e}")
+    return g, c, s, d
+
+
+@pytest.mark.parametrize("name,hR", [("stoker", 0.5), ("ritter", 0.0)])
+def test_lts_dam_break(fsa, golden, name, hR):
+    nx = 400
+    opts = dict(nx=nx, ny=3, dx=100 / nx, z=np.zeros(nx * 3), manning=0)
+    g, c, s, d = pair(fsa, opts, lambda o: o.setDepth(lambda X, Y, Z: np.where(X < 50, 1.0, hR)), 6)
+    assert d <= TOL_H
+    assert abs(s["mass"]["rel"]) <= TOL_MASS
+    assert g.steps == c.steps == golden["gpu"]["lts"][name]["cycles"]
+    assert g.clock().cellUpdates == c.cellSteps
+
+
+def channel_opts(still=False):
+    nx, ny = 256, 192
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = np.where((J >= 90) & (J < 102), -7.0, 0.3 - 0.002 * I + (0 if still else 0.01 * np.sin(J / 5)))
+    return dict(nx=nx, ny=ny, dx=1, z=z, manning=0.03, bc={} if still else {"east": "outflow"})
+
+
+def test_lts_still_water(fsa):
+    """run-lts-tests.mjs:51-58: still water over a deep channel beside a shallow floodplain stays still."""
+    g = fsa.GPUSolver(**channel_opts(True), lts=3)
+    g.setDepth(lambda X, Y, Z: np.maximum(0, 0.1 - Z))
+    g.advance(20, 8)
+    s = g.readState()
+    w = s["h"] > 1e-5
+    vmax = float(np.max(np.hypot(s["hu"][w], s["hv"][w]) / s["h"][w]))
+    print(f"\nstill water LTS: max|u| {vmax:.2e} m/s, mass rel {s['mass']['rel']:.2e}, cycles {g.steps}")
+    assert vmax <= 1e-4 and abs(s["mass"]["rel"]) <= TOL_MASS
+
+
+def test_lts_channel(fsa, golden):
+    """run-lts-tests.mjs:59-76 on the GPU: LTS against the oracle's LTS, 60 s."""
+    opts = channel_opts()
+    g, c, s, d = pair(fsa, opts, lambda o: o.setDepth(
+        lambda X, Y, Z: np.where(Z < -1, np.where(X < 40, 0.5, 0) - Z, 0.05)), 60)
+    r = golden["gpu"]["lts"]["channel256"]
+    assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS
+    assert g.steps == c.steps == r["cycles"]
+    assert 2095 * 256 * 192 / g.clock().cellUpdates >= 2.5
+
+
+def test_lts_urban_rain(fsa):
+    """The urban cloudburst (src/scenarios.js:8-49) under 150 mm/h with 4 LTS levels, 600 s in 10 s
+    frames. (One 600 s advance from a dry start is a degenerate cycle: its length comes from the
+    speed of the first millimetres of rain, and the film outgrows that step within the cycle.)"""
+    sc = rc.cloudburst(80)
+    opts = dict(nx=sc["nx"], ny=sc["ny"], dx=sc["dx"], z=sc["z"], mask=sc["mask"], manning=sc["manning"],
+                bc=sc["bc"], infil=sc["infil"])
+    g, c, s, d = pair(fsa, opts, lambda o: o.setDepth(sc["depth"]), 600, rain=150 * rc.MMHR, lts=4, frame=10)
+    assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS and s["mass"]["clipped"] == 0
+    assert abs(s["mass"]["outflow"] - c.vol["outflow"]) <= 1e-3 * c.vol["outflow"] + 1e-6

@@ -195,36 +195,50 @@ def test_rerun_identical(fsa):
     assert a["mass"]["outflow"] == b["mass"]["outflow"]
 
 
+def _inlet_case(o, rate_scale=1.0):
+    o.setDepth(lambda X, Y, Z: 0 * X)
+    o.rainRate = 60 * rc.MMHR * rate_scale
+    o.setInletRates([0.05, 0.05, -0.002, 0.5])
+
+
 def test_inlets_books(fsa):
-    """gpu-suite.js:142-160: sinks bounded by the water a cell holds, a source, books close."""
+    """gpu-suite.js:142-160: sinks bounded by the water a cell holds, a source, books close.
+
+    This case is ill-conditioned at the free-outfall edge (the 1e-4 surface-slope threshold switches
+    the outflow on and off): two FP64 runs whose rain rates differ by 1 part in 1e7 already differ
+    by ~1 mm at the west edge. The FP32 result is held to twice that FP64 self-sensitivity (printed),
+    besides the books."""
     nx, ny, dx = 60, 40, 2
     I, J = np.meshgrid(np.arange(nx), np.arange(ny))
     z = 0.01 * I + 0.002 * (J - 20) ** 2
     inlets = [5 * nx + 5, 20 * nx + 2, 30 * nx + 40, 12 * nx + 50]
     g = fsa.GPUSolver(nx=nx, ny=ny, dx=dx, z=z, manning=0.02, bc={"west": "outflow"}, inlets=inlets)
-    c = oracle.CPUSolver(nx=nx, ny=ny, dx=dx, z=z, manning=0.02, bc={"west": "outflow"}, inlets=inlets, hdry=g.hdry)
-    for o in (g, c):
-        o.setDepth(lambda X, Y, Z: 0 * X)
-        o.rainRate = 60 * rc.MMHR
-        o.setInletRates([0.05, 0.05, -0.002, 0.5])
+    mk = lambda: oracle.CPUSolver(nx=nx, ny=ny, dx=dx, z=z, manning=0.02, bc={"west": "outflow"}, inlets=inlets,
+                                  hdry=g.hdry)
+    c, c2 = mk(), mk()
+    _inlet_case(g)
+    _inlet_case(c)
+    _inlet_case(c2, 1 + 1e-7)
     tot_g = tot_c = 0.0
     for t in range(60, 901, 60):
         g.advance(t)
         c.run(t)
+        c2.run(t)
         tot_g += g.readInlets()["vol"].sum()
         tot_c += c.readInlets()["vol"].sum()
         if t == 480:
-            g.setInletRates([0.05, 0.05, 0, 0.5])
-            c.setInletRates([0.05, 0.05, 0, 0.5])
+            for o in (g, c, c2):
+                o.setInletRates([0.05, 0.05, 0, 0.5])
     s = g.readState()
-    d = float(np.max(np.abs(s["h"] - c.field("h"))))
     dd = np.abs(s["h"] - c.field("h"))
+    d = float(dd.max())
+    sens = float(np.abs(c2.field("h") - c.field("h")).max())
     k = int(np.argmax(dd))
-    print(f"\ninlets: exchanged gpu {tot_g:.4f} fp64 {tot_c:.4f} m3, max|dh| {d:.2e} at cell {k} (i={k % nx}, j={k // nx}) "
-          f"h gpu {s['h'][k]:.6f} fp64 {c.field('h')[k]:.6f}; mass rel {s['mass']['rel']:.2e}; "
-          f"99.9th pct |dh| {np.percentile(dd, 99.9):.2e}")
+    print(f"\ninlets: exchanged gpu {tot_g:.4f} fp64 {tot_c:.4f} m3, max|dh| {d:.2e} at (i={k % nx}, j={k // nx}); "
+          f"FP64 self-sensitivity (rain x (1+1e-7)) {sens:.2e} m; mass rel {s['mass']['rel']:.2e}")
     assert abs(s["mass"]["rel"]) <= TOL_MASS
-    assert s["h"].min() >= 0 and d <= TOL_H
+    assert s["h"].min() >= 0
+    assert d <= max(TOL_H, 2 * sens)
     assert abs(tot_g - tot_c) <= 1e-3 * abs(tot_c) + 1e-6
 
 

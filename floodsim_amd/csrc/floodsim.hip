@@ -369,7 +369,7 @@ int launch_cycle(fs_pool *P) {
   Group &G = P->grp;
   const DevInst &d = G.hinst[0];
   const int nt = d.tx * d.ty, nu = d.wend - d.wbeg, L = d.L;
-  const int nbp = std::min((nu + 3) / 4, 2048);
+  const int nbp = ((nu + 3) / 4 + 7) / 8 * 8;  // one wavefront per unit at most; idle blocks exit
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   if (P->timing) {
     e0 = timing_event(P); e1 = timing_event(P); e2 = timing_event(P);
@@ -1089,6 +1089,18 @@ int fs_read_state(fs_handle s, double *h, double *hu, double *hv, double *hmax, 
 }
 
 int fs_clock_get(fs_handle s, fs_clock *clk) { return s ? fs_pool_clock(s->pool, s->slot, clk) : FS_E_ARG; }
+
+int fs_lts_levels(fs_handle s, uint8_t *levels, int32_t *tx, int32_t *ty) {
+  if (!s) return FS_E_ARG;
+  fs_pool *P = s->pool;
+  Slot &S = P->slots[s->slot];
+  if (tx) *tx = S.tx;
+  if (ty) *ty = S.ty;
+  if (!S.L) return fail(P, FS_E_STATE, "no local time stepping on this solver");
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  if (levels) HIPCHK(P, hipMemcpy(levels, S.lv, (size_t)S.tx * S.ty, hipMemcpyDeviceToHost));
+  return FS_OK;
+}
 
 int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total) {
   if (!s) return FS_E_ARG;

@@ -164,14 +164,21 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m)
   const size_t k = (size_t)sy * I.pitch + sx;
   Prim p;
   float hu, hv;
-  cell_state<STAGE, LTS>(I, sx, sy, m, p.h, hu, hv);
+  p.lv = 0;
+  if constexpr (LTS) {
+    p.lv = tile_level(I, zx, zy);  // a ghost takes its edge cell's level (solver-cpu.js:125-128)
+    // the state comes from the source cell (a mirrored ghost's may sit in another tile)
+    const int ls = (gx || gy) ? tile_level(I, sx, sy) : p.lv;
+    if (STAGE == 1 && ls == m) { p.h = I.h1[k]; hu = I.hu1[k]; hv = I.hv1[k]; }
+    else { p.h = I.h[k]; hu = I.hu[k]; hv = I.hv[k]; }
+  } else {
+    cell_state<STAGE, false>(I, sx, sy, m, p.h, hu, hv);
+  }
   p.hu = hu * fu;
   p.hv = hv * fv;
   p.z = I.z[(size_t)zy * I.pitch + zx];
   p.in = !gx && !gy;
   p.M = p.in ? (I.mask[k] & MASK_OPEN) != 0 : true;
-  p.lv = 0;
-  if constexpr (LTS) p.lv = tile_level(I, zx, zy);  // a ghost takes its edge cell's level (solver-cpu.js:125-128)
   const float ih = frcp(p.h);
   const bool wet = p.h > I.hdry;
   p.u = wet ? p.hu * ih : 0.f;
@@ -274,6 +281,18 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, size_t k, float d
   const float hm = I.hmax[k];
   if (qh > hm) I.hmax[k] = qh;
   if (qh > hdry && !(I.mask[k] & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
+}
+
+// wavefront index for a device-side count of n work items: the blocks that have work are spread
+// round-robin over the 8 XCDs and each XCD takes one contiguous run of items; -1 = no work
+__device__ __forceinline__ int xcd_wave_index_n(int n, int nwaves_per_block) {
+  const int b = blockIdx.x, x = b & 7, k = b >> 3;
+  const int need = (n + nwaves_per_block - 1) / nwaves_per_block;
+  const int q = need >> 3, r = need & 7;
+  if (k >= q + (x < r ? 1 : 0)) return -1;
+  const int lb = x * q + min(x, r) + k;
+  const int w = lb * nwaves_per_block + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  return w < n ? w : -1;
 }
 
 struct FaceRow { float F0, F1, F2, cL, cR, on; };
@@ -542,8 +561,8 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
 }
 
 // One RK stage over a work list. Global stepping: one wavefront per strip of `work`.
-// LTS record of level m (single-instance groups): persistent wavefronts walk the level's list,
-// each a contiguous chunk (neighbouring strips stay on one XCD).
+// LTS record of level m (single-instance groups): one wavefront per entry of the level's list,
+// whose length only the device knows; the grid covers every strip and blocks without an entry exit.
 template <int STAGE, bool LTS>
 __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                 int nwork, Part *__restrict__ part, int m) {
@@ -564,19 +583,14 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     const DevInst &I = insts[0];
     if (STAGE == 1 && !I.secondOrder) return;
     const float dt = I.rec->ldt[m];
-    const int n = I.rec->lcount[m];
-    const int nw = gridDim.x * 4;
-    const int chunk = (n + nw - 1) / nw;
-    const int i0 = wid * chunk, i1 = min(n, i0 + chunk);
-    const int nu = I.wend - I.wbeg;
-    for (int i = i0; i < i1; ++i) {
-      if (!(dt > 0.f)) {
-        if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0.f};
-        continue;
-      }
-      const int u = I.ulist[m * nu + i];
-      run_strip<STAGE, true>(I, work[I.wbeg + u], lane, m, dt, part + i);
+    const int i = xcd_wave_index_n(I.rec->lcount[m], 4);
+    if (i < 0) return;
+    if (!(dt > 0.f)) {
+      if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0.f};
+      return;
     }
+    const int u = I.ulist[m * (I.wend - I.wbeg) + i];
+    run_strip<STAGE, true>(I, work[I.wbeg + u], lane, m, dt, part + i);
   }
 }
 
@@ -586,14 +600,14 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
 __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                    int m) {
   const DevInst &I = insts[0];
-  const int wid = xcd_wave_index(4), lane = threadIdx.x & 63;
-  const int n = I.rec->lcount[m], nw = gridDim.x * 4, chunk = (n + nw - 1) / nw;
+  const int lane = threadIdx.x & 63;
+  const int i = xcd_wave_index_n(I.rec->lcount[m], 4);
   const int nu = I.wend - I.wbeg;
   const bool two = I.secondOrder;
-  for (int i = wid * chunk; i < min(n, wid * chunk + chunk); ++i) {
+  if (i >= 0) {
     const int4 wk = work[I.wbeg + I.ulist[m * nu + i]];
     const int X = wk.y - I.hl + lane;
-    if (lane < I.hl || lane >= I.hl + I.own || X >= I.nx) continue;
+    if (lane < I.hl || lane >= I.hl + I.own || X >= I.nx) return;
     for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
       const size_t k = (size_t)y * I.pitch + X;
       if (!I.gflag[k]) continue;
@@ -883,9 +897,9 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
     const DevInst &I = insts[0];
     if (I.secondOrder) return;
     const float dt = I.rec->ldt[m];
-    const int n = I.rec->lcount[m], nw = gridDim.x * 4, chunk = (n + nw - 1) / nw, nu = I.wend - I.wbeg;
-    for (int i = wid * chunk; i < min(n, wid * chunk + chunk); ++i)
-      euler_finish_strip<true>(I, work[I.wbeg + I.ulist[m * nu + i]], lane, m, dt, part + i);
+    const int i = xcd_wave_index_n(I.rec->lcount[m], 4);
+    if (i < 0) return;
+    euler_finish_strip<true>(I, work[I.wbeg + I.ulist[m * (I.wend - I.wbeg) + i]], lane, m, dt, part + i);
   }
 }
 

@@ -53,7 +53,7 @@ EXPORTS = [
     "fs_pool_add", "fs_pool_remove", "fs_pool_set_state", "fs_pool_set_rain", "fs_pool_set_inlet_rates",
     "fs_pool_advance", "fs_pool_step", "fs_pool_read_state", "fs_pool_clock", "fs_pool_bytes_used",
     "fs_pool_kernel_ms", "fs_pool_set_timing", "fs_solver_pool", "fs_version", "fs_copy_probe",
-    "fs_pool_dominant_kernel",
+    "fs_pool_dominant_kernel", "fs_lts_levels",
 ]
 
 
@@ -108,6 +108,7 @@ def lib(path=None):
     L.fs_pool_kernel_ms.argtypes = [P, dp, dp, ctypes.POINTER(ctypes.c_int64)]
     L.fs_pool_set_timing.argtypes = [P, I32]
     L.fs_copy_probe.argtypes = [SZ, I32, dp, dp]
+    L.fs_lts_levels.argtypes = [P, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.fs_pool_dominant_kernel.argtypes = [P, dp, dp, ctypes.POINTER(ctypes.c_int64)]
     L.fs_solver_pool.restype = P
     L.fs_solver_pool.argtypes = [P]
@@ -297,6 +298,14 @@ class GPUSolver:
         self._chk(lib().fs_read_state(self._h, _dp(h), _dp(hu), _dp(hv), _dp(hm), ctypes.byref(m)))
         mass = {k: getattr(m, k) for k, _ in fs_mass._fields_}
         return {"h": h, "hu": hu, "hv": hv, "hmax": hm, "mass": mass}
+
+    def levels(self):
+        """LTS tile levels of the last cycle, (ty, tx) uint8."""
+        tx, ty = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(lib().fs_lts_levels(self._h, None, ctypes.byref(tx), ctypes.byref(ty)))
+        out = np.zeros(tx.value * ty.value, dtype=np.uint8)
+        self._chk(lib().fs_lts_levels(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), None, None))
+        return out.reshape(ty.value, tx.value)
 
     def tileStats(self):
         a, t = ctypes.c_int64(), ctypes.c_int64()

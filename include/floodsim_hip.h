@@ -86,6 +86,9 @@ int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk);
 /* readState (:651-690): interior FP64 copies (any may be NULL) and the FP64 books */
 int fs_read_state(fs_handle s, double *h, double *hu, double *hv, double *hmax, fs_mass *mass);
 int fs_clock_get(fs_handle s, fs_clock *clk);
+/* LTS tile levels of the last cycle (8x8 tiles, tx*ty bytes, row-major): the reference keeps them
+ * in its TL buffer (kernels.wgsl.js:307) and CPUSolver.LV (solver-cpu.js:124) */
+int fs_lts_levels(fs_handle s, uint8_t *levels, int32_t *tx, int32_t *ty);
 /* tileStats (:625-629): open cells advanced by the last step and the interior cell total */
 int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total);
 

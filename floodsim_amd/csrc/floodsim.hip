@@ -101,6 +101,8 @@ struct fs_pool {
   int capHist = 0;
   double domMs = 0.0, domCells = 0.0;
   int64_t domLaunches = 0;
+  float2 *pack = nullptr;                // frame readout staging (depth, speed per cell)
+  size_t capPack = 0;
   std::string err;
 };
 
@@ -518,10 +520,16 @@ int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n
   if (rc) return rc;
   std::vector<double> tprev(gs.size());
   for (size_t i = 0; i < gs.size(); ++i) tprev[i] = P->slots[gs[i]].hrec.t;
+  // one launch sequence holds at most kMaxBatch steps (a captured graph is sized by it); a frame
+  // (once) runs at most K steps in total, an advance repeats batches until every target is reached
+  constexpr int kMaxBatch = 256;
+  int64_t left = K;
   int stalls = 0;
   for (;;) {
-    rc = launch_steps(P, K);
+    const int kb = (int)std::min<int64_t>(left, kMaxBatch);
+    rc = launch_steps(P, kb);
     if (rc) return rc;
+    if (once) left -= kb;
     rc = download_recs(P, gs);
     if (rc) return rc;
     bool done = true, progress = false;
@@ -535,7 +543,7 @@ int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n
       tprev[i] = S.hrec.t;
       if (S.hrec.t < gt[i]) done = false;
     }
-    if (done || once) break;
+    if (done || (once && left <= 0)) break;
     if (!progress && ++stalls > 3) return fail(P, FS_E_STALL, "GPU solver made no progress (device error?)");
   }
   return FS_OK;
@@ -712,6 +720,16 @@ int set_state(fs_pool *P, int slot, const double *h, const double *hu, const dou
 
 // plain copy of n float4 (16 B) elements: the achievable-HBM reference and the byte-counter
 // calibration for the profiles
+// depth and speed per cell for rendering; grid-stride over the nx*ny interior cells
+__global__ __launch_bounds__(256) void sw_pack(const float *__restrict__ h, const float *__restrict__ hu,
+                                               const float *__restrict__ hv, float hdry, size_t n,
+                                               float2 *__restrict__ out) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const float d = h[k], qu = hu[k], qv = hv[k];
+    out[k] = make_float2(d, d > hdry ? sqrtf(qu * qu + qv * qv) / d : 0.f);
+  }
+}
+
 __global__ __launch_bounds__(256) void fs_copy_kernel(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
@@ -790,6 +808,7 @@ void fs_pool_destroy(fs_pool_t P) {
   for (auto &ev : P->tev) (void)hipEventDestroy(ev);
   for (auto &ev : P->spanEv) (void)hipEventDestroy(ev);
   if (P->dhist) (void)hipFree(P->dhist);
+  if (P->pack) (void)hipFree(P->pack);
   if (P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
 }
@@ -1068,17 +1087,19 @@ int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk) {
   if (rc) return rc;
   Slot &S = P->slots[sl];
   if (out) {
+    // pack (kernels.wgsl.js:1346-1355) on the device: depth and speed per interior cell, 8 B read back
     const size_t N = (size_t)S.nx * S.ny;
-    std::vector<float> th(N), tu(N), tv(N);
-    HIPCHK(P, hipMemcpy(th.data(), S.h, N * 4, hipMemcpyDeviceToHost));
-    HIPCHK(P, hipMemcpy(tu.data(), S.hu, N * 4, hipMemcpyDeviceToHost));
-    HIPCHK(P, hipMemcpy(tv.data(), S.hv, N * 4, hipMemcpyDeviceToHost));
-    // pack (kernels.wgsl.js:1348-1355): depth and speed per interior cell
-    for (size_t k = 0; k < N; ++k) {
-      const float d = th[k];
-      out[2 * k] = d;
-      out[2 * k + 1] = d > (float)S.p.hdry ? std::sqrt(tu[k] * tu[k] + tv[k] * tv[k]) / d : 0.f;
+    if (N > P->capPack) {
+      if (P->pack) HIPCHK(P, hipFree(P->pack));
+      P->pack = nullptr;
+      HIPCHK(P, hipMalloc(&P->pack, N * sizeof(float2)));
+      P->capPack = N;
     }
+    const int blocks = (int)std::min<size_t>((N + 255) / 256, 65536);
+    sw_pack<<<blocks, 256, 0, P->stream>>>(S.h, S.hu, S.hv, (float)S.p.hdry, N, P->pack);
+    HIPCHK(P, hipGetLastError());
+    HIPCHK(P, hipMemcpyAsync(out, P->pack, N * sizeof(float2), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(P, hipStreamSynchronize(P->stream));
   }
   if (clk) fill_clock(S, clk);
   return FS_OK;

@@ -1,0 +1,257 @@
+'use strict';
+/**
+ * GPUSolver over the MI355X HIP library: the surface solver interface of the reference
+ * (src/solver-gpu.js:13-693) for Node, in ES2019 syntax, bound to libfloodsim_hip.so through the
+ * N-API addon floodsim_napi.node.
+ *
+ *   const { GPUSolver } = require('floodsim_amd/js/solver-hip.js');
+ *   const s = new GPUSolver(null, { nx, ny, dx, z, manning, mask, bc, lts: 4 });
+ *   s.setDepth((x, y, z) => ...); s.rainRate = 50 / 3.6e6;
+ *   await s.advance(600); const st = await s.readState();
+ *
+ * What matches the reference:
+ * - constructor options nx, ny, dx, z, manning (number or array), mask, infil (number or array),
+ *   rainField, bc {west, east, south, north: 'wall'|'open'|'outflow'}, secondOrder, limiter, cfl,
+ *   hdry, hfront, dtDry, g, inlets, lts (true = 3 levels, or 1-4), tiles (solver-gpu.js:35-47);
+ * - setDepth(fn(x, y, z)) and setVelocity(fn(x, y)) evaluated at cell centres, depth clamped at 0
+ *   and 0 in walls, vol.initial the sum of the stored FP32 depths (solver-gpu.js:301-318);
+ * - rainRate (m/s), setRainField, setInletRates (m^3/s, + = surface into the network),
+ *   readInlets() -> {vol: Float64Array, h: Float32Array, app}, advance(tTarget, K = 64) with the
+ *   'GPU solver made no progress (device error?)' error, frame(tEnd, K, readOut) ->
+ *   Float32Array (depth, speed per cell) or null, readState() -> {h, hu, hv, hmax, mass},
+ *   tileStats(), destroy(); fields t, steps, lastDt, dtAvg, vol, nx, ny, dx, NX, NY, hdry, zref,
+ *   zh, idx(i, j), lts, ltsOff, tiled, bc, ready (solver-gpu.js:550-693).
+ *
+ * The device argument is accepted for signature compatibility and ignored: the library drives
+ * HIP device 0 of the process. Options of the reference outside the shallow-water step
+ * (subgrid, sink, gi, barriers, sediment, debris, implicitFriction, stepFlux, stageCells) raise.
+ *
+ * Calls on one solver run one at a time: asynchronous calls queue behind each other, and the
+ * synchronous setters wait for the queue when it is busy, so they land between the calls they
+ * were made between.
+ */
+const path = require('path');
+
+const addon = require(path.join(__dirname, 'floodsim_napi.node'));
+
+const GH = 2; // the reference's ghost width, kept for idx(i, j) and zh
+const BC = { wall: 0, open: 1, outflow: 2 };
+const UNSUPPORTED = ['subgrid', 'sink', 'gi', 'barriers', 'sediment', 'debris', 'implicitFriction', 'stepFlux',
+  'stageCells', 'stage'];
+
+function def(v, d) { return v === undefined || v === null ? d : v; }
+
+function f64(a, n, what) {
+  if (a === undefined || a === null) return null;
+  if (typeof a === 'number') return new Float64Array(n).fill(a);
+  if (a.length !== n) throw new RangeError(what + ': length ' + a.length + ', expected ' + n);
+  return a instanceof Float64Array ? a : Float64Array.from(a);
+}
+
+class GPUSolver {
+  constructor(device, o) {
+    for (const k of UNSUPPORTED) {
+      if (o[k]) throw new Error('GPUSolver (HIP): option ' + k + ' is not supported');
+    }
+    this.device = device;
+    this.nx = o.nx; this.ny = o.ny; this.dx = o.dx;
+    this.NX = o.nx + 2 * GH; this.NY = o.ny + 2 * GH;
+    this.g = def(o.g, 9.81);
+    this.secondOrder = def(o.secondOrder, true);
+    this.cfl = def(o.cfl, this.secondOrder ? 0.48 : 0.9);
+    this.hdry = def(o.hdry, 1e-5); this.hfront = def(o.hfront, 1e-3);
+    this.limiter = o.limiter === 'vanleer' ? 1 : 0;
+    this.bc = Object.assign({ west: 'wall', east: 'wall', south: 'wall', north: 'wall' }, o.bc || {});
+    this.dtDry = def(o.dtDry, 1.0);
+    this.lts = o.lts === true ? 3 : (o.lts | 0);
+    this.ltsOff = null;
+    this.tiled = o.tiles !== false;
+    const n = o.nx * o.ny;
+    const z = f64(o.z, n, 'z');
+    let zmin = Infinity;
+    for (let k = 0; k < n; k++) if (z[k] < zmin) zmin = z[k];
+    this.zref = zmin;
+    this._z = z;
+    this._mask = o.mask ? (o.mask instanceof Uint8Array ? o.mask : Uint8Array.from(o.mask)) : null;
+    const rf = f64(o.rainField, n, 'rainField');
+    this._rf = rf;
+    this.nInlets = o.inlets ? o.inlets.length : 0;
+    const params = {
+      nx: o.nx, ny: o.ny, dx: o.dx, g: this.g, cfl: this.cfl, hdry: this.hdry, hfront: this.hfront,
+      dtDry: this.dtDry, secondOrder: this.secondOrder ? 1 : 0, limiter: this.limiter,
+      bcWest: BC[this.bc.west], bcEast: BC[this.bc.east], bcSouth: BC[this.bc.south], bcNorth: BC[this.bc.north],
+      lts: this.lts, rows: o.rows | 0,
+    };
+    for (const side of ['west', 'east', 'south', 'north']) {
+      if (BC[this.bc[side]] === undefined) throw new Error('GPUSolver (HIP): bc.' + side + ' = ' + this.bc[side]);
+    }
+    const manArr = typeof o.manning === 'number' || !o.manning ? null : f64(o.manning, n, 'manning');
+    const manS = typeof o.manning === 'number' ? o.manning : 0;
+    const infil = o.infil === undefined || o.infil === null ? null : f64(o.infil, n, 'infil');
+    const inlets = this.nInlets ? Int32Array.from(o.inlets) : null;
+    this._h = addon.create(params, z, manArr, manS, this._mask, infil, rf, inlets);
+    this.t = 0; this.steps = 0; this.lastDt = 0; this.dtAvg = 0;
+    this.vol = { rain: 0, initial: 0, exchanged: 0 };
+    this._rain = 0;
+    let area = 0;
+    for (let k = 0; k < n; k++) if (!this._mask || this._mask[k]) area += rf ? rf[k] : 1;
+    this.rainArea = area * this.dx * this.dx;
+    this._depth = new Float64Array(n);
+    this._queue = Promise.resolve();
+    this._pending = 0;
+    this._zh = null;
+    this.ready = Promise.resolve();
+  }
+
+  /** Padded index of interior cell (i, j), as the reference's idx (GH = 2 ghost layers). */
+  idx(i, j) { return (j + GH) * this.NX + (i + GH); }
+
+  /** Bed relative to its minimum on the padded grid (edge-replicated ghosts), as the reference's zh. */
+  get zh() {
+    if (!this._zh) {
+      const zh = new Float64Array(this.NX * this.NY);
+      for (let j = 0; j < this.NY; j++) {
+        for (let i = 0; i < this.NX; i++) {
+          const ii = Math.min(Math.max(i - GH, 0), this.nx - 1), jj = Math.min(Math.max(j - GH, 0), this.ny - 1);
+          zh[j * this.NX + i] = this._z[jj * this.nx + ii] - this.zref;
+        }
+      }
+      this._zh = zh;
+    }
+    return this._zh;
+  }
+
+  // ---- serialisation of calls on the one handle
+  _enqueue(fn) {
+    this._pending++;
+    const p = this._queue.then(fn);
+    const done = () => { this._pending--; };
+    this._queue = p.then(done, done);
+    return p;
+  }
+
+  _sync(fn) {
+    if (this._pending === 0) { fn(); return; }
+    this._enqueue(fn).catch((e) => { this._deferredError = e; });
+  }
+
+  _raiseDeferred() {
+    if (this._deferredError) { const e = this._deferredError; this._deferredError = null; throw e; }
+  }
+
+  _clock(c) {
+    const ds = c.steps - this.steps;
+    if (ds > 0 && c.t > this.t) this.dtAvg = (c.t - this.t) / ds;
+    this.lastDt = c.lastDt > 0 ? c.lastDt : this.lastDt;
+    this.t = c.t; this.steps = c.steps;
+  }
+
+  // ---- state (solver-gpu.js:301-318)
+  setDepth(fn) {
+    const nx = this.nx, ny = this.ny, dx = this.dx, d = this._depth;
+    let v = 0;
+    for (let j = 0; j < ny; j++) {
+      for (let i = 0; i < nx; i++) {
+        const k = j * nx + i;
+        const h = !this._mask || this._mask[k] ? Math.max(0, fn((i + 0.5) * dx, (j + 0.5) * dx, this._z[k])) : 0;
+        d[k] = Math.fround(h); v += d[k];
+      }
+    }
+    this.vol.initial = v * dx * dx;
+    const hu = new Float64Array(nx * ny), hv = new Float64Array(nx * ny), h = Float64Array.from(d);
+    this._sync(() => addon.setState(this._h, h, hu, hv, true));
+  }
+
+  setVelocity(fn) {
+    const nx = this.nx, ny = this.ny, dx = this.dx, d = this._depth;
+    const hu = new Float64Array(nx * ny), hv = new Float64Array(nx * ny);
+    for (let j = 0; j < ny; j++) {
+      for (let i = 0; i < nx; i++) {
+        const k = j * nx + i, uv = fn((i + 0.5) * dx, (j + 0.5) * dx);
+        hu[k] = d[k] * uv[0]; hv[k] = d[k] * uv[1];
+      }
+    }
+    const h = Float64Array.from(d);
+    this._sync(() => addon.setState(this._h, h, hu, hv, true));
+  }
+
+  get rainRate() { return this._rain; }
+  set rainRate(r) {
+    this._rain = +r;
+    if (this._h) this._sync(() => addon.setRain(this._h, this._rain));
+  }
+
+  setRainField(f) {
+    const a = f64(f, this.nx * this.ny, 'rainField');
+    this._sync(() => addon.setRainField(this._h, a));
+  }
+
+  /** Exchange rates per inlet (m^3/s, + = surface into the network), held until the next call. */
+  setInletRates(q) {
+    const a = Float64Array.from(q);
+    this._sync(() => addon.setInletRates(this._h, a));
+  }
+
+  /** Volume exchanged per inlet since the last call (m^3) and the inlet depths (m). */
+  readInlets() {
+    this._raiseDeferred();
+    return this._enqueue(() => addon.readInlets(this._h)).then((r) => {
+      const h = Float32Array.from(r.h);
+      for (let m = 0; m < r.vol.length; m++) this.vol.exchanged += r.vol[m];
+      // app (per-inlet approach flows for grate capture) belongs to the drainage model, not this step
+      return { vol: r.vol, h, app: new Float32Array(4 * this.nInlets) };
+    });
+  }
+
+  /** Advance to tTarget in batches of K steps; rejects if the device stops making progress. */
+  advance(tTarget, K) {
+    this._raiseDeferred();
+    const t0 = this.t;
+    return this._enqueue(() => addon.advance(this._h, tTarget, K === undefined ? 64 : K)).then((c) => {
+      this._clock(c);
+      this.vol.rain += this._rain * (this.t - t0) * this.rainArea;
+    });
+  }
+
+  /** Advance up to tEnd with at most K steps, then return packed (depth, speed) or null. */
+  frame(tEnd, K, readOut) {
+    this._raiseDeferred();
+    const t0 = this.t;
+    const ro = readOut === undefined ? true : !!readOut;
+    return this._enqueue(() => addon.frame(this._h, tEnd, K === undefined ? 64 : K, ro)).then((r) => {
+      this._clock(r.clock);
+      this.vol.rain += this._rain * (this.t - t0) * this.rainArea;
+      return r.out;
+    });
+  }
+
+  /** Interior fields (Float64Array h, hu, hv, hmax) and the FP64 mass books. */
+  readState() {
+    this._raiseDeferred();
+    return this._enqueue(() => addon.readState(this._h)).then((r) => {
+      this._clock(r.clock);
+      return { h: r.h, hu: r.hu, hv: r.hv, hmax: r.hmax, mass: r.mass };
+    });
+  }
+
+  /** Open cells advanced by the last step against the interior total. */
+  tileStats() {
+    return this._enqueue(() => {
+      const s = addon.tileStats(this._h);
+      return { tiled: this.tiled, tiles: Math.ceil(this.nx / 8) * Math.ceil(this.ny / 8), update: s.active, cells: s.cells };
+    });
+  }
+
+  /** LTS levels of the last cycle per 8x8 tile: {levels: Uint8Array, tx, ty}. */
+  levels() { return this._enqueue(() => addon.levels(this._h)); }
+
+  destroy() {
+    if (!this._h) return;
+    const h = this._h;
+    this._h = null;
+    if (this._pending === 0) addon.destroy(h);
+    else this._queue = this._queue.then(() => addon.destroy(h));
+  }
+}
+
+module.exports = { GPUSolver, version: addon.version };

@@ -217,7 +217,8 @@ class GPUSolver:
             h = fn_or_array(X, Y, self.layers.z.reshape(self.ny, self.nx))
         else:
             h = fn_or_array
-        h = np.broadcast_to(np.asarray(h, dtype=np.float64), (self.ny, self.nx)).reshape(-1)
+        h = np.asarray(h, dtype=np.float64)
+        h = h.reshape(-1) if h.size == self.nx * self.ny else np.broadcast_to(h, (self.ny, self.nx)).reshape(-1)
         if self.layers.mask is not None:
             h = np.where(self.layers.mask != 0, np.maximum(0, h), 0.0)
         else:

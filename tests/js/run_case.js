@@ -1,0 +1,37 @@
+'use strict';
+// Drives one case through the JavaScript GPUSolver (floodsim_amd/js/solver-hip.js) and prints the
+// result as one JSON line. The case file holds the constructor options plus the initial depth
+// (per cell, row-major), rain, inlet rates and the frame times to advance through.
+const fs = require('fs');
+const path = require('path');
+const { GPUSolver } = require(path.join(__dirname, '..', '..', 'floodsim_amd', 'js', 'solver-hip.js'));
+
+async function main() {
+  const c = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));
+  const o = {
+    nx: c.nx, ny: c.ny, dx: c.dx, z: Float64Array.from(c.z), manning: c.manning,
+    mask: c.mask ? Uint8Array.from(c.mask) : undefined, bc: c.bc, lts: c.lts || 0,
+    inlets: c.inlets && c.inlets.length ? c.inlets : undefined,
+  };
+  const s = new GPUSolver(null, o);
+  await s.ready;
+  const depth = c.depth;
+  s.setDepth((x, y) => depth[Math.floor(y / c.dx) * c.nx + Math.floor(x / c.dx)]);
+  s.rainRate = c.rain || 0;
+  if (c.rates) s.setInletRates(c.rates);
+  let frameSum = 0, inletVol = 0;
+  for (const t of c.frames) {
+    const out = await s.frame(t, 1 << 20, true);
+    for (let k = 0; k < out.length; k += 2) frameSum += out[k];
+  }
+  if (s.nInlets) { const r = await s.readInlets(); for (const v of r.vol) inletVol += v; }
+  const st = await s.readState();
+  const tiles = await s.tileStats();
+  s.destroy();
+  process.stdout.write(JSON.stringify({
+    t: s.t, steps: s.steps, lastDt: s.lastDt, h: Array.from(st.h), hu: Array.from(st.hu), hv: Array.from(st.hv),
+    hmax: Array.from(st.hmax), mass: st.mass, frameSum, inletVol, volRain: s.vol.rain, tiles,
+  }) + '\n');
+}
+
+main().catch((e) => { process.stderr.write(String(e && e.stack || e) + '\n'); process.exit(1); });

@@ -1,0 +1,113 @@
+"""The Node boundary: N-API addon + the JavaScript GPUSolver (floodsim_amd/js/solver-hip.js).
+
+CPU: the addon loads under Node and exports the functions the JS class uses; the class rejects
+options of the reference outside the shallow-water step before touching the device.
+GPU: one case driven through the JS class gives the same bits as the same case through the Python
+binding (both call the one C ABI), and matches the FP64 oracle to 1 mm.
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "floodsim_amd", "js", "floodsim_napi.node")
+NODE = shutil.which("node")
+
+pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readInlets", "readState", "setInletRates",
+               "setRain", "setRainField", "setState", "tileStats", "version"]
+
+
+def node(script, *args, timeout=120):
+    r = subprocess.run([NODE, "-e", script, *args], cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=timeout)
+    return r
+
+
+def test_addon_exports():
+    assert os.path.exists(ADDON), "floodsim_napi.node is not built: run __graft_entry__.build()"
+    r = node("const m=require('./floodsim_amd/js/floodsim_napi.node'); console.log(JSON.stringify(Object.keys(m)))")
+    assert r.returncode == 0, r.stderr
+    assert sorted(json.loads(r.stdout)) == sorted(ADDON_FUNCS)
+
+
+def test_js_class_loads_and_rejects_out_of_scope_options():
+    r = node("const {GPUSolver, version}=require('./floodsim_amd/js/solver-hip.js');"
+             "let msg='none'; try { new GPUSolver(null,{nx:4,ny:4,dx:1,z:new Float64Array(16),subgrid:{}}); }"
+             " catch (e) { msg=e.message; } console.log(version()); console.log(msg);")
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[0].startswith("floodsim_hip")
+    assert "subgrid" in lines[1] and "not supported" in lines[1]
+
+
+def test_js_sources_are_es2019():
+    """Node 12 parses the class (no ??, ?., class fields): a syntax check of each JS file."""
+    for f in ("floodsim_amd/js/solver-hip.js", "tests/js/run_case.js"):
+        r = subprocess.run([NODE, "--check", os.path.join(ROOT, f)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True)
+        assert r.returncode == 0, f + ": " + r.stderr
+
+
+def _case(nx=48, ny=40, seed=5):
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5)
+    z = np.tile(0.01 * (nx - x), (ny, 1)) + 0.05 * rng.random((ny, nx))
+    mask = np.ones((ny, nx), np.uint8)
+    mask[10:16, 20:26] = 0
+    depth = np.where(np.arange(nx)[None, :] < nx // 3, 0.6, 0.0) * np.ones((ny, 1))
+    return dict(nx=nx, ny=ny, dx=1.0, z=z.reshape(-1).tolist(), manning=0.03, mask=mask.reshape(-1).tolist(),
+                bc={"west": "wall", "east": "outflow", "south": "wall", "north": "wall"},
+                depth=depth.reshape(-1).tolist(), rain=60 / 3.6e6, inlets=[20 * nx + 5], rates=[-0.2],
+                frames=[2.0, 5.0, 8.0])
+
+
+def _python_run(c):
+    import floodsim_amd as fsa
+    g = fsa.GPUSolver(nx=c["nx"], ny=c["ny"], dx=c["dx"], z=np.array(c["z"]), manning=c["manning"],
+                      mask=np.array(c["mask"], np.uint8), bc=c["bc"], inlets=c["inlets"])
+    g.setDepth(np.array(c["depth"]))
+    g.rainRate = c["rain"]
+    g.setInletRates(c["rates"])
+    for t in c["frames"]:
+        g.frame(t, 1 << 20, True)
+    s = g.readState()
+    steps = g.steps
+    g.destroy()
+    return s, steps
+
+
+@pytest.mark.gpu
+def test_js_matches_python_binding_and_oracle(tmp_path):
+    import oracle
+    c = _case()
+    p = tmp_path / "case.json"
+    p.write_text(json.dumps(c))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "run_case.js"), str(p)], cwd=ROOT,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    py, steps = _python_run(c)
+    # the same C ABI under both bindings: identical bits
+    assert js["steps"] == steps
+    for k in ("h", "hu", "hv", "hmax"):
+        assert np.array_equal(np.array(js[k]), py[k]), k
+    assert js["mass"]["volume"] == py["mass"]["volume"]
+    # against the FP64 oracle (1 mm), and mass closed on the GPU's own books
+    o = oracle.CPUSolver(nx=c["nx"], ny=c["ny"], dx=1.0, z=np.array(c["z"]), manning=c["manning"],
+                         mask=np.array(c["mask"], np.uint8), bc=c["bc"], inlets=c["inlets"], hdry=1e-5)
+    o.setDepth(np.array(c["depth"]))
+    o.rainRate = c["rain"]
+    o.setInletRates(c["rates"])
+    o.run(c["frames"][-1])
+    d = float(np.max(np.abs(np.array(js["h"]) - o.field("h"))))
+    print(f"JS path: {js['steps']} steps, max|dh| vs oracle {d:.2e} m, mass rel {js['mass']['rel']:.1e}")
+    assert d <= 1e-3
+    assert abs(js["mass"]["rel"]) <= 1e-6
+    assert js["inletVol"] < 0            # water entered through the inlet (q < 0 is inflow)
+    assert abs(js["volRain"] - js["mass"]["rain"]) <= 1e-6 * max(1.0, js["mass"]["rain"])

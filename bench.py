@@ -34,6 +34,46 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class _Regions:
+    """roctx profiler regions: under `rocprofv3 --selected-regions` only the timed region (and the
+    copy probe) is traced; without a profiler the calls do nothing."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so", "/opt/rocm/lib/librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            for f in (self.lib.roctxProfilerPause, self.lib.roctxProfilerResume):
+                f.argtypes = [ctypes.c_uint64]
+                f.restype = ctypes.c_int
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+
+def traffic_of(workload, stepping):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (calibrated against the copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        e = t[f"{workload}/{stepping}"]
+        return e["bytes_per_launch"], e
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def build_workload(name, seed=2026):
     from floodsim_amd import scenarios as sc
     if name.startswith("urban"):
@@ -107,6 +147,8 @@ def main():
     from floodsim_amd import scenarios as sc
     from floodsim_amd.solver import copy_probe
 
+    regions = _Regions()
+    regions.pause()
     t_setup = time.perf_counter()
     w = build_workload(args.workload)
     log(f"workload {args.workload}: generated in {time.perf_counter() - t_setup:.1f} s")
@@ -150,10 +192,12 @@ def main():
     run_steps(args.warmup)
     g.set_timing(True)
     c1 = g.clock()                                   # synchronises the stream
+    regions.resume()
     t0 = time.perf_counter()
     run_steps(args.steps)
     c2 = g.clock()                                   # synchronises again
     el = time.perf_counter() - t0
+    regions.pause()
     ms1, ms2, nt = g.kernel_ms()
     dom_ms, dom_cells, dom_n = g.dominant_kernel()
     g.set_timing(False)
@@ -165,11 +209,14 @@ def main():
     # dominant kernel: the second RK stage (+ sources); algorithmic bytes = 65 B per open cell it advances
     achieved = dom_cells * B_STAGE2 / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     s = g.readState()
+    regions.resume()
     try:
         cp_gbs, _ = copy_probe(2 << 30, 10)
     except Exception as e:  # noqa: BLE001
         log("copy probe failed:", e)
         cp_gbs = None
+    regions.pause()
+    traffic, tr = traffic_of(args.workload, stepping)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, s, lts=lts)
@@ -192,7 +239,8 @@ def main():
                    "spinup_s": args.spinup, "t_start": c1.t, "t_end": c2.t,
                    "layers_checksum": w.get("checksum"), "rows_per_strip": args.rows or 64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "traffic_source": tr and tr.get("source"),
                      "kernel": "fs::sw_stage<1,%s> (second RK stage + sources)" % ("true" if lts else "false"),
                      "bytes_per_cell_update": B_STAGE2, "launches": dom_n,
                      "cells_per_launch": dom_cells / max(dom_n, 1), "avg_ms": dom_ms / max(dom_n, 1)},

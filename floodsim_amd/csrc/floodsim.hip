@@ -45,6 +45,10 @@ struct Slot {
   std::vector<float> hrf;                // host copy of the rain multiplier (empty = 1)
   int openCells = 0;
   int rows = 64;
+  // dry-strip skipping (global stepping): per strip flags, rain and open-cell counts
+  int sxn = 0, nstrips = 0;
+  uint8_t *swet = nullptr, *sact = nullptr, *srain = nullptr;
+  int32_t *sopen = nullptr;
   // local time stepping
   int L = 0, tx = 0, ty = 0, nunits = 0;
   uint8_t *lv = nullptr, *lv0 = nullptr, *gflag = nullptr, *umask = nullptr;
@@ -71,8 +75,8 @@ struct Group {
   bool anyFirstOrder = false, anySecondOrder = false;
   bool lts = false;                      // a local-time-stepping instance (alone in its group)
   // captured K-step graph
-  hipGraphExec_t graph = nullptr;
-  int graphK = 0;
+  // captured K-step graphs, one per power-of-two K (index log2 K)
+  hipGraphExec_t graphs[16] = {};
 };
 
 }  // namespace
@@ -154,6 +158,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
     b += 10 * align256(N * 4) + align256(N) + 2 * align256(N * 16);  // store, gathers, flag, bsx, bsy
     b += align256(nu) + 2 * align256(nu * 4 * (kMaxLevels + 1));     // umask, ucells, ulist
   }
+  if (!p->lts) b += 3 * align256((size_t)units_of(p)) + align256((size_t)units_of(p) * 4);  // strip flags
   b += 8 * align256(N * 4);               // A and B
   b += 3 * align256(N * 4);               // z, n, hmax
   b += align256(N);                       // mask
@@ -166,7 +171,11 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
 int check_params(fs_pool *P, const fs_params *p) {
   if (!p) return fail(P, FS_E_ARG, "params is NULL");
   if (p->nx < 2 || p->ny < 2) return fail(P, FS_E_ARG, "nx and ny must be >= 2 (got %d x %d)", p->nx, p->ny);
-  if ((int64_t)p->nx * p->ny > (int64_t)1 << 31) return fail(P, FS_E_ARG, "grid too large");
+  // every plane is addressed with 32-bit byte offsets: 4-byte planes up to 2^30 cells, the LTS
+  // 16-byte reconstruction planes up to 2^28
+  if ((int64_t)p->nx * p->ny > ((int64_t)1 << (p->lts ? 28 : 30)))
+    return fail(P, FS_E_ARG, "grid too large: %d x %d (at most 2^30 cells, 2^28 with local time stepping)", p->nx,
+                p->ny);
   if (!(p->dx > 0)) return fail(P, FS_E_ARG, "dx must be > 0");
   if (!(p->cfl > 0) || !(p->hdry > 0) || !(p->dtDry > 0)) return fail(P, FS_E_ARG, "cfl, hdry, dtDry must be > 0");
   for (int k = 0; k < 4; ++k)
@@ -194,8 +203,8 @@ int ensure_rec_capacity(fs_pool *P, int n) {
 }
 
 void drop_graph(Group &G) {
-  if (G.graph) { (void)hipGraphExecDestroy(G.graph); G.graph = nullptr; }
-  G.graphK = 0;
+  for (auto &g : G.graphs)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
 int upload_rec(fs_pool *P, int slot) {
@@ -247,6 +256,10 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.inl = S.inl; d.ispeed = S.ispeed; d.nin = S.nin;
     d.hl = S.L ? 4 : kHalo;
     d.own = own;
+    d.rows = rows;
+    d.sxn = S.sxn;
+    d.wet = S.nstrips && !S.p.untiled ? S.swet : nullptr;
+    d.act = S.sact; d.srain = S.srain; d.sopen = S.sopen;
     d.L = S.L; d.tx = S.tx; d.ty = S.ty;
     d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.tileOpen = S.tileOpen; d.tileRf = S.rf ? S.tileRf : nullptr;
     d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
@@ -440,21 +453,26 @@ int launch_steps(fs_pool *P, int K) {
     }
     return P->timing && (P->nTev > 3 * 256 || P->spans.size() > 4096) ? collect_timing(P) : FS_OK;
   }
-  if (!G.graph || G.graphK != K) {
-    drop_graph(G);
-    hipGraph_t g = nullptr;
-    HIPCHK(P, hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
-    int rc = FS_OK;
-    for (int k = 0; k < K && !rc; ++k) rc = one();
-    hipError_t e = hipStreamEndCapture(P->stream, &g);
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(P, FS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&G.graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) { G.graph = nullptr; return fail(P, FS_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
-    G.graphK = K;
+  // a batch is a run of captured power-of-two graphs (K = 64 + 8: the 64- and the 8-step graph)
+  while (K > 0) {
+    int b = 0;
+    while (b + 1 < 16 && (2 << b) <= K) ++b;
+    hipGraphExec_t &ge = G.graphs[b];
+    if (!ge) {
+      hipGraph_t g = nullptr;
+      HIPCHK(P, hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+      int rc = FS_OK;
+      for (int k = 0; k < (1 << b) && !rc; ++k) rc = one();
+      hipError_t e = hipStreamEndCapture(P->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(P, FS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) { ge = nullptr; return fail(P, FS_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+    }
+    HIPCHK(P, hipGraphLaunch(ge, P->stream));
+    K -= 1 << b;
   }
-  HIPCHK(P, hipGraphLaunch(G.graph, P->stream));
   return FS_OK;
 }
 
@@ -525,8 +543,22 @@ int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n
   constexpr int kMaxBatch = 256;
   int64_t left = K;
   int stalls = 0;
+  std::vector<double> recent(gs.size(), 0.0);   // mean step of each slot's last batch
   for (;;) {
-    const int kb = (int)std::min<int64_t>(left, kMaxBatch);
+    int kb = (int)std::min<int64_t>(left, kMaxBatch);
+    if (!once) {
+      // size an advance's batch to the steps still needed at the mean step so far, so that few
+      // zero-length steps run past the targets (the first batch, with no mean yet, takes K)
+      double need = 0.0;
+      bool known = true;
+      for (size_t i = 0; i < gs.size(); ++i) {
+        const Slot &S = P->slots[gs[i]];
+        const double mean = recent[i] > 0.0 ? recent[i] : (S.dtSteps > 0 ? S.dtSum / (double)S.dtSteps : 0.0);
+        if (mean > 0.0) need = std::max(need, (gt[i] - S.hrec.t) / mean);
+        else known = false;
+      }
+      if (known) kb = (int)std::max<double>(1.0, std::min<double>(kb, std::ceil(need * 1.02) + 1.0));
+    }
     rc = launch_steps(P, kb);
     if (rc) return rc;
     if (once) left -= kb;
@@ -538,7 +570,11 @@ int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n
       if (S.hrec.t > tprev[i]) {
         progress = true;
         const int64_t ds = S.hrec.steps - S.dtSteps;
-        if (ds > 0) { S.dtSum += S.hrec.t - tprev[i]; S.dtSteps = S.hrec.steps; }
+        if (ds > 0) {
+          S.dtSum += S.hrec.t - tprev[i];
+          S.dtSteps = S.hrec.steps;
+          recent[i] = (S.hrec.t - tprev[i]) / (double)ds;
+        }
       }
       tprev[i] = S.hrec.t;
       if (S.hrec.t < gt[i]) done = false;
@@ -546,6 +582,22 @@ int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n
     if (done || (once && left <= 0)) break;
     if (!progress && ++stalls > 3) return fail(P, FS_E_STALL, "GPU solver made no progress (device error?)");
   }
+  return FS_OK;
+}
+
+// strip of interior cell (i, j) under global stepping (work items run x fastest)
+inline int strip_of(const Slot &S, int i, int j) { return (j / S.rows) * S.sxn + i / strip_own(&S.p); }
+
+// per strip: an open cell with a positive rain multiplier (all open cells without a rain field)
+int upload_strip_rain(fs_pool *P, Slot &S) {
+  if (!S.nstrips) return FS_OK;
+  std::vector<uint8_t> sr(S.nstrips, 0);
+  for (int j = 0; j < S.ny; ++j)
+    for (int i = 0; i < S.nx; ++i) {
+      const size_t k = (size_t)j * S.nx + i;
+      if ((S.hmask[k] & MASK_OPEN) && (S.hrf.empty() || S.hrf[k] > 0.f)) sr[strip_of(S, i, j)] = 1;
+    }
+  HIPCHK(P, hipMemcpy(S.srain, sr.data(), (size_t)S.nstrips, hipMemcpyHostToDevice));
   return FS_OK;
 }
 
@@ -606,6 +658,11 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     S.umask = (uint8_t *)take((size_t)S.nunits);
     S.ucells = (int32_t *)take((size_t)S.nunits * 4 * (kMaxLevels + 1));
     S.ulist = (int32_t *)take((size_t)S.nunits * 4 * (kMaxLevels + 1));
+  } else {
+    S.sxn = (nx + strip_own(p) - 1) / strip_own(p);
+    S.nstrips = units_of(p);
+    S.swet = (uint8_t *)take((size_t)S.nstrips); S.sact = (uint8_t *)take((size_t)S.nstrips);
+    S.srain = (uint8_t *)take((size_t)S.nstrips); S.sopen = (int32_t *)take((size_t)S.nstrips * 4);
   }
   S.used = true;
   P->used += need;
@@ -656,6 +713,17 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     for (size_t k = 0; k < N; ++k) { S.hrf[k] = (float)rainField[k]; if (S.hmask[k] & MASK_OPEN) rainArea += S.hrf[k]; }
     HIPCHK(P, hipMemcpy(S.rf, S.hrf.data(), N * 4, hipMemcpyHostToDevice));
   } else rainArea = open;
+  if (S.nstrips) {
+    std::vector<int32_t> so(S.nstrips, 0);
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        if (S.hmask[(size_t)j * nx + i] & MASK_OPEN) so[strip_of(S, i, j)] += 1;
+    HIPCHK(P, hipMemcpy(S.sopen, so.data(), (size_t)S.nstrips * 4, hipMemcpyHostToDevice));
+    HIPCHK(P, hipMemset(S.swet, 0, (size_t)S.nstrips));
+    HIPCHK(P, hipMemset(S.sact, 1, (size_t)S.nstrips));
+    rc = upload_strip_rain(P, S);
+    if (rc) return rc;
+  }
   // zero state (A and B, contiguous) and max depth; the static layers sit between them
   HIPCHK(P, hipMemset(S.h, 0, (size_t)((char *)S.z - (char *)S.h)));
   HIPCHK(P, hipMemset(S.hmax, 0, N * 4));
@@ -695,9 +763,20 @@ int set_state(fs_pool *P, int slot, const double *h, const double *hu, const dou
     const bool open = S.hmask[k] & MASK_OPEN;
     const double d = h ? h[k] : 0.0;
     th[k] = open ? (float)std::max(0.0, d) : 0.f;  // setDepth: mask ? max(0, fn) : 0
-    tu[k] = hu ? (float)hu[k] : 0.f;
-    tv[k] = hv ? (float)hv[k] : 0.f;
+    tu[k] = hu && open ? (float)hu[k] : 0.f;        // setVelocity: h * u, so 0 in walls
+    tv[k] = hv && open ? (float)hv[k] : 0.f;
     if (open) vol += (double)th[k];
+  }
+  if (S.nstrips) {
+    std::vector<uint8_t> wet(S.nstrips, 0);
+    for (int j = 0; j < S.ny; ++j)
+      for (int i = 0; i < S.nx; ++i) {
+        const size_t k = (size_t)j * S.nx + i;
+        if (th[k] != 0.f || tu[k] != 0.f || tv[k] != 0.f) wet[strip_of(S, i, j)] = 1;
+      }
+    HIPCHK(P, hipMemcpyAsync(S.swet, wet.data(), (size_t)S.nstrips, hipMemcpyHostToDevice, P->stream));
+    HIPCHK(P, hipStreamSynchronize(P->stream));
+    HIPCHK(P, hipMemsetAsync(S.sact, 1, (size_t)S.nstrips, P->stream));
   }
   HIPCHK(P, hipMemcpyAsync(S.h, th.data(), N * 4, hipMemcpyHostToDevice, P->stream));
   HIPCHK(P, hipMemcpyAsync(S.hu, tu.data(), N * 4, hipMemcpyHostToDevice, P->stream));
@@ -1059,10 +1138,15 @@ int fs_set_rain_field(fs_handle s, const double *field) {
   double area = 0.0;
   for (size_t k = 0; k < N; ++k) { S.hrf[k] = (float)field[k]; if (S.hmask[k] & MASK_OPEN) area += S.hrf[k]; }
   HIPCHK(P, hipMemcpy(S.rf, S.hrf.data(), N * 4, hipMemcpyHostToDevice));
-  int rc = download_recs(P, {s->slot});
+  int rc = upload_strip_rain(P, S);
+  if (rc) return rc;
+  rc = download_recs(P, {s->slot});
   if (rc) return rc;
   S.hrec.rainArea = area * S.p.dx * S.p.dx;
-  return upload_rec(P, s->slot) || hipStreamSynchronize(P->stream);
+  rc = upload_rec(P, s->slot);
+  if (rc) return rc;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  return FS_OK;
 }
 
 int fs_set_inlet_rates(fs_handle s, const double *q, int32_t n) {
@@ -1126,7 +1210,10 @@ int fs_lts_levels(fs_handle s, uint8_t *levels, int32_t *tx, int32_t *ty) {
 int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total) {
   if (!s) return FS_E_ARG;
   Slot &S = s->pool->slots[s->slot];
-  if (active) *active = S.openCells;
+  int rc = download_recs(s->pool, {s->slot});
+  if (rc) return rc;
+  // global stepping: open cells of the strips the last step advanced; LTS: all open cells
+  if (active) *active = S.nstrips ? S.hrec.active : S.openCells;
   if (total) *total = (int64_t)S.nx * S.ny;
   return FS_OK;
 }

@@ -193,6 +193,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   p.bc[3] = (int32_t)num_prop(env, a[0], "bcNorth", 0);
   p.lts = (int32_t)num_prop(env, a[0], "lts", 0);
   p.rows = (int32_t)num_prop(env, a[0], "rows", 0);
+  p.untiled = (int32_t)num_prop(env, a[0], "untiled", 0);
   if (p.nx <= 0 || p.ny <= 0) {
     napi_throw_range_error(env, NULL, "create: nx and ny must be positive");
     return NULL;

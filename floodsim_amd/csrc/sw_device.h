@@ -14,6 +14,30 @@ constexpr int kMaxLevels = 4;                // LTS levels 1..4 (solver-cpu.js:1
 constexpr double kLtsSafe = 0.9;             // share of its CFL step a tile may take (solver-cpu.js:15)
 
 enum { BC_WALL = 0, BC_OPEN = 1, BC_OUTFLOW = 2 };
+
+// Device loads and stores through the global address space from a wave-uniform row base plus a
+// 32-bit lane offset: global_load/global_store with an SGPR base and a 32-bit VGPR offset, no
+// 64-bit per-lane address arithmetic. A plane holds at most 2^32 bytes (checked at creation).
+#define FS_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T gld(const T *row, int x) {
+  return *(const FS_GLOBAL T *)((const FS_GLOBAL char *)row + (uint32_t)x * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *row, int x, T v) {
+  *(FS_GLOBAL T *)((FS_GLOBAL char *)row + (uint32_t)x * (uint32_t)sizeof(T)) = v;
+}
+// HIP's vector types are classes bound to the generic address space: move them as clang vectors
+typedef float fs_v2 __attribute__((ext_vector_type(2)));
+typedef float fs_v4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gld(const float4 *row, int x) {
+  const fs_v4 v = *(const FS_GLOBAL fs_v4 *)((const FS_GLOBAL char *)row + (uint32_t)x * 16u);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(float2 *row, int x, float2 v) {
+  const fs_v2 t = {v.x, v.y};
+  *(FS_GLOBAL fs_v2 *)((FS_GLOBAL char *)row + (uint32_t)x * 8u) = t;
+}
 enum { MASK_OPEN = 1, MASK_INLET = 2 };      // mask byte bits: open (not a wall) cell, inlet cell
 enum { FLAG_INFIL = 1, FLAG_RF = 2 };
 
@@ -48,6 +72,7 @@ struct StepRec {
   int32_t lcells[kMaxLevels + 1]; // own open cells per level
   int32_t lcount[kMaxLevels + 1]; // work-list length per level
   int32_t records;                // records run so far (diagnostic)
+  int32_t active;                 // open cells the last step advanced (dry strips skipped)
 };
 
 // one inlet: cell index (y*pitch + x), rate (m^3/s, + into the network, - a point inflow)
@@ -76,6 +101,14 @@ struct DevInst {
   float *ispeed;                // wave speed of each inlet cell after the exchange
   int nin;
   int hl, own;                  // strip geometry: halo lanes on the left, owned columns
+  int rows, sxn;                // rows per strip, strips across
+  // dry-strip skipping (global stepping; null = every strip every step). A strip is quiet when
+  // all its cells hold h = w = hu = hv = 0; a step advances a strip only if a strip of its 3x3
+  // block is not quiet or rain falls on it, which leaves the results identical to advancing all.
+  uint8_t *wet;                 // per strip: not quiet after the last step (+ inlet exchange)
+  uint8_t *act;                 // per strip: advanced in the current step
+  const uint8_t *srain;         // per strip: an open cell with a positive rain multiplier
+  const int32_t *sopen;         // per strip: open cells it owns
   // local time stepping (L > 0)
   int L, tx, ty;                // levels; LTS tiles across and down (8x8 cells)
   uint8_t *lv, *lv0;            // tile levels (smoothed, raw)
@@ -93,10 +126,12 @@ struct DevInst {
   int32_t *ulist;               // per level (stride wend - wbeg): work units of the level's records
 };
 
-// per wavefront strip and stage: boundary outflow (m^3), clipped (m^3), infiltration (m^3), max speed
+// per wavefront strip and stage: boundary outflow (m^3), clipped (m^3), infiltration (m^3), max
+// speed, open cells advanced (second stage)
 struct Part {
   double out, clip, infil;
-  float smax, pad;
+  float smax;
+  int32_t cells;
 };
 
 

@@ -208,6 +208,31 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m)
   return p;
 }
 
+// Primitive of interior cell (X, Y) for a row whose 64 lanes are all interior cells (the common
+// case): row-based global loads, no ghost logic
+template <int STAGE, bool LTS>
+__device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, int m) {
+  const size_t row = (size_t)Y * I.pitch;
+  Prim p;
+  float hu, hv;
+  p.lv = 0;
+  if constexpr (LTS) p.lv = gld(I.lv + (size_t)(Y >> 3) * I.tx, X >> 3);
+  bool useB = STAGE == 1;
+  if constexpr (LTS) useB = STAGE == 1 && p.lv == m;
+  if (useB) { p.h = gld(I.h1 + row, X); hu = gld(I.hu1 + row, X); hv = gld(I.hv1 + row, X); }
+  else { p.h = gld(I.h + row, X); hu = gld(I.hu + row, X); hv = gld(I.hv + row, X); }
+  p.hu = hu;
+  p.hv = hv;
+  p.z = gld(I.z + row, X);
+  p.in = true;
+  p.M = (gld(I.mask + row, X) & MASK_OPEN) != 0;
+  const float ih = frcp(p.h);
+  const bool wet = p.h > I.hdry;
+  p.u = wet ? hu * ih : 0.f;
+  p.v = wet ? hv * ih : 0.f;
+  return p;
+}
+
 // MUSCL slopes of (h, eta, un, ut) from the two neighbours along one axis (solver-cpu.js:216-233;
 // kernels.wgsl.js:159-180): first order when the cell or an open neighbour is thinner than hfront;
 // wall neighbours are left out of the slope. dza = z_c - z_a, dzb = z_b - z_c.
@@ -236,6 +261,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+__device__ __forceinline__ int wave_sum_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
 // wavefront index with blocks of one XCD (blockIdx % 8) mapped to one contiguous run of work,
 // so strips that share halo rows and columns share an L2
@@ -255,11 +284,12 @@ __device__ __noinline__ float cell_speed(float h, float hu, float hv, float g) {
 // Friction, rain, infiltration and max depth of one cell after its update, then the cell's wave
 // speed for the next dt (sourcesAt, kernels.wgsl.js:544-590; _sources, solver-cpu.js:456-483).
 // Semi-implicit Manning: (hu, hv) /= 1 + dt g n^2 |u| / h^(4/3).
-__device__ __forceinline__ void sources_cell(const DevInst &I, size_t k, float dt, float rain, float areaf, float &qh,
-                                             float &qw, float &qhu, float &qhv, double &acc_inf, float &smax) {
+__device__ __forceinline__ void sources_cell(const DevInst &I, size_t row, int X, float dt, float rain, float areaf,
+                                             float &qh, float &qw, float &qhu, float &qhv, double &acc_inf,
+                                             float &smax) {
   const float g = I.g, hdry = I.hdry;
   const float d = qh;
-  const float nn = I.n[k];
+  const float nn = gld(I.n + row, X);
   if (d > hdry && nn > 0.f) {
     const float id = frcp(d);
     const float u = qhu * id, v = qhv * id;
@@ -269,18 +299,18 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, size_t k, float d
     qhu *= kf;
     qhv *= kf;
   }
-  const float rr = (I.flags & FLAG_RF) ? rain * I.rf[k] * dt : rain * dt;
+  const float rr = (I.flags & FLAG_RF) ? rain * gld(I.rf + row, X) * dt : rain * dt;
   float inf = 0.f;
-  if (I.flags & FLAG_INFIL) inf = fminf(I.infil[k] * dt, d + rr);
+  if (I.flags & FLAG_INFIL) inf = fminf(gld(I.infil + row, X) * dt, d + rr);
   float nh = qh, nw = qw;
   kadd(nh, nw, rr - inf);
   if (nh < 0.f) { inf = d - qw + rr; qh = 0.f; qw = 0.f; }
   else { qh = nh; qw = nw; }
   acc_inf += (double)inf * areaf;
   if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
-  const float hm = I.hmax[k];
-  if (qh > hm) I.hmax[k] = qh;
-  if (qh > hdry && !(I.mask[k] & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
+  const float hm = gld(I.hmax + row, X);
+  if (qh > hm) gst(I.hmax + row, X, qh);
+  if (qh > hdry && !(gld(I.mask + row, X) & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
 }
 
 // wavefront index for a device-side count of n work items: the blocks that have work are spread
@@ -307,8 +337,8 @@ struct FaceRow { float F0, F1, F2, cL, cR, on; };
 // stepping cell takes, for a face whose neighbour is finer, its reconstruction from that face's last
 // evaluation (kept in bsx/bsy by the finer record's last stage).
 template <int STAGE, bool LTS>
-__device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
-                                          Part *__restrict__ part) {
+__device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
+                                          Part *__restrict__ part, const int cells) {
   const float rain = I.rec->rainRate;
   const float g = I.g, hdry = I.hdry, hfront = I.hfront, inv_dx = I.inv_dx, dx = I.dx;
   const int so = I.secondOrder, vl = I.limiter;
@@ -317,6 +347,7 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
   const bool own_col = lane >= I.hl && lane < I.hl + I.own && X < nx;
   const int y0 = wk.z, y1 = min(wk.z + wk.w, ny);
   const bool last = STAGE == 1 || !so;   // the stage that ends the step (or the record)
+  const bool xin = wk.y - I.hl >= 0 && wk.y - I.hl + kLanes <= nx;  // every lane an interior column
 
   // writes go to B in stage 0 and to A in stage 1; Ws is the residual of the evaluated state
   const float *__restrict__ Ws = STAGE == 0 ? I.w : I.w1;
@@ -345,10 +376,13 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
 
   double acc_out = 0.0, acc_clip = 0.0, acc_inf = 0.0;
   float smax = 0.f;
+  bool stir = false;   // an owned cell ends with h, w, hu or hv non-zero
 
   for (int r = y0 - kHalo; r < y1 + kHalo; ++r) {
     // ---------------- row r: primitives
-    const Prim p = load_prim<STAGE, LTS>(I, X, r, m);
+    Prim p;
+    if (xin && r >= 0 && r < ny) p = load_prim_row<STAGE, LTS>(I, X, r, m);
+    else p = load_prim<STAGE, LTS>(I, X, r, m);
 
     // ---------------- x reconstruction of row r (neighbours by lane shifts)
     const float hl = from_left(p.h), ul = from_left(p.u), vlf = from_left(p.v), zl = from_left(p.z);
@@ -377,10 +411,9 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
       // a waiting cell keeps its reconstruction toward the faces this record evaluates (last stage)
       const float onW = from_left(xf0.on);
       if (last && own_col && p.in && p.M && p.lv == m + 1 && r >= y0 && r < y1) {
-        const size_t k = (size_t)r * pitch + X;
-        float2 *b = reinterpret_cast<float2 *>(&I.bsx[k]);
-        if (onW > 0.f) b[0] = make_float2(rx0.x, rx0.y);
-        if (xf0.on > 0.f) b[1] = make_float2(rx0.z, rx0.w);
+        float2 *b = reinterpret_cast<float2 *>(I.bsx + (size_t)r * pitch);
+        if (onW > 0.f) gst(b, 2 * X, make_float2(rx0.x, rx0.y));
+        if (xf0.on > 0.f) gst(b, 2 * X + 1, make_float2(rx0.z, rx0.w));
       }
     }
 
@@ -403,10 +436,9 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
       const int ry = r - 1;
       const bool onN = min(lv1, p.lv) == m;
       if (last && own_col && in1 && M1r && lv1 == m + 1 && ry >= y0 && ry < y1) {
-        const size_t k = (size_t)ry * pitch + X;
-        float2 *b = reinterpret_cast<float2 *>(&I.bsy[k]);
-        if (yf2.on > 0.f) b[0] = make_float2(ry1.x, ry1.y);
-        if (onN) b[1] = make_float2(ry1.z, ry1.w);
+        float2 *b = reinterpret_cast<float2 *>(I.bsy + (size_t)ry * pitch);
+        if (yf2.on > 0.f) gst(b, 2 * X, make_float2(ry1.x, ry1.y));
+        if (onN) gst(b, 2 * X + 1, make_float2(ry1.z, ry1.w));
       }
     }
 
@@ -421,9 +453,9 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
         take = take && (lv2 == m || lv2 == m + 1);
         // a waiting cell, or a stepping one in its first stage, holds its water plus its store
         if (take && (lv2 == m + 1 || STAGE == 0)) {
-          const size_t k2 = (size_t)(r - 2) * pitch + X;
-          float st = I.sh[k2] - I.sw[k2];
-          if (STAGE == 1 && I.gflag[k2]) st += I.g0h[k2];  // the first stage's gather, not yet folded
+          const size_t row2 = (size_t)(r - 2) * pitch;
+          float st = gld(I.sh + row2, X) - gld(I.sw + row2, X);
+          if (STAGE == 1 && gld(I.gflag + row2, X)) st += gld(I.g0h + row2, X);  // first-stage gather, not yet folded
           hv_ = fmaxf(0.f, h2r + st);
         }
       }
@@ -463,7 +495,7 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
         upd = upd && (own || coarse);
       }
       if (upd) {
-        const size_t k = (size_t)ry * pitch + X;
+        const size_t row = (size_t)ry * pitch;
         float bx = sx3, by = sy3;
         if constexpr (LTS) {
           if (own) {
@@ -471,12 +503,12 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
             // neighbour contributes the values of its last evaluation
             float4 xs = rx3, ys = ry3;
             if (kw == 0.f || ke == 0.f) {
-              const float4 b = I.bsx[k];
+              const float4 b = gld(I.bsx + row, X);
               if (kw == 0.f) { xs.x = b.x; xs.y = b.y; }
               if (ke == 0.f) { xs.z = b.z; xs.w = b.w; }
             }
             if (ks == 0.f || kn == 0.f) {
-              const float4 b = I.bsy[k];
+              const float4 b = gld(I.bsy + row, X);
               if (ks == 0.f) { ys.x = b.x; ys.y = b.y; }
               if (kn == 0.f) { ys.z = b.z; ys.w = b.w; }
             }
@@ -496,19 +528,23 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
           // during this kernel; sw_lts_fold adds the deltas once the record's stages are done)
           if (kw + ke + ks + kn > 0.f) {
             const float wd = so ? 0.5f * dt : dt;
-            if (STAGE == 0) { I.g0h[k] = wd * dh; I.g0hu[k] = wd * dhu; I.g0hv[k] = wd * dhv; I.gflag[k] = 1; }
-            else { I.g1h[k] = wd * dh; I.g1hu[k] = wd * dhu; I.g1hv[k] = wd * dhv; }
+            if (STAGE == 0) {
+              gst(I.g0h + row, X, wd * dh); gst(I.g0hu + row, X, wd * dhu); gst(I.g0hv + row, X, wd * dhv);
+              gst(I.gflag + row, X, (uint8_t)1);
+            } else {
+              gst(I.g1h + row, X, wd * dh); gst(I.g1hu + row, X, wd * dhu); gst(I.g1hv + row, X, wd * dhv);
+            }
           }
         } else {
-          float qh = hh3, qhu = hu3, qhv = hv3, qw = Ws[k];
+          float qh = hh3, qhu = hu3, qhv = hv3, qw = gld(Ws + row, X);
           // RK start state: U^n, plus (LTS) what finer neighbours passed while the cell waited
           float h0 = 0.f, w0n = 0.f, hu0 = 0.f, hv0 = 0.f;
           if (STAGE == 1 || LTS) {
-            h0 = I.h[k]; w0n = I.w[k]; hu0 = I.hu[k]; hv0 = I.hv[k];
+            h0 = gld(I.h + row, X); w0n = gld(I.w + row, X); hu0 = gld(I.hu + row, X); hv0 = gld(I.hv + row, X);
             if constexpr (LTS) {
-              const float sh = I.sh[k], sw = I.sw[k];
-              hu0 += I.shu[k];
-              hv0 += I.shv[k];
+              const float sh = gld(I.sh + row, X), sw = gld(I.sw + row, X);
+              hu0 += gld(I.shu + row, X);
+              hv0 += gld(I.shv + row, X);
               kadd(h0, w0n, sh - sw);
             }
           }
@@ -520,7 +556,11 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
           } else {
             qhu = 0.5f * (hu0 + qhu + dt * dhu);
             qhv = 0.5f * (hv0 + qhv + dt * dhv);
-            kadd(qh, qw, 0.5f * ((h0 - w0n) - (qh - qw) + dt * dh));
+            // Heun average of the true depths (h - w) as one compensated increment (kernels.wgsl.js:
+            // 493-498), with the stored depths differenced first and the residuals kept apart:
+            // (h0 - w0) - (h* - w*) rounded at the depth's ulp drops the residuals every step, a drift
+            // of 1e-6 of the volume over 10^4 steps of a point inflow
+            kadd(qh, qw, 0.5f * (((h0 - qh) + (qw - w0n)) + dt * dh));
           }
           if (qh < 0.f) { acc_clip += -(double)(qh - qw) * areaf; qh = 0.f; qw = 0.f; }
           if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
@@ -534,10 +574,13 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
             acc_out += o * ((double)dx * (double)dt * (double)wgt);
           }
           if (STAGE == 1) {
-            sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
-            if constexpr (LTS) { I.sh[k] = 0.f; I.shu[k] = 0.f; I.shv[k] = 0.f; I.sw[k] = 0.f; }
+            sources_cell(I, row, X, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+            if constexpr (LTS) {
+              gst(I.sh + row, X, 0.f); gst(I.shu + row, X, 0.f); gst(I.shv + row, X, 0.f); gst(I.sw + row, X, 0.f);
+            }
           }
-          Ho[k] = qh; HUo[k] = qhu; HVo[k] = qhv; Wo[k] = qw;
+          gst(Ho + row, X, qh); gst(HUo + row, X, qhu); gst(HVo + row, X, qhv); gst(Wo + row, X, qw);
+          stir = stir || qh != 0.f || qw != 0.f || qhu != 0.f || qhv != 0.f;
         }
       }
     }
@@ -557,7 +600,39 @@ __device__ __forceinline__ void run_strip(const DevInst &I, const int4 wk, const
 
   const double o = wave_sum(acc_out), c = wave_sum(acc_clip), f = wave_sum(acc_inf);
   const float s = wave_max(smax);
-  if (lane == 0) *part = Part{o, c, f, s, 0.f};
+  if (lane == 0) *part = Part{o, c, f, s, cells};
+  return __any(stir);
+}
+
+// ------------------------------------------------------------ dry strips (global stepping)
+// advance strip u this step: rain falls on it, or a strip of its 3x3 block is not quiet
+// (the stencil of a whole step reaches 6 cells, less than a strip's width and height)
+__device__ __forceinline__ bool strip_active(const DevInst &I, int u) {
+  if (I.rec->rainRate > 0.f && I.srain[u]) return true;
+  const int sx = I.sxn, sy = (I.wend - I.wbeg) / I.sxn;
+  const int ux = u % sx, uy = u / sx;
+  bool any = false;
+  for (int j = max(uy - 1, 0); j <= min(uy + 1, sy - 1); ++j)
+    for (int i = max(ux - 1, 0); i <= min(ux + 1, sx - 1); ++i) any = any || I.wet[j * sx + i] != 0;
+  return any;
+}
+
+// first stage: decide whether strip u steps; a strip that stops stepping clears its U* once, so the
+// neighbours' second stage reads the zero state a computed first stage would have left there
+__device__ __forceinline__ bool strip_begin(const DevInst &I, const int4 wk, int u, int lane) {
+  const bool active = strip_active(I, u);
+  const bool was = I.act[u] != 0;
+  if (active == was) return active;
+  if (lane == 0) I.act[u] = active ? 1 : 0;
+  if (!active) {
+    const int X = wk.y - I.hl + lane;
+    if (lane >= I.hl && lane < I.hl + I.own && X < I.nx)
+      for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
+        const size_t k = (size_t)y * I.pitch + X;
+        I.h1[k] = 0.f; I.hu1[k] = 0.f; I.hv1[k] = 0.f; I.w1[k] = 0.f;
+      }
+  }
+  return active;
 }
 
 // One RK stage over a work list. Global stepping: one wavefront per strip of `work`.
@@ -574,11 +649,15 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     const DevInst &I = insts[wk.x];
     if (STAGE == 1 && !I.secondOrder) return;  // first order: sw_euler_finish writes this strip's partial
     const float dt = I.rec->dt;
-    if (!(dt > 0.f)) {
-      if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0.f};
+    const int u = wid - I.wbeg;
+    bool active = dt > 0.f;
+    if (active && I.wet) active = STAGE == 0 ? strip_begin(I, wk, u, lane) : I.act[u] != 0;
+    if (!active) {
+      if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
       return;
     }
-    run_strip<STAGE, false>(I, wk, lane, 0, dt, part + wid);
+    const bool stir = run_strip<STAGE, false>(I, wk, lane, 0, dt, part + wid, I.sopen[u]);
+    if (STAGE == 1 && I.wet && lane == 0) I.wet[u] = stir ? 1 : 0;
   } else {
     const DevInst &I = insts[0];
     if (STAGE == 1 && !I.secondOrder) return;
@@ -586,11 +665,11 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     const int i = xcd_wave_index_n(I.rec->lcount[m], 4);
     if (i < 0) return;
     if (!(dt > 0.f)) {
-      if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0.f};
+      if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0};
       return;
     }
     const int u = I.ulist[m * (I.wend - I.wbeg) + i];
-    run_strip<STAGE, true>(I, work[I.wbeg + u], lane, m, dt, part + i);
+    run_strip<STAGE, true>(I, work[I.wbeg + u], lane, m, dt, part + i, 0);
   }
 }
 
@@ -853,32 +932,36 @@ __global__ __launch_bounds__(256) void sw_speed(const DevInst *__restrict__ inst
     }
   }
   const float s = wave_max(smax);
-  if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, s, 0.f};
+  if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, s, 0};
 }
 
 // first order (secondOrder = false): the Euler stage's U* plus the sources becomes U^{n+1}
 // (step, solver-cpu.js:432-438: one stage, then _sources). LTS: the level-m cells of a record,
 // whose store is then spent (solver-cpu.js:550-557).
 template <bool LTS>
-__device__ __forceinline__ void euler_finish_strip(const DevInst &I, const int4 wk, int lane, int m, float dt, Part *part) {
+__device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 wk, int lane, int m, float dt, Part *part,
+                                                   int cells) {
   double acc_inf = 0.0;
   float smax = 0.f;
+  bool stir = false;
   const int X = wk.y - I.hl + lane;
   if (dt > 0.f && lane >= I.hl && lane < I.hl + I.own && X < I.nx) {
     const float rain = I.rec->rainRate, areaf = I.dx * I.dx;
     for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
-      const size_t k = (size_t)y * I.pitch + X;
-      if (!(I.mask[k] & MASK_OPEN)) continue;
+      const size_t row = (size_t)y * I.pitch;
+      if (!(gld(I.mask + row, X) & MASK_OPEN)) continue;
       if (LTS && tile_level(I, X, y) != m) continue;
-      float qh = I.h1[k], qw = I.w1[k], qhu = I.hu1[k], qhv = I.hv1[k];
-      sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
-      I.h[k] = qh; I.w[k] = qw; I.hu[k] = qhu; I.hv[k] = qhv;
-      if (LTS) { I.sh[k] = 0.f; I.shu[k] = 0.f; I.shv[k] = 0.f; I.sw[k] = 0.f; }
+      float qh = gld(I.h1 + row, X), qw = gld(I.w1 + row, X), qhu = gld(I.hu1 + row, X), qhv = gld(I.hv1 + row, X);
+      sources_cell(I, row, X, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+      gst(I.h + row, X, qh); gst(I.w + row, X, qw); gst(I.hu + row, X, qhu); gst(I.hv + row, X, qhv);
+      if (LTS) { gst(I.sh + row, X, 0.f); gst(I.shu + row, X, 0.f); gst(I.shv + row, X, 0.f); gst(I.sw + row, X, 0.f); }
+      stir = stir || qh != 0.f || qw != 0.f || qhu != 0.f || qhv != 0.f;
     }
   }
   const double f = wave_sum(acc_inf);
   const float s = wave_max(smax);
-  if (lane == 0) *part = Part{0.0, 0.0, f, s, 0.f};
+  if (lane == 0) *part = Part{0.0, 0.0, f, s, cells};
+  return __any(stir);
 }
 
 template <bool LTS>
@@ -892,14 +975,21 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
     const int4 wk = work[wid];
     const DevInst &I = insts[wk.x];
     if (I.secondOrder) return;  // second order: sw_stage<1> writes this strip's partial
-    euler_finish_strip<false>(I, wk, lane, 0, I.rec->dt, part + wid);
+    const int u = wid - I.wbeg;
+    if (I.wet && !I.act[u]) {
+      if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
+      return;
+    }
+    const float dt = I.rec->dt;
+    const bool stir = euler_finish_strip<false>(I, wk, lane, 0, dt, part + wid, dt > 0.f ? I.sopen[u] : 0);
+    if (I.wet && dt > 0.f && lane == 0) I.wet[u] = stir ? 1 : 0;
   } else {
     const DevInst &I = insts[0];
     if (I.secondOrder) return;
     const float dt = I.rec->ldt[m];
     const int i = xcd_wave_index_n(I.rec->lcount[m], 4);
     if (i < 0) return;
-    euler_finish_strip<true>(I, work[I.wbeg + I.ulist[m * (I.wend - I.wbeg) + i]], lane, m, dt, part + i);
+    euler_finish_strip<true>(I, work[I.wbeg + I.ulist[m * (I.wend - I.wbeg) + i]], lane, m, dt, part + i, 0);
   }
 }
 
@@ -928,6 +1018,11 @@ __global__ void sw_exchange(const DevInst *__restrict__ insts, int inst, int m) 
     if (h <= I.hdry) { hu = 0.f; hv = 0.f; }
     I.h[k] = h; I.w[k] = w; I.hu[k] = hu; I.hv[k] = hv;
     q.vol += (double)dd * (double)A;
+    // water put into a quiet strip wakes it for the next step
+    if (!LTS && I.wet && (h != 0.f || w != 0.f)) {
+      const int x = (int)(k % I.pitch), y = (int)(k / I.pitch);
+      I.wet[(y / I.rows) * I.sxn + x / I.own] = 1;
+    }
   }
   I.ispeed[j] = h > I.hdry ? cell_speed(h, hu, hv, I.g) : 0.f;
 }
@@ -944,8 +1039,10 @@ __global__ __launch_bounds__(256) void sw_dt_final(const DevInst *__restrict__ i
   const int t = threadIdx.x;
   __shared__ double s_o[4], s_c[4], s_f[4];
   __shared__ float s_m[4];
+  __shared__ int s_n[4];
   double o = 0.0, c = 0.0, f = 0.0;
   float m = 0.f;
+  int cells = 0;
   const bool fresh = mode == 0 || rec->dirty;
   if (fresh) {
     for (int k = I.wbeg + t; k < I.wend; k += blockDim.x) {
@@ -956,16 +1053,20 @@ __global__ __launch_bounds__(256) void sw_dt_final(const DevInst *__restrict__ i
         o += a.out + b.out;
         c += a.clip + b.clip;
         f += b.infil;
+        cells += b.cells;
       }
     }
     if (mode == 0)
       for (int k = t; k < I.nin; k += blockDim.x) m = fmaxf(m, I.ispeed[k]);
   }
   o = wave_sum(o); c = wave_sum(c); f = wave_sum(f); m = wave_max(m);
-  if ((t & 63) == 0) { s_o[t >> 6] = o; s_c[t >> 6] = c; s_f[t >> 6] = f; s_m[t >> 6] = m; }
+  cells = wave_sum_i(cells);
+  if ((t & 63) == 0) { s_o[t >> 6] = o; s_c[t >> 6] = c; s_f[t >> 6] = f; s_m[t >> 6] = m; s_n[t >> 6] = cells; }
   __syncthreads();
   if (t != 0) return;
-  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) { o += s_o[k]; c += s_c[k]; f += s_f[k]; m = fmaxf(m, s_m[k]); }
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+    o += s_o[k]; c += s_c[k]; f += s_f[k]; m = fmaxf(m, s_m[k]); cells += s_n[k];
+  }
   // a no-op step (dt = 0, past tEnd) measured nothing: keep the last real max speed
   if (!fresh || (mode == 0 && !(rec->dt > 0.f))) m = rec->smax;
   rec->dirty = 0;
@@ -979,7 +1080,8 @@ __global__ __launch_bounds__(256) void sw_dt_final(const DevInst *__restrict__ i
     rec->t = rec->tAfter;
     rec->steps += 1;
     rec->lastDt = dtp;
-    rec->cellUpdates += (double)rec->openCells;
+    rec->cellUpdates += (double)cells;
+    rec->active = cells;
   }
   rec->smax = m;
   const double rem = rec->tEnd - rec->t;

@@ -80,7 +80,7 @@ class GPUSolver {
       nx: o.nx, ny: o.ny, dx: o.dx, g: this.g, cfl: this.cfl, hdry: this.hdry, hfront: this.hfront,
       dtDry: this.dtDry, secondOrder: this.secondOrder ? 1 : 0, limiter: this.limiter,
       bcWest: BC[this.bc.west], bcEast: BC[this.bc.east], bcSouth: BC[this.bc.south], bcNorth: BC[this.bc.north],
-      lts: this.lts, rows: o.rows | 0,
+      lts: this.lts, rows: o.rows | 0, untiled: this.tiled ? 0 : 1,
     };
     for (const side of ['west', 'east', 'south', 'north']) {
       if (BC[this.bc[side]] === undefined) throw new Error('GPUSolver (HIP): bc.' + side + ' = ' + this.bc[side]);

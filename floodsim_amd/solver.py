@@ -33,6 +33,7 @@ class fs_params(ctypes.Structure):
         ("hdry", ctypes.c_double), ("hfront", ctypes.c_double), ("dtDry", ctypes.c_double),
         ("secondOrder", ctypes.c_int32), ("limiter", ctypes.c_int32),
         ("bc", ctypes.c_int32 * 4), ("lts", ctypes.c_int32), ("rows", ctypes.c_int32),
+        ("untiled", ctypes.c_int32),
     ]
 
 
@@ -125,7 +126,7 @@ def _f64(a, n):
 
 
 def make_params(nx, ny, dx, bc=None, secondOrder=True, limiter="minmod", cfl=None, hdry=1e-5, hfront=1e-3,
-                dtDry=1.0, g=9.81, lts=0, rows=0, **_):
+                dtDry=1.0, g=9.81, lts=0, rows=0, tiles=True, **_):
     """fs_params with the defaults of GPUSolver (solver-gpu.js:39-47)."""
     p = fs_params()
     p.nx, p.ny, p.dx, p.g = int(nx), int(ny), float(dx), float(g)
@@ -141,6 +142,7 @@ def make_params(nx, ny, dx, bc=None, secondOrder=True, limiter="minmod", cfl=Non
         lts = 3
     p.lts = int(lts or 0)
     p.rows = int(rows or 0)
+    p.untiled = 0 if tiles else 1   # tiles=False: every strip every step (the reference's o.tiles)
     return p
 
 

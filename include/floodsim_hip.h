@@ -46,6 +46,8 @@ typedef struct fs_params {
   int32_t bc[4];       /* west, east, south, north: FS_BC_* */
   int32_t lts;         /* local time stepping levels 0..4 (0 = global steps) */
   int32_t rows;        /* rows per wavefront strip (0 = autotuned) */
+  int32_t untiled;     /* 1: advance every strip every step (the reference's tiles: false); 0: skip
+                          quiet dry strips (the reference's active tiles, kernels.wgsl.js:639-673) */
 } fs_params;
 
 /* clock fields of GPUSolver: t, steps, lastDt, dtAvg (solver-gpu.js:578-586) */
@@ -89,7 +91,8 @@ int fs_clock_get(fs_handle s, fs_clock *clk);
 /* LTS tile levels of the last cycle (8x8 tiles, tx*ty bytes, row-major): the reference keeps them
  * in its TL buffer (kernels.wgsl.js:307) and CPUSolver.LV (solver-cpu.js:124) */
 int fs_lts_levels(fs_handle s, uint8_t *levels, int32_t *tx, int32_t *ty);
-/* tileStats (:625-629): open cells advanced by the last step and the interior cell total */
+/* tileStats (:625-629): open cells advanced by the last step (global stepping skips quiet dry
+ * strips) and the interior cell total */
 int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total);
 
 /* ---- batched / served instances (no counterpart in the reference, which runs one

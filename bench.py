@@ -11,7 +11,11 @@ state stays resident in HBM throughout.
 
 Prints ONE JSON line: cell-updates/s (open cells advanced per second), the roofline of the
 dominant kernel (second RK stage + sources) measured with HIP events over the timed region, and
-the FP64 CPU oracle timed on a bounded window of the same state as the CPU baseline.
+the FP64 CPU oracle timed on a bounded window of the same state as the CPU baseline. Two more
+legs follow (skip with --no-batch / --no-serve): `batched`, 256 uneven urban instances (sides
+128-1024) solved for 300 s from dry in one pool vs one at a time (solves/hour), with the smallest
+checked against the oracle; `served`, a Poisson stream (1 request/s, 24 requests, sides
+256-4096, 30-minute storms) under continuous batching (p50/p99 latency, solves/hour).
 """
 import argparse
 import json
@@ -127,6 +131,72 @@ def cpu_baseline(w, state, lts=0, window=1024, steps=None, budget_s=20.0):
                        f"spun-up state, {k} {kind} in {el:.1f} s")
 
 
+def bench_batched(count=256, t_end=300.0, seed=7):
+    """Config 4: `count` uneven urban instances (sides log-uniform 128-1024, own seeds, rain 40 mm/h
+    x LogNormal(0, 0.3), Manning n x U(0.8, 1.2)) solved from dry for t_end simulated seconds, all
+    in one pool (autotuned strip rows), then one at a time; the smallest instance is checked
+    against the FP64 oracle."""
+    import oracle
+    from floodsim_amd import batch
+    from floodsim_amd import scenarios as sc
+    t0 = time.perf_counter()
+    specs = sc.ensemble(count, seed=seed)
+    insts = batch.instances(specs)
+    cells = [i["nx"] * i["ny"] for i in insts]
+    log(f"batched: {count} instances, {sum(cells)} cells, generated in {time.perf_counter() - t0:.1f} s")
+    b = batch.run_batched(insts, t_end, read=False)
+    log(f"batched: {b['wall_s']:.2f} s (rows {b['rows']})")
+    q = batch.run_sequential(insts, t_end)
+    log(f"one at a time: {q['wall_s']:.2f} s (rows per size class {q['rows']})")
+    # parity sample: the smallest instance, batched result vs the FP64 oracle
+    k = int(np.argmin(cells))
+    inst = insts[k]
+    P = batch.Pool(0)
+    slots = batch.add_all(P, [inst])
+    P.advance(slots, [t_end])
+    g = P.read_state(slots[0])
+    P.destroy()
+    oracle.set_threads(1)
+    c = oracle.CPUSolver(nx=inst["nx"], ny=inst["ny"], dx=inst["dx"], z=inst["z"], mask=inst["mask"],
+                         manning=inst["manning"], bc=inst["bc"], hdry=1e-5)
+    c.setDepth(np.zeros(cells[k]))
+    c.rainRate = inst["rain"]
+    c.run(t_end)
+    dh = float(np.max(np.abs(g["h"] - c.field("h"))))
+    return {"instances": count, "cells": int(sum(cells)), "sim_s": t_end, "seed": seed,
+            "solves_per_hour": 3600.0 * count / b["wall_s"],
+            "sequential_solves_per_hour": 3600.0 * count / q["wall_s"],
+            "speedup_vs_one_at_a_time": q["wall_s"] / b["wall_s"],
+            "batched_wall_s": b["wall_s"], "sequential_wall_s": q["wall_s"],
+            "cell_updates_per_s": b["cell_updates"] / b["wall_s"], "rows_autotuned": b["rows"],
+            "tuning_ms_per_step": b["tune_ms"],
+            "parity_sample": {"instance": k, "side": inst["nx"], "max_dh_m": dh, "mass_rel": g["mass"]["rel"],
+                              "steps_gpu": b["steps"][k], "steps_oracle": c.steps}}
+
+
+def bench_served(rate=1.0, count=24, budget_gb=64, seed=11):
+    """Config 5: a Poisson stream of urban solves (sides 256-4096, 30-minute peaked storms of
+    25 mm x LogNormal(0, 0.25)) served with continuous batching under an HBM budget; one request
+    is re-solved alone to check that its result is the same in the mix."""
+    from floodsim_amd import scenarios as sc
+    from floodsim_amd import serve
+    arr = sc.poisson_arrivals(rate, count, seed=seed)
+    reqs = [serve.storm_request(i, a["t"], a["side"], a["seed"], a["depth_scale"]) for i, a in enumerate(arr)]
+    S = serve.Server(int(budget_gb) << 30, chunk_s=60.0)
+    try:
+        fin = S.run(reqs)
+        st = S.stats()
+    finally:
+        S.close()
+    probe = min(fin, key=lambda r: (r.cells, r.id))
+    alone = serve.solve_alone(probe)
+    same = bool(alone["steps"] == probe.result["steps"] and np.array_equal(alone["hmax"], probe.result["hmax"]))
+    st.update({"rate_per_s": rate, "budget_GiB": budget_gb, "seed": seed,
+               "sides": sorted({a["side"] for a in arr}), "storm": "30 min, peak at 40 %, 25 mm x scale",
+               "chunk_s": 60.0, "alone_equals_mixed": same, "max_mass_rel": max(abs(r.result["mass"]["rel"]) for r in fin)})
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +207,10 @@ def main():
     ap.add_argument("--stepping", choices=("lts", "global"), default=None)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (256 instances) measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-count", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by "
@@ -220,8 +294,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, s, lts=lts)
+    g.destroy()
+    batched = None if args.no_batch else bench_batched()
+    served = None if args.no_serve else bench_served(args.serve_rate, args.serve_count)
     out = {
-        "metric": "cell-updates/s + % HBM roofline",
+        "metric": "cell-updates/s + % HBM roofline; batched solves/hr, p99 latency @ \u22641 mm",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -245,6 +322,8 @@ def main():
                      "bytes_per_cell_update": B_STAGE2, "launches": dom_n,
                      "cells_per_launch": dom_cells / max(dom_n, 1), "avg_ms": dom_ms / max(dom_n, 1)},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
         "extra": {"stage1_ms": ms1, "stage2_ms": ms2, "timed_launches": nt,
                   "step_GBps_106B": value * B_STEP / 1e9,
                   "step_frac_106B": value * B_STEP / 1e9 / HBM_PEAK_GBS,

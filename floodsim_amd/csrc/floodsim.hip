@@ -141,12 +141,18 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // whole 8x8 level tiles
 int strip_own(const fs_params *p) { return p->lts ? 56 : kOwn; }
 int strip_rows(const fs_params *p) {
-  int r = p->rows > 0 ? p->rows : 64;
-  if (p->lts) r = std::max(8, (r + 7) / 8 * 8);
+  int r = p->rows > 0 ? std::max(8, p->rows) : 64;   // at least 8 rows (strip flag capacity)
+  if (p->lts) r = (r + 7) / 8 * 8;
   return r;
 }
 int units_of(const fs_params *p) {
   return ((p->nx + strip_own(p) - 1) / strip_own(p)) * ((p->ny + strip_rows(p) - 1) / strip_rows(p));
+}
+// strip flag capacity of a global-stepping instance: room for the shortest strips (8 rows), so that
+// the rows per strip can change after creation (fs_pool_set_rows)
+constexpr int kMinRows = 8;
+size_t strip_cap(const fs_params *p) {
+  return (size_t)((p->nx + strip_own(p) - 1) / strip_own(p)) * (size_t)((p->ny + kMinRows - 1) / kMinRows);
 }
 
 size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
@@ -158,7 +164,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
     b += 10 * align256(N * 4) + align256(N) + 2 * align256(N * 16);  // store, gathers, flag, bsx, bsy
     b += align256(nu) + 2 * align256(nu * 4 * (kMaxLevels + 1));     // umask, ucells, ulist
   }
-  if (!p->lts) b += 3 * align256((size_t)units_of(p)) + align256((size_t)units_of(p) * 4);  // strip flags
+  if (!p->lts) b += 3 * align256(strip_cap(p)) + align256(strip_cap(p) * 4);  // strip flags
   b += 8 * align256(N * 4);               // A and B
   b += 3 * align256(N * 4);               // z, n, hmax
   b += align256(N);                       // mask
@@ -601,6 +607,18 @@ int upload_strip_rain(fs_pool *P, Slot &S) {
   return FS_OK;
 }
 
+// per strip: open cells, rain flag, and every strip marked as stepping (its U* is then cleared by
+// the first stage that finds it quiet)
+int upload_strip_tables(fs_pool *P, Slot &S) {
+  std::vector<int32_t> so(S.nstrips, 0);
+  for (int j = 0; j < S.ny; ++j)
+    for (int i = 0; i < S.nx; ++i)
+      if (S.hmask[(size_t)j * S.nx + i] & MASK_OPEN) so[strip_of(S, i, j)] += 1;
+  HIPCHK(P, hipMemcpy(S.sopen, so.data(), (size_t)S.nstrips * 4, hipMemcpyHostToDevice));
+  HIPCHK(P, hipMemset(S.sact, 1, (size_t)S.nstrips));
+  return upload_strip_rain(P, S);
+}
+
 // per-cell host preparation shared by create/add
 int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *manning, double manningScalar,
              const uint8_t *mask, const double *infil, const double *rainField, const int32_t *inlets, int nInlets,
@@ -661,8 +679,9 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   } else {
     S.sxn = (nx + strip_own(p) - 1) / strip_own(p);
     S.nstrips = units_of(p);
-    S.swet = (uint8_t *)take((size_t)S.nstrips); S.sact = (uint8_t *)take((size_t)S.nstrips);
-    S.srain = (uint8_t *)take((size_t)S.nstrips); S.sopen = (int32_t *)take((size_t)S.nstrips * 4);
+    const size_t cap = strip_cap(p);
+    S.swet = (uint8_t *)take(cap); S.sact = (uint8_t *)take(cap);
+    S.srain = (uint8_t *)take(cap); S.sopen = (int32_t *)take(cap * 4);
   }
   S.used = true;
   P->used += need;
@@ -714,15 +733,9 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     HIPCHK(P, hipMemcpy(S.rf, S.hrf.data(), N * 4, hipMemcpyHostToDevice));
   } else rainArea = open;
   if (S.nstrips) {
-    std::vector<int32_t> so(S.nstrips, 0);
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i)
-        if (S.hmask[(size_t)j * nx + i] & MASK_OPEN) so[strip_of(S, i, j)] += 1;
-    HIPCHK(P, hipMemcpy(S.sopen, so.data(), (size_t)S.nstrips * 4, hipMemcpyHostToDevice));
-    HIPCHK(P, hipMemset(S.swet, 0, (size_t)S.nstrips));
-    HIPCHK(P, hipMemset(S.sact, 1, (size_t)S.nstrips));
-    rc = upload_strip_rain(P, S);
+    rc = upload_strip_tables(P, S);
     if (rc) return rc;
+    HIPCHK(P, hipMemset(S.swet, 0, (size_t)S.nstrips));
   }
   // zero state (A and B, contiguous) and max depth; the static layers sit between them
   HIPCHK(P, hipMemset(S.h, 0, (size_t)((char *)S.z - (char *)S.h)));
@@ -905,6 +918,26 @@ int fs_pool_add(fs_pool_t P, const fs_params *p, const double *z, const double *
                 int32_t nInlets, int32_t *slot) {
   if (!P || !slot) return fail(P, FS_E_ARG, "pool or slot is NULL");
   return add_slot(P, p, z, manning, manningScalar, mask, infil, rainField, inlets, nInlets, slot);
+}
+
+int fs_pool_set_rows(fs_pool_t P, int32_t slot, int32_t rows) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (S.L) return fail(P, FS_E_STATE, "rows of a local-time-stepping instance are fixed at creation");
+  if (rows < kMinRows || rows > 4096) return fail(P, FS_E_ARG, "rows must be in [%d, 4096] (got %d)", kMinRows, rows);
+  if (rows == S.rows) return FS_OK;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  S.p.rows = rows;
+  S.rows = strip_rows(&S.p);
+  S.nstrips = units_of(&S.p);
+  rc = upload_strip_tables(P, S);
+  if (rc) return rc;
+  // every strip steps once: the next second stage sets the quiet flags of the new strips
+  HIPCHK(P, hipMemset(S.swet, 1, (size_t)S.nstrips));
+  HIPCHK(P, hipDeviceSynchronize());
+  P->grp.slots.clear();  // the work list changes: rebuild the group (and its graphs)
+  return FS_OK;
 }
 
 int fs_pool_remove(fs_pool_t P, int32_t slot) {

@@ -54,7 +54,7 @@ EXPORTS = [
     "fs_pool_add", "fs_pool_remove", "fs_pool_set_state", "fs_pool_set_rain", "fs_pool_set_inlet_rates",
     "fs_pool_advance", "fs_pool_step", "fs_pool_read_state", "fs_pool_clock", "fs_pool_bytes_used",
     "fs_pool_kernel_ms", "fs_pool_set_timing", "fs_solver_pool", "fs_version", "fs_copy_probe",
-    "fs_pool_dominant_kernel", "fs_lts_levels",
+    "fs_pool_dominant_kernel", "fs_lts_levels", "fs_pool_set_rows",
 ]
 
 
@@ -99,6 +99,7 @@ def lib(path=None):
     L.fs_pool_bytes_used.argtypes = [P]
     L.fs_pool_add.argtypes = [P, pp, dp, dp, D, u8p, dp, dp, ip, I32, ip]
     L.fs_pool_remove.argtypes = [P, I32]
+    L.fs_pool_set_rows.argtypes = [P, I32, I32]
     L.fs_pool_set_state.argtypes = [P, I32, dp, dp, dp, I32]
     L.fs_pool_set_rain.argtypes = [P, I32, D]
     L.fs_pool_set_inlet_rates.argtypes = [P, I32, dp, I32]
@@ -388,6 +389,10 @@ class Pool:
         self._chk(rc)
         self.meta[slot.value] = (int(nx), int(ny), float(dx), lay)
         return slot.value
+
+    def set_rows(self, slot, rows):
+        """Rows per wavefront strip of one instance (the autotuned tile shape)."""
+        self._chk(lib().fs_pool_set_rows(self._p, int(slot), int(rows)))
 
     def remove(self, slot):
         self._chk(lib().fs_pool_remove(self._p, int(slot)))

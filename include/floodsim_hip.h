@@ -106,6 +106,9 @@ int fs_pool_add(fs_pool_t pool, const fs_params *p, const double *z, const doubl
                 const uint8_t *mask, const double *infil, const double *rainField, const int32_t *inlets,
                 int32_t nInlets, int32_t *slot);
 int fs_pool_remove(fs_pool_t pool, int32_t slot);
+/* rows per wavefront strip of a global-stepping instance (8..4096), the tile shape the autotuner
+ * picks (floodsim_amd/batch.py); results do not depend on it */
+int fs_pool_set_rows(fs_pool_t pool, int32_t slot, int32_t rows);
 int fs_pool_set_state(fs_pool_t pool, int32_t slot, const double *h, const double *hu, const double *hv,
                       int32_t bookInitial);
 int fs_pool_set_rain(fs_pool_t pool, int32_t slot, double rate_ms);

@@ -38,33 +38,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-class _Regions:
-    """roctx profiler regions: under `rocprofv3 --selected-regions` only the timed region (and the
-    copy probe) is traced; without a profiler the calls do nothing."""
-
-    def __init__(self):
-        import ctypes
-        self.lib = None
-        for name in ("librocprofiler-sdk-roctx.so", "/opt/rocm/lib/librocprofiler-sdk-roctx.so"):
-            try:
-                self.lib = ctypes.CDLL(name)
-                break
-            except OSError:
-                continue
-        if self.lib is not None:
-            for f in (self.lib.roctxProfilerPause, self.lib.roctxProfilerResume):
-                f.argtypes = [ctypes.c_uint64]
-                f.restype = ctypes.c_int
-
-    def pause(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
-
-    def resume(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
-
-
 def traffic_of(workload, stepping):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of this workload (calibrated against the copy kernel), or None."""
@@ -207,6 +180,8 @@ def main():
     ap.add_argument("--stepping", choices=("lts", "global"), default=None)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="no HIP events around the timed launches (for counter-collection runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (256 instances) measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--serve-rate", type=float, default=1.0)
@@ -221,8 +196,6 @@ def main():
     from floodsim_amd import scenarios as sc
     from floodsim_amd.solver import copy_probe
 
-    regions = _Regions()
-    regions.pause()
     t_setup = time.perf_counter()
     w = build_workload(args.workload)
     log(f"workload {args.workload}: generated in {time.perf_counter() - t_setup:.1f} s")
@@ -264,17 +237,20 @@ def main():
 
     c0 = g.clock()
     run_steps(args.warmup)
-    g.set_timing(True)
+    if not args.no_timing:
+        g.set_timing(True)
     c1 = g.clock()                                   # synchronises the stream
-    regions.resume()
     t0 = time.perf_counter()
     run_steps(args.steps)
     c2 = g.clock()                                   # synchronises again
     el = time.perf_counter() - t0
-    regions.pause()
-    ms1, ms2, nt = g.kernel_ms()
-    dom_ms, dom_cells, dom_n = g.dominant_kernel()
-    g.set_timing(False)
+    if args.no_timing:
+        ms1 = ms2 = dom_ms = dom_cells = -1.0
+        nt = dom_n = 0
+    else:
+        ms1, ms2, nt = g.kernel_ms()
+        dom_ms, dom_cells, dom_n = g.dominant_kernel()
+        g.set_timing(False)
     steps = c2.steps - c1.steps
     updates = c2.cellUpdates - c1.cellUpdates
     cells = w["nx"] * w["ny"]
@@ -283,13 +259,11 @@ def main():
     # dominant kernel: the second RK stage (+ sources); algorithmic bytes = 65 B per open cell it advances
     achieved = dom_cells * B_STAGE2 / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else None
     s = g.readState()
-    regions.resume()
     try:
         cp_gbs, _ = copy_probe(2 << 30, 10)
     except Exception as e:  # noqa: BLE001
         log("copy probe failed:", e)
         cp_gbs = None
-    regions.pause()
     traffic, tr = traffic_of(args.workload, stepping)
     cpu = None
     if not args.no_cpu:

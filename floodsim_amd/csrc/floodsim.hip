@@ -285,11 +285,13 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     G.capInst = std::max(ni, G.capInst * 2);
     HIPCHK(P, hipMalloc(&G.dinst, sizeof(DevInst) * G.capInst));
   }
-  if (nw > G.capWork) {
+  // partials: one per wavefront (an LTS record launches up to kLtsWaveCap per unit)
+  const int needWork = G.lts ? kLtsWaveCap * nw : nw;
+  if (needWork > G.capWork) {
     if (G.dwork) HIPCHK(P, hipFree(G.dwork));
     if (G.p1) HIPCHK(P, hipFree(G.p1));
     if (G.p2) HIPCHK(P, hipFree(G.p2));
-    G.capWork = std::max(nw, G.capWork * 2);
+    G.capWork = std::max(needWork, G.capWork * 2);
     HIPCHK(P, hipMalloc(&G.dwork, sizeof(int4) * G.capWork));
     HIPCHK(P, hipMalloc(&G.p1, sizeof(Part) * G.capWork));
     HIPCHK(P, hipMalloc(&G.p2, sizeof(Part) * G.capWork));
@@ -343,7 +345,7 @@ int launch_step(fs_pool *P) {
     const int nin = G.hinst[gi].nin;
     sw_exchange<false><<<(nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
   }
-  sw_dt_final<<<ni, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, 0);
+  sw_dt_final<<<ni, 1024, 0, P->stream>>>(G.dinst, G.p1, G.p2, 0);
   HIPCHK(P, hipGetLastError());
   return FS_OK;
 }
@@ -390,7 +392,8 @@ int launch_cycle(fs_pool *P) {
   Group &G = P->grp;
   const DevInst &d = G.hinst[0];
   const int nt = d.tx * d.ty, nu = d.wend - d.wbeg, L = d.L;
-  const int nbp = ((nu + 3) / 4 + 7) / 8 * 8;  // one wavefront per unit at most; idle blocks exit
+  // kLtsWaveCap wavefronts per unit at most (records of few units split them); idle blocks exit
+  const int nbp = ((kLtsWaveCap * nu + 3) / 4 + 7) / 8 * 8;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   if (P->timing) {
     e0 = timing_event(P); e1 = timing_event(P); e2 = timing_event(P);
@@ -441,7 +444,7 @@ int launch_cycle(fs_pool *P) {
       }
       sw_lts_fold<<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, m);
       if (d.nin) sw_exchange<true><<<(d.nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
-      sw_lts_record_end<<<1, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, m);
+      sw_lts_record_end<<<1, 1024, 0, P->stream>>>(G.dinst, G.p1, G.p2, m);
     }
   sw_lts_cycle_end<<<1, 1, 0, P->stream>>>(G.dinst);
   if (P->timing) HIPCHK(P, hipEventRecord(e2, P->stream));
@@ -499,7 +502,7 @@ int begin_advance(fs_pool *P, const std::vector<int> &slots, const std::vector<d
   for (int s : slots) dirty = dirty || P->slots[s].hrec.dirty;
   if (!G.lts) {
     if (dirty) sw_speed<<<(G.nwork + 3) / 4, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2);
-    sw_dt_final<<<ni, 256, 0, P->stream>>>(G.dinst, G.p1, G.p2, 1);
+    sw_dt_final<<<ni, 1024, 0, P->stream>>>(G.dinst, G.p1, G.p2, 1);
   }
   HIPCHK(P, hipGetLastError());
   for (int s : slots) P->slots[s].hrec.dirty = 0;
@@ -1221,6 +1224,89 @@ int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk) {
   if (clk) fill_clock(S, clk);
   return FS_OK;
 }
+
+// ---- device-state snapshot / restore (checkpoint.js:128-150 keeps the solver's buffers and its
+// host-side clock and books): the slot's whole device allocation, its step record and host books
+namespace {
+struct SnapHeader {
+  char magic[8];
+  uint32_t version, headerBytes;
+  uint64_t slotBytes;
+  int32_t nx, ny, lts, rows, secondOrder, limiter, bc[4];
+  double dx, g, cfl, hdry, hfront, dtDry;
+  StepRec rec;
+  double volInitial, exchangedRead, dtSum;
+  int64_t dtSteps;
+};
+constexpr char kSnapMagic[8] = {'F', 'S', 'S', 'N', 'A', 'P', '0', '1'};
+
+bool same_model(const SnapHeader &h, const Slot &S) {
+  const fs_params &p = S.p;
+  return h.nx == p.nx && h.ny == p.ny && h.lts == p.lts && h.secondOrder == p.secondOrder && h.limiter == p.limiter &&
+         h.bc[0] == p.bc[0] && h.bc[1] == p.bc[1] && h.bc[2] == p.bc[2] && h.bc[3] == p.bc[3] && h.dx == p.dx &&
+         h.g == p.g && h.cfl == p.cfl && h.hdry == p.hdry && h.hfront == p.hfront && h.dtDry == p.dtDry &&
+         h.slotBytes == S.bytes;
+}
+}  // namespace
+
+size_t fs_pool_snapshot_bytes(fs_pool_t P, int32_t slot) {
+  if (valid_slot(P, slot)) return 0;
+  return sizeof(SnapHeader) + P->slots[slot].bytes;
+}
+
+int fs_pool_snapshot(fs_pool_t P, int32_t slot, void *out, size_t bytes) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (!out || bytes < sizeof(SnapHeader) + S.bytes)
+    return fail(P, FS_E_ARG, "snapshot needs %zu bytes", sizeof(SnapHeader) + S.bytes);
+  rc = download_recs(P, {slot});
+  if (rc) return rc;
+  SnapHeader h{};
+  std::memcpy(h.magic, kSnapMagic, 8);
+  h.version = 1;
+  h.headerBytes = sizeof(SnapHeader);
+  h.slotBytes = S.bytes;
+  h.nx = S.p.nx; h.ny = S.p.ny; h.lts = S.p.lts; h.rows = S.p.rows; h.secondOrder = S.p.secondOrder;
+  h.limiter = S.p.limiter;
+  for (int k = 0; k < 4; ++k) h.bc[k] = S.p.bc[k];
+  h.dx = S.p.dx; h.g = S.p.g; h.cfl = S.p.cfl; h.hdry = S.p.hdry; h.hfront = S.p.hfront; h.dtDry = S.p.dtDry;
+  h.rec = S.hrec;
+  h.volInitial = S.volInitial; h.exchangedRead = S.exchangedRead; h.dtSum = S.dtSum; h.dtSteps = S.dtSteps;
+  std::memcpy(out, &h, sizeof h);
+  HIPCHK(P, hipMemcpy((char *)out + sizeof h, S.mem, S.bytes, hipMemcpyDeviceToHost));
+  return FS_OK;
+}
+
+int fs_pool_restore(fs_pool_t P, int32_t slot, const void *in, size_t bytes) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  SnapHeader h;
+  if (!in || bytes < sizeof h) return fail(P, FS_E_ARG, "not a snapshot (too short)");
+  std::memcpy(&h, in, sizeof h);
+  if (std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != 1 || h.headerBytes != sizeof h)
+    return fail(P, FS_E_ARG, "not a snapshot of this library version");
+  if (!same_model(h, S)) return fail(P, FS_E_ARG, "snapshot of a different model (grid, options or layout)");
+  if (bytes < sizeof h + S.bytes) return fail(P, FS_E_ARG, "snapshot truncated");
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  HIPCHK(P, hipMemcpy(S.mem, (const char *)in + sizeof h, S.bytes, hipMemcpyHostToDevice));
+  // the strip tables in the restored memory belong to the snapshot's strip height
+  S.p.rows = h.rows;
+  S.rows = strip_rows(&S.p);
+  if (S.nstrips) S.nstrips = units_of(&S.p);
+  S.hrec = h.rec;
+  S.volInitial = h.volInitial; S.exchangedRead = h.exchangedRead; S.dtSum = h.dtSum; S.dtSteps = h.dtSteps;
+  rc = upload_rec(P, slot);
+  if (rc) return rc;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  P->grp.slots.clear();
+  return FS_OK;
+}
+
+size_t fs_snapshot_bytes(fs_handle s) { return s ? fs_pool_snapshot_bytes(s->pool, s->slot) : 0; }
+int fs_snapshot(fs_handle s, void *out, size_t bytes) { return s ? fs_pool_snapshot(s->pool, s->slot, out, bytes) : FS_E_ARG; }
+int fs_restore(fs_handle s, const void *in, size_t bytes) { return s ? fs_pool_restore(s->pool, s->slot, in, bytes) : FS_E_ARG; }
 
 int fs_read_state(fs_handle s, double *h, double *hu, double *hv, double *hmax, fs_mass *mass) {
   return s ? fs_pool_read_state(s->pool, s->slot, h, hu, hv, hmax, mass) : FS_E_ARG;

@@ -366,7 +366,7 @@ static napi_value js_version(napi_env env, napi_callback_info info) {
 
 /* ---------- asynchronous calls (libuv thread pool) ---------- */
 
-enum { W_ADVANCE, W_FRAME, W_READ_STATE, W_READ_INLETS };
+enum { W_ADVANCE, W_FRAME, W_READ_STATE, W_READ_INLETS, W_SNAPSHOT, W_RESTORE };
 
 typedef struct Work {
   napi_async_work work;
@@ -381,7 +381,15 @@ typedef struct Work {
   fs_mass mass;
   float *out;            /* frame: 2*nx*ny */
   double *f[4];          /* readState: h, hu, hv, hmax; readInlets: vol, h */
+  void *blob;            /* snapshot (owned) / restore (a copy of the caller's bytes) */
+  size_t blobBytes;
 } Work;
+
+static void free_blob(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  free(data);
+}
 
 static void work_execute(napi_env env, void *data) {
   (void)env;
@@ -401,6 +409,12 @@ static void work_execute(napi_env env, void *data) {
       break;
     case W_READ_INLETS:
       w->rc = fs_read_inlets(h, w->f[0], w->f[1], w->s->nInlets);
+      break;
+    case W_SNAPSHOT:
+      w->rc = fs_snapshot(h, w->blob, w->blobBytes);
+      break;
+    case W_RESTORE:
+      w->rc = fs_restore(h, w->blob, w->blobBytes);
       break;
   }
   (void)n;
@@ -471,14 +485,31 @@ static void work_complete(napi_env env, napi_status status, void *data) {
         napi_set_named_property(env, result, "h", make_f64(env, w->f[1], (size_t)w->s->nInlets));
         break;
       }
+      case W_SNAPSHOT: {
+        /* the snapshot buffer becomes the ArrayBuffer's memory (freed by its finalizer) */
+        napi_value ab;
+        if (napi_create_external_arraybuffer(env, w->blob, w->blobBytes, free_blob, NULL, &ab) == napi_ok) {
+          napi_create_typedarray(env, napi_uint8_array, w->blobBytes, ab, 0, &result);
+          w->blob = NULL;
+        } else {
+          napi_get_null(env, &result);
+        }
+        break;
+      }
+      case W_RESTORE:
+        napi_get_undefined(env, &result);
+        break;
     }
     napi_resolve_deferred(env, w->deferred, result);
   }
   napi_delete_async_work(env, w->work);
   free(w->out);
   for (int k = 0; k < 4; ++k) free(w->f[k]);
+  free(w->blob);
   free(w);
 }
+
+static napi_value queue_work_blob(napi_env env, Solver *s, int kind, void *blob, size_t bytes);
 
 static napi_value queue_work(napi_env env, Solver *s, int kind, double t, int32_t K, int readOut) {
   Work *w = (Work *)calloc(1, sizeof(Work));
@@ -519,6 +550,68 @@ static napi_value queue_work(napi_env env, Solver *s, int kind, double t, int32_
     return NULL;
   }
   return promise;
+}
+
+static napi_value queue_work_blob(napi_env env, Solver *s, int kind, void *blob, size_t bytes) {
+  Work *w = (Work *)calloc(1, sizeof(Work));
+  if (!w) {
+    free(blob);
+    napi_throw_error(env, NULL, "floodsim: out of host memory");
+    return NULL;
+  }
+  w->s = s;
+  w->kind = kind;
+  w->blob = blob;
+  w->blobBytes = bytes;
+  napi_value promise, name;
+  napi_create_promise(env, &w->deferred, &promise);
+  napi_create_string_utf8(env, "floodsim", NAPI_AUTO_LENGTH, &name);
+  napi_create_async_work(env, NULL, name, work_execute, work_complete, w, &w->work);
+  s->busy = 1;
+  if (napi_queue_async_work(env, w->work) != napi_ok) {
+    s->busy = 0;
+    napi_delete_async_work(env, w->work);
+    free(w->blob);
+    free(w);
+    napi_throw_error(env, NULL, "floodsim: napi_queue_async_work failed");
+    return NULL;
+  }
+  return promise;
+}
+
+/* snapshot(h) -> Promise<Uint8Array> */
+static napi_value js_snapshot(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  const size_t n = fs_snapshot_bytes(s->h);
+  void *blob = malloc(n ? n : 1);
+  if (!blob) {
+    napi_throw_error(env, NULL, "floodsim: out of host memory for the snapshot");
+    return NULL;
+  }
+  return queue_work_blob(env, s, W_SNAPSHOT, blob, n);
+}
+
+/* restore(h, Uint8Array) -> Promise<undefined> (the bytes are copied first) */
+static napi_value js_restore(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  void *data;
+  size_t n = 0;
+  if (typed(env, a[1], napi_uint8_array, -1, &data, &n, "snapshot")) return NULL;
+  void *blob = malloc(n ? n : 1);
+  if (!blob) {
+    napi_throw_error(env, NULL, "floodsim: out of host memory for the snapshot");
+    return NULL;
+  }
+  if (n) memcpy(blob, data, n);
+  return queue_work_blob(env, s, W_RESTORE, blob, n);
 }
 
 /* advance(h, tTarget, K) -> Promise<clock> */
@@ -587,6 +680,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"frame", NULL, js_frame, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readState", NULL, js_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readInlets", NULL, js_read_inlets, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"snapshot", NULL, js_snapshot, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"restore", NULL, js_restore, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;
   return exports;

@@ -635,42 +635,112 @@ __device__ __forceinline__ bool strip_begin(const DevInst &I, const int4 wk, int
   return active;
 }
 
-// One RK stage over a work list. Global stepping: one wavefront per strip of `work`.
-// LTS record of level m (single-instance groups): one wavefront per entry of the level's list,
-// whose length only the device knows; the grid covers every strip and blocks without an entry exit.
+// ------------------------------------------------------------ LTS record work
+// A tile takes part in the record of level m when it steps at m, or waits at m + 1 beside a face
+// neighbour at m (the unit rule of sw_lts_units).
+__device__ __forceinline__ bool tile_relevant(const DevInst &I, int a, int b, int m) {
+  if (a < 0 || a >= I.tx || b < 0 || b >= I.ty) return false;
+  const uint8_t *lv = I.lv;
+  const int l = lv[b * I.tx + a];
+  if (l == m) return true;
+  if (l != m + 1) return false;
+  return (a > 0 && lv[b * I.tx + a - 1] == m) || (a + 1 < I.tx && lv[b * I.tx + a + 1] == m) ||
+         (b > 0 && lv[(b - 1) * I.tx + a] == m) || (b + 1 < I.ty && lv[(b + 1) * I.tx + a] == m);
+}
+
+// A record's wavefronts: every unit on its list is split into 2, 4 or 8 parts of whole tile rows,
+// one wavefront each (at most kLtsWaveCap per unit of the domain, the grid of a record launch):
+// the records of the fine levels, whose few units would otherwise each walk their 64 rows alone,
+// then fill the chip, and every part is cut to the tile rows that take part.
+constexpr int kLtsWaveCap = 2;
+__device__ __forceinline__ int lts_split(const DevInst &I, int m) {
+  // parts of 8, 16 or 32 rows: a list of up to 1/8 of the units, up to 1/4, or more
+  const int n = I.rec->lcount[m], nu = I.wend - I.wbeg;
+  return n * 8 <= nu ? 8 : n * 4 <= nu ? 4 : 2;
+}
+
+// The rows one wavefront of a level-m record walks: part `sub` of `split` parts of the unit, cut
+// to the span from its first to its last tile row holding a tile that takes part (w = 0: none)
+__device__ __forceinline__ int4 lts_rows(const DevInst &I, int4 wk, int m, int lane, int split, int sub) {
+  const int tx0 = wk.y / kLtsTile, ty0 = wk.z / kLtsTile;
+  const int ntx = min(I.own / kLtsTile, I.tx - tx0);
+  const int nty = min((wk.w + kLtsTile - 1) / kLtsTile, I.ty - ty0);   // tile rows of the unit
+  const int per = ((wk.w + kLtsTile - 1) / kLtsTile + split - 1) / split;  // tile rows per part
+  const int r0 = sub * per, r1 = min(nty, r0 + per);
+  const int c = lane & 7, r = r0 + (lane >> 3);
+  const bool rel = c < ntx && r < r1 && tile_relevant(I, tx0 + c, ty0 + r, m);
+  const uint64_t bits = __ballot(rel);
+  int first = -1, last = -1;
+  for (int q = 0; q < 8; ++q)
+    if ((bits >> (8 * q)) & 0xffull) { if (first < 0) first = q; last = q; }
+  if (first < 0) { wk.w = 0; return wk; }
+  wk.z += kLtsTile * (r0 + first);
+  wk.w = kLtsTile * (last - first + 1);
+  return wk;
+}
+
+// LTS record of level m (single-instance groups): the parts of the level's units (lts_split), whose
+// number only the device knows; the grid covers kLtsWaveCap per unit and waves without a part exit.
+template <int STAGE>
+__device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
+                                               Part *__restrict__ part, int m) {
+  const int lane = threadIdx.x & 63;
+  const DevInst &I = insts[0];
+  if (STAGE == 1 && !I.secondOrder) return;
+  const float dt = I.rec->ldt[m];
+  const int split = lts_split(I, m);
+  const int i = xcd_wave_index_n(I.rec->lcount[m] * split, 4);
+  if (i < 0) return;
+  int4 wk = {0, 0, 0, 0};
+  if (dt > 0.f) {
+    const int u = I.ulist[m * (I.wend - I.wbeg) + i / split];
+    wk = lts_rows(I, work[I.wbeg + u], m, lane, split, i % split);
+  }
+  if (wk.w == 0) {
+    if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0};
+    return;
+  }
+  run_strip<STAGE, true>(I, wk, lane, m, dt, part + i, 0);
+}
+
+// One RK stage over a work list. Global stepping: one wavefront per strip of `work`; a strip the
+// dry-strip rule leaves out writes an empty partial.
 template <int STAGE, bool LTS>
 __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                 int nwork, Part *__restrict__ part, int m) {
+  static_assert(!LTS, "local time stepping records use the specialisations below");
   const int wid = xcd_wave_index(4);
   const int lane = threadIdx.x & 63;
-  if constexpr (!LTS) {
-    if (wid >= nwork) return;
-    const int4 wk = work[wid];
-    const DevInst &I = insts[wk.x];
-    if (STAGE == 1 && !I.secondOrder) return;  // first order: sw_euler_finish writes this strip's partial
-    const float dt = I.rec->dt;
-    const int u = wid - I.wbeg;
-    bool active = dt > 0.f;
-    if (active && I.wet) active = STAGE == 0 ? strip_begin(I, wk, u, lane) : I.act[u] != 0;
-    if (!active) {
-      if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
-      return;
-    }
-    const bool stir = run_strip<STAGE, false>(I, wk, lane, 0, dt, part + wid, I.sopen[u]);
-    if (STAGE == 1 && I.wet && lane == 0) I.wet[u] = stir ? 1 : 0;
-  } else {
-    const DevInst &I = insts[0];
-    if (STAGE == 1 && !I.secondOrder) return;
-    const float dt = I.rec->ldt[m];
-    const int i = xcd_wave_index_n(I.rec->lcount[m], 4);
-    if (i < 0) return;
-    if (!(dt > 0.f)) {
-      if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0};
-      return;
-    }
-    const int u = I.ulist[m * (I.wend - I.wbeg) + i];
-    run_strip<STAGE, true>(I, work[I.wbeg + u], lane, m, dt, part + i, 0);
+  if (wid >= nwork) return;
+  const int4 wk = work[wid];
+  const DevInst &I = insts[wk.x];
+  if (STAGE == 1 && !I.secondOrder) return;  // first order: sw_euler_finish writes this strip's partial
+  const float dt = I.rec->dt;
+  const int u = wid - I.wbeg;
+  bool active = dt > 0.f;
+  if (active && I.wet) active = STAGE == 0 ? strip_begin(I, wk, u, lane) : I.act[u] != 0;
+  if (!active) {
+    if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
+    return;
   }
+  const bool stir = run_strip<STAGE, false>(I, wk, lane, 0, dt, part + wid, I.sopen[u]);
+  if (STAGE == 1 && I.wet && lane == 0) I.wet[u] = stir ? 1 : 0;
+}
+
+#ifndef FS_LTS_MIN_BLOCKS
+#define FS_LTS_MIN_BLOCKS 1
+#endif
+template <>
+__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<0, true>(
+    const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
+  (void)nwork;
+  lts_stage_body<0>(insts, work, part, m);
+}
+template <>
+__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<1, true>(
+    const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
+  (void)nwork;
+  lts_stage_body<1>(insts, work, part, m);
 }
 
 // ------------------------------------------------------------ local time stepping, per record
@@ -680,11 +750,12 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
                                                    int m) {
   const DevInst &I = insts[0];
   const int lane = threadIdx.x & 63;
-  const int i = xcd_wave_index_n(I.rec->lcount[m], 4);
+  const int split = lts_split(I, m);
+  const int i = xcd_wave_index_n(I.rec->lcount[m] * split, 4);
   const int nu = I.wend - I.wbeg;
   const bool two = I.secondOrder;
   if (i >= 0) {
-    const int4 wk = work[I.wbeg + I.ulist[m * nu + i]];
+    const int4 wk = lts_rows(I, work[I.wbeg + I.ulist[m * nu + i / split]], m, lane, split, i % split);
     const int X = wk.y - I.hl + lane;
     if (lane < I.hl || lane >= I.hl + I.own || X >= I.nx) return;
     for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
@@ -869,14 +940,14 @@ __global__ __launch_bounds__(1024) void sw_lts_lists(const DevInst *__restrict__
 }
 
 // end of a level-m record: books in a fixed order (as sw_dt_final)
-__global__ __launch_bounds__(256) void sw_lts_record_end(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
+__global__ __launch_bounds__(1024) void sw_lts_record_end(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
                                                         const Part *__restrict__ p2, int m) {
   const DevInst &I = insts[0];
   StepRec *rec = I.rec;
   const float dt = rec->ldt[m];
   if (!(dt > 0.f)) return;
-  const int n = rec->lcount[m];
-  __shared__ double s_o[4], s_c[4], s_f[4];
+  const int n = rec->lcount[m] * lts_split(I, m);   // one partial per wavefront of the record
+  __shared__ double s_o[16], s_c[16], s_f[16];
   double o = 0.0, c = 0.0, f = 0.0;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     const Part a = p1[k];
@@ -987,9 +1058,12 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
     const DevInst &I = insts[0];
     if (I.secondOrder) return;
     const float dt = I.rec->ldt[m];
-    const int i = xcd_wave_index_n(I.rec->lcount[m], 4);
+    const int split = lts_split(I, m);
+    const int i = xcd_wave_index_n(I.rec->lcount[m] * split, 4);
     if (i < 0) return;
-    euler_finish_strip<true>(I, work[I.wbeg + I.ulist[m * (I.wend - I.wbeg) + i]], lane, m, dt, part + i, 0);
+    const int u = I.ulist[m * (I.wend - I.wbeg) + i / split];
+    const int4 wk = lts_rows(I, work[I.wbeg + u], m, lane, split, i % split);
+    euler_finish_strip<true>(I, wk, lane, m, wk.w > 0 ? dt : 0.f, part + i, 0);
   }
 }
 
@@ -1032,14 +1106,14 @@ __global__ void sw_exchange(const DevInst *__restrict__ insts, int inst, int m) 
 // max wave speed: dt = cfl dx / smax (dtDry when dry), clamped to land exactly on tEnd.
 // mode 0: end of a step; 1: start of an advance: instances the host marked dirty take their max
 // speed from the sw_speed partials (p2), the others re-clamp their last one to the new tEnd.
-__global__ __launch_bounds__(256) void sw_dt_final(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
+__global__ __launch_bounds__(1024) void sw_dt_final(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
                                                    const Part *__restrict__ p2, int mode) {
   const DevInst &I = insts[blockIdx.x];
   StepRec *rec = I.rec;
   const int t = threadIdx.x;
-  __shared__ double s_o[4], s_c[4], s_f[4];
-  __shared__ float s_m[4];
-  __shared__ int s_n[4];
+  __shared__ double s_o[16], s_c[16], s_f[16];
+  __shared__ float s_m[16];
+  __shared__ int s_n[16];
   double o = 0.0, c = 0.0, f = 0.0;
   float m = 0.f;
   int cells = 0;

@@ -242,6 +242,24 @@ class GPUSolver {
     });
   }
 
+  /** Everything the solver holds on the device, with its clock and books (checkpoint.js:128-150). */
+  snapshot() {
+    this._raiseDeferred();
+    const meta = { t: this.t, steps: this.steps, lastDt: this.lastDt, dtAvg: this.dtAvg, vol: Object.assign({}, this.vol),
+      rainRate: this._rain };
+    return this._enqueue(() => addon.snapshot(this._h)).then((bytes) => ({ bytes, meta }));
+  }
+
+  /** Put back a snapshot of this solver, or of one built from the same model. */
+  restore(snap) {
+    this._raiseDeferred();
+    return this._enqueue(() => addon.restore(this._h, snap.bytes)).then(() => {
+      const m = snap.meta;
+      this.t = m.t; this.steps = m.steps; this.lastDt = m.lastDt; this.dtAvg = m.dtAvg;
+      this.vol = Object.assign({}, m.vol); this._rain = m.rainRate;
+    });
+  }
+
   /** LTS levels of the last cycle per 8x8 tile: {levels: Uint8Array, tx, ty}. */
   levels() { return this._enqueue(() => addon.levels(this._h)); }
 

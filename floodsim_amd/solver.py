@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfloodsim_hip.so")
+LIB_PATH = os.environ.get("FS_LIB_PATH") or os.path.join(_HERE, "libfloodsim_hip.so")
 _LIB = None
 
 BC = {"wall": 0, "open": 1, "outflow": 2}
@@ -54,7 +54,8 @@ EXPORTS = [
     "fs_pool_add", "fs_pool_remove", "fs_pool_set_state", "fs_pool_set_rain", "fs_pool_set_inlet_rates",
     "fs_pool_advance", "fs_pool_step", "fs_pool_read_state", "fs_pool_clock", "fs_pool_bytes_used",
     "fs_pool_kernel_ms", "fs_pool_set_timing", "fs_solver_pool", "fs_version", "fs_copy_probe",
-    "fs_pool_dominant_kernel", "fs_lts_levels", "fs_pool_set_rows",
+    "fs_pool_dominant_kernel", "fs_lts_levels", "fs_pool_set_rows", "fs_snapshot_bytes", "fs_snapshot",
+    "fs_restore", "fs_pool_snapshot_bytes", "fs_pool_snapshot", "fs_pool_restore",
 ]
 
 
@@ -112,6 +113,14 @@ def lib(path=None):
     L.fs_copy_probe.argtypes = [SZ, I32, dp, dp]
     L.fs_lts_levels.argtypes = [P, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.fs_pool_dominant_kernel.argtypes = [P, dp, dp, ctypes.POINTER(ctypes.c_int64)]
+    for f in (L.fs_snapshot_bytes, L.fs_pool_snapshot_bytes):
+        f.restype = SZ
+    L.fs_snapshot_bytes.argtypes = [P]
+    L.fs_pool_snapshot_bytes.argtypes = [P, I32]
+    L.fs_snapshot.argtypes = [P, P, SZ]
+    L.fs_restore.argtypes = [P, P, SZ]
+    L.fs_pool_snapshot.argtypes = [P, I32, P, SZ]
+    L.fs_pool_restore.argtypes = [P, I32, P, SZ]
     L.fs_solver_pool.restype = P
     L.fs_solver_pool.argtypes = [P]
     _LIB = L
@@ -302,6 +311,19 @@ class GPUSolver:
         self._chk(lib().fs_read_state(self._h, _dp(h), _dp(hu), _dp(hv), _dp(hm), ctypes.byref(m)))
         mass = {k: getattr(m, k) for k, _ in fs_mass._fields_}
         return {"h": h, "hu": hu, "hv": hv, "hmax": hm, "mass": mass}
+
+    # --- device-state snapshot / restore (checkpoint.js:128-150)
+    def snapshot(self):
+        """Everything the solver holds on the device plus its clock and books, as a uint8 array."""
+        n = lib().fs_snapshot_bytes(self._h)
+        buf = np.empty(n, dtype=np.uint8)
+        self._chk(lib().fs_snapshot(self._h, buf.ctypes.data_as(ctypes.c_void_p), n))
+        return buf
+
+    def restore(self, buf):
+        """Put a snapshot of this solver (or of one built from the same model) back."""
+        b = np.ascontiguousarray(np.frombuffer(buf, dtype=np.uint8) if not isinstance(buf, np.ndarray) else buf)
+        self._chk(lib().fs_restore(self._h, b.ctypes.data_as(ctypes.c_void_p), b.nbytes))
 
     def levels(self):
         """LTS tile levels of the last cycle, (ty, tx) uint8."""

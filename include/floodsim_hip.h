@@ -139,6 +139,17 @@ fs_pool_t fs_solver_pool(fs_handle s);
  * achievable bandwidth the roofline is quoted against, and calibrates byte counters) */
 int fs_copy_probe(size_t bytes, int32_t iters, double *gbps, double *ms);
 
+/* Device-state snapshot / restore: a host copy of everything the solver holds on the device (state,
+ * layers, books, LTS stores) plus its clock and FP64 books, restorable into the same solver or one
+ * built from the same model (checkpoint.js:128-150 saves the solver's buffers and its JS state).
+ * The buffer is opaque, fs_snapshot_bytes long. */
+size_t fs_snapshot_bytes(fs_handle s);
+int fs_snapshot(fs_handle s, void *out, size_t bytes);
+int fs_restore(fs_handle s, const void *in, size_t bytes);
+size_t fs_pool_snapshot_bytes(fs_pool_t pool, int32_t slot);
+int fs_pool_snapshot(fs_pool_t pool, int32_t slot, void *out, size_t bytes);
+int fs_pool_restore(fs_pool_t pool, int32_t slot, const void *in, size_t bytes);
+
 /* library version string */
 const char *fs_version(void);
 

@@ -27,10 +27,23 @@ async function main() {
   if (s.nInlets) { const r = await s.readInlets(); for (const v of r.vol) inletVol += v; }
   const st = await s.readState();
   const tiles = await s.tileStats();
+  const t1 = s.t, steps1 = s.steps, lastDt1 = s.lastDt, volRain1 = s.vol.rain;
+  // snapshot, run on, restore, run the same span again: the same bits
+  const snap = await s.snapshot();
+  const tEnd = s.t + 3;
+  await s.advance(tEnd);
+  const a = await s.readState();
+  await s.restore(snap);
+  const tBack = s.t;
+  await s.advance(tEnd);
+  const b = await s.readState();
+  let same = a.h.length === b.h.length;
+  for (let k = 0; k < a.h.length && same; k++) same = a.h[k] === b.h[k] && a.hu[k] === b.hu[k] && a.hv[k] === b.hv[k];
   s.destroy();
   process.stdout.write(JSON.stringify({
-    t: s.t, steps: s.steps, lastDt: s.lastDt, h: Array.from(st.h), hu: Array.from(st.hu), hv: Array.from(st.hv),
-    hmax: Array.from(st.hmax), mass: st.mass, frameSum, inletVol, volRain: s.vol.rain, tiles,
+    t: t1, steps: steps1, lastDt: lastDt1, h: Array.from(st.h), hu: Array.from(st.hu), hv: Array.from(st.hv),
+    hmax: Array.from(st.hmax), mass: st.mass, frameSum, inletVol, volRain: volRain1, tiles,
+    snapshotBytes: snap.bytes.length, restoredT: tBack, restoreSame: same,
   }) + '\n');
 }
 

@@ -19,8 +19,8 @@ NODE = shutil.which("node")
 
 pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
-ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readInlets", "readState", "setInletRates",
-               "setRain", "setRainField", "setState", "tileStats", "version"]
+ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readInlets", "readState", "restore",
+               "setInletRates", "setRain", "setRainField", "setState", "snapshot", "tileStats", "version"]
 
 
 def node(script, *args, timeout=120):
@@ -92,6 +92,7 @@ def test_js_matches_python_binding_and_oracle(tmp_path):
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     js = json.loads(r.stdout)
+    assert js["restoreSame"] and js["snapshotBytes"] > 0
     py, steps = _python_run(c)
     # the same C ABI under both bindings: identical bits
     assert js["steps"] == steps

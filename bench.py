@@ -180,6 +180,7 @@ def main():
     ap.add_argument("--stepping", choices=("lts", "global"), default=None)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--hazard", action="store_true", help="track the hazard products every step (as the reference does)")
     ap.add_argument("--no-timing", action="store_true",
                     help="no HIP events around the timed launches (for counter-collection runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (256 instances) measurement")
@@ -206,7 +207,7 @@ def main():
     if args.warmup is None:
         args.warmup = 4 if lts else 20
     g = fsa.GPUSolver(nx=w["nx"], ny=w["ny"], dx=w["dx"], z=w["z"], manning=w["manning"], mask=w["mask"],
-                      bc=w["bc"], inlets=w["inlets"], cfl=w.get("cfl"), rows=args.rows, lts=lts)
+                      bc=w["bc"], inlets=w["inlets"], cfl=w.get("cfl"), rows=args.rows, lts=lts, hazard=args.hazard)
     if "depth" in w:
         g.setDepth(w["depth"])
     else:
@@ -288,7 +289,8 @@ def main():
                    "open_cells": int(cells if w["mask"] is None else int(np.asarray(w["mask"]).sum())),
                    "stepping": ("LTS, %d levels (a step is one cycle)" % lts) if lts else "global SSP-RK2",
                    "spinup_s": args.spinup, "t_start": c1.t, "t_end": c2.t,
-                   "layers_checksum": w.get("checksum"), "rows_per_strip": args.rows or 64},
+                   "layers_checksum": w.get("checksum"), "rows_per_strip": args.rows or 64,
+                   "hazard_products": bool(args.hazard)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "traffic_source": tr and tr.get("source"),

@@ -49,6 +49,7 @@ struct Slot {
   int sxn = 0, nstrips = 0;
   uint8_t *swet = nullptr, *sact = nullptr, *srain = nullptr;
   int32_t *sopen = nullptr;
+  float4 *hzA = nullptr, *hzB = nullptr;   // hazard products (fs_params.hazard)
   // local time stepping
   int L = 0, tx = 0, ty = 0, nunits = 0;
   uint8_t *lv = nullptr, *lv0 = nullptr, *gflag = nullptr, *umask = nullptr;
@@ -73,6 +74,7 @@ struct Group {
   std::vector<DevInst> hinst;
   std::vector<int> withInlets;           // group indices of instances with inlets
   bool anyFirstOrder = false, anySecondOrder = false;
+  bool anyHazard = false;                // an instance tracks the hazard products
   bool lts = false;                      // a local-time-stepping instance (alone in its group)
   // captured K-step graph
   // captured K-step graphs, one per power-of-two K (index log2 K)
@@ -165,6 +167,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
     b += align256(nu) + 2 * align256(nu * 4 * (kMaxLevels + 1));     // umask, ucells, ulist
   }
   if (!p->lts) b += 3 * align256(strip_cap(p)) + align256(strip_cap(p) * 4);  // strip flags
+  if (p->hazard) b += 2 * align256(N * 16);  // hazard products
   b += 8 * align256(N * 4);               // A and B
   b += 3 * align256(N * 4);               // z, n, hmax
   b += align256(N);                       // mask
@@ -237,6 +240,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
   G.hinst.clear();
   G.withInlets.clear();
   G.anyFirstOrder = G.anySecondOrder = false;
+  G.anyHazard = false;
   G.lts = false;
   std::vector<int4> work;
   for (size_t gi = 0; gi < slots.size(); ++gi) {
@@ -266,6 +270,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.sxn = S.sxn;
     d.wet = S.nstrips && !S.p.untiled ? S.swet : nullptr;
     d.act = S.sact; d.srain = S.srain; d.sopen = S.sopen;
+    d.hzA = S.hzA; d.hzB = S.hzB;
     d.L = S.L; d.tx = S.tx; d.ty = S.ty;
     d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.tileOpen = S.tileOpen; d.tileRf = S.rf ? S.tileRf : nullptr;
     d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
@@ -273,6 +278,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
     d.umask = S.umask; d.ucells = S.ucells; d.ulist = S.ulist;
     if (S.L) G.lts = true;
+    if (S.hzA) G.anyHazard = true;
     if (S.nin) G.withInlets.push_back((int)gi);
     if (!S.p.secondOrder) G.anyFirstOrder = true;
     else G.anySecondOrder = true;
@@ -337,9 +343,15 @@ int launch_step(fs_pool *P) {
   sw_stage<0, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
   if (P->timing) HIPCHK(P, hipEventRecord(e1, P->stream));
   // each instance runs its own scheme: the kernel of the other order returns at once for it
-  if (G.anySecondOrder) sw_stage<1, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+  if (G.anySecondOrder) {
+    if (G.anyHazard) sw_stage<1, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    else sw_stage<1, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+  }
   if (P->timing) P->spans.push_back({e1, nullptr, -1, -1});  // completed below with e2
-  if (G.anyFirstOrder) sw_euler_finish<false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+  if (G.anyFirstOrder) {
+    if (G.anyHazard) sw_euler_finish<false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    else sw_euler_finish<false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+  }
   if (P->timing) { HIPCHK(P, hipEventRecord(e2, P->stream)); P->spans.back().b = e2; }
   for (int gi : G.withInlets) {
     const int nin = G.hinst[gi].nin;
@@ -436,8 +448,15 @@ int launch_cycle(fs_pool *P) {
         if (!sa || !sb) return fail(P, FS_E_HIP, "hipEventCreate failed");
         HIPCHK(P, hipEventRecord(sa, P->stream));
       }
-      if (d.secondOrder) sw_stage<1, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, m);
-      else sw_euler_finish<true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, m);
+      // the record's base step rides in the level argument's upper bits (its end time: hazard clock)
+      const int mk = m | (k << 8);
+      if (d.secondOrder) {
+        if (d.hzA) sw_stage<1, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        else sw_stage<1, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+      } else {
+        if (d.hzA) sw_euler_finish<true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        else sw_euler_finish<true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+      }
       if (P->timing) {
         HIPCHK(P, hipEventRecord(sb, P->stream));
         P->spans.push_back({sa, sb, cyc, m});
@@ -610,6 +629,14 @@ int upload_strip_rain(fs_pool *P, Slot &S) {
   return FS_OK;
 }
 
+// hazard products before the first step: nothing recorded, no arrival, no first wetting
+__global__ __launch_bounds__(256) void sw_hz_init(float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    a[k] = make_float4(0.f, 0.f, 0.f, -1.f);
+    b[k] = make_float4(0.f, -1.f, 0.f, 0.f);
+  }
+}
+
 // per strip: open cells, rain flag, and every strip marked as stepping (its U* is then cleared by
 // the first stage that finds it quiet)
 int upload_strip_tables(fs_pool *P, Slot &S) {
@@ -659,6 +686,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   S.mask = (uint8_t *)take(N);
   if (infil) S.infil = (float *)take(N * 4);
   if (rainField) S.rf = (float *)take(N * 4);
+  if (p->hazard) { S.hzA = (float4 *)take(N * 16); S.hzB = (float4 *)take(N * 16); }
   S.nin = nInlets;
   S.inl = (Inlet *)take((size_t)std::max(nInlets, 1) * sizeof(Inlet));
   S.ispeed = (float *)take((size_t)std::max(nInlets, 1) * 4);
@@ -739,6 +767,12 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     rc = upload_strip_tables(P, S);
     if (rc) return rc;
     HIPCHK(P, hipMemset(S.swet, 0, (size_t)S.nstrips));
+  }
+  if (S.hzA) {
+    const int blocks = (int)std::min<size_t>((N + 255) / 256, 65536);
+    sw_hz_init<<<blocks, 256, 0, P->stream>>>(S.hzA, S.hzB, N);
+    HIPCHK(P, hipGetLastError());
+    HIPCHK(P, hipStreamSynchronize(P->stream));
   }
   // zero state (A and B, contiguous) and max depth; the static layers sit between them
   HIPCHK(P, hipMemset(S.h, 0, (size_t)((char *)S.z - (char *)S.h)));
@@ -1301,6 +1335,33 @@ int fs_pool_restore(fs_pool_t P, int32_t slot, const void *in, size_t bytes) {
   if (rc) return rc;
   HIPCHK(P, hipStreamSynchronize(P->stream));
   P->grp.slots.clear();
+  return FS_OK;
+}
+
+int fs_read_hazard(fs_handle s, float *dv, float *force, float *duration, float *arrival, float *vmax, float *tWet,
+                   uint8_t *arr, uint8_t *defra, uint8_t *defraUrban) {
+  if (!s) return FS_E_ARG;
+  fs_pool *P = s->pool;
+  Slot &S = P->slots[s->slot];
+  if (!S.hzA) return fail(P, FS_E_STATE, "hazard products are tracked only for a solver created with hazard = 1");
+  const size_t N = (size_t)S.nx * S.ny;
+  std::vector<float4> a(N), b(N);
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  HIPCHK(P, hipMemcpy(a.data(), S.hzA, N * 16, hipMemcpyDeviceToHost));
+  HIPCHK(P, hipMemcpy(b.data(), S.hzB, N * 16, hipMemcpyDeviceToHost));
+  const float nan = std::nanf("");
+  for (size_t k = 0; k < N; ++k) {
+    if (dv) dv[k] = a[k].x;
+    if (force) force[k] = a[k].y;
+    if (duration) duration[k] = a[k].z;
+    if (arrival) arrival[k] = a[k].w >= 0.f ? a[k].w : nan;
+    if (vmax) vmax[k] = b[k].x;
+    if (tWet) tWet[k] = b[k].y >= 0.f ? b[k].y : nan;
+    const int c = (int)b[k].z;
+    if (arr) arr[k] = (uint8_t)(c & 7);
+    if (defra) defra[k] = (uint8_t)((c >> 3) & 7);
+    if (defraUrban) defraUrban[k] = (uint8_t)(c >> 6);
+  }
   return FS_OK;
 }
 

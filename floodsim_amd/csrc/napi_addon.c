@@ -194,6 +194,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   p.lts = (int32_t)num_prop(env, a[0], "lts", 0);
   p.rows = (int32_t)num_prop(env, a[0], "rows", 0);
   p.untiled = (int32_t)num_prop(env, a[0], "untiled", 0);
+  p.hazard = (int32_t)num_prop(env, a[0], "hazard", 0);
   if (p.nx <= 0 || p.ny <= 0) {
     napi_throw_range_error(env, NULL, "create: nx and ny must be positive");
     return NULL;
@@ -366,7 +367,7 @@ static napi_value js_version(napi_env env, napi_callback_info info) {
 
 /* ---------- asynchronous calls (libuv thread pool) ---------- */
 
-enum { W_ADVANCE, W_FRAME, W_READ_STATE, W_READ_INLETS, W_SNAPSHOT, W_RESTORE };
+enum { W_ADVANCE, W_FRAME, W_READ_STATE, W_READ_INLETS, W_SNAPSHOT, W_RESTORE, W_HAZARD };
 
 typedef struct Work {
   napi_async_work work;
@@ -381,6 +382,8 @@ typedef struct Work {
   fs_mass mass;
   float *out;            /* frame: 2*nx*ny */
   double *f[4];          /* readState: h, hu, hv, hmax; readInlets: vol, h */
+  float *hz[6];          /* readHazard: dv, force, duration, arrival, vmax, tWet */
+  uint8_t *cls[3];       /* readHazard: arr, defra, defraUrban */
   void *blob;            /* snapshot (owned) / restore (a copy of the caller's bytes) */
   size_t blobBytes;
 } Work;
@@ -412,6 +415,10 @@ static void work_execute(napi_env env, void *data) {
       break;
     case W_SNAPSHOT:
       w->rc = fs_snapshot(h, w->blob, w->blobBytes);
+      break;
+    case W_HAZARD:
+      w->rc = fs_read_hazard(h, w->hz[0], w->hz[1], w->hz[2], w->hz[3], w->hz[4], w->hz[5], w->cls[0], w->cls[1],
+                             w->cls[2]);
       break;
     case W_RESTORE:
       w->rc = fs_restore(h, w->blob, w->blobBytes);
@@ -499,12 +506,32 @@ static void work_complete(napi_env env, napi_status status, void *data) {
       case W_RESTORE:
         napi_get_undefined(env, &result);
         break;
+      case W_HAZARD: {
+        static const char *fn[6] = {"dv", "force", "duration", "arrival", "vmax", "tWet"};
+        static const char *cn[3] = {"arr", "defra", "defraUrban"};
+        napi_create_object(env, &result);
+        for (int k = 0; k < 6; ++k) napi_set_named_property(env, result, fn[k], make_f32(env, w->hz[k], n));
+        napi_value c;
+        napi_create_object(env, &c);
+        for (int k = 0; k < 3; ++k) {
+          void *d;
+          napi_value ab, ta;
+          napi_create_arraybuffer(env, n, &d, &ab);
+          if (n) memcpy(d, w->cls[k], n);
+          napi_create_typedarray(env, napi_uint8_array, n, ab, 0, &ta);
+          napi_set_named_property(env, c, cn[k], ta);
+        }
+        napi_set_named_property(env, result, "cls", c);
+        break;
+      }
     }
     napi_resolve_deferred(env, w->deferred, result);
   }
   napi_delete_async_work(env, w->work);
   free(w->out);
   for (int k = 0; k < 4; ++k) free(w->f[k]);
+  for (int k = 0; k < 6; ++k) free(w->hz[k]);
+  for (int k = 0; k < 3; ++k) free(w->cls[k]);
   free(w->blob);
   free(w);
 }
@@ -528,9 +555,15 @@ static napi_value queue_work(napi_env env, Solver *s, int kind, double t, int32_
     for (int k = 0; k < 4; ++k) oom |= !(w->f[k] = (double *)malloc(n * sizeof(double)));
   if (kind == W_READ_INLETS)
     for (int k = 0; k < 2; ++k) oom |= !(w->f[k] = (double *)calloc((size_t)s->nInlets + 1, sizeof(double)));
+  if (kind == W_HAZARD) {
+    for (int k = 0; k < 6; ++k) oom |= !(w->hz[k] = (float *)malloc(n * sizeof(float) + 4));
+    for (int k = 0; k < 3; ++k) oom |= !(w->cls[k] = (uint8_t *)malloc(n + 1));
+  }
   if (oom) {
     free(w->out);
     for (int k = 0; k < 4; ++k) free(w->f[k]);
+    for (int k = 0; k < 6; ++k) free(w->hz[k]);
+    for (int k = 0; k < 3; ++k) free(w->cls[k]);
     free(w);
     napi_throw_error(env, NULL, "floodsim: out of host memory");
     return NULL;
@@ -653,6 +686,15 @@ static napi_value js_read_state(napi_env env, napi_callback_info info) {
   return queue_work(env, s, W_READ_STATE, 0, 0, 0);
 }
 
+static napi_value js_read_hazard(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  return queue_work(env, s, W_HAZARD, 0, 0, 0);
+}
+
 static napi_value js_read_inlets(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
@@ -681,6 +723,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readState", NULL, js_read_state, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readInlets", NULL, js_read_inlets, NULL, NULL, NULL, napi_enumerable, NULL},
       {"snapshot", NULL, js_snapshot, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readHazard", NULL, js_read_hazard, NULL, NULL, NULL, napi_enumerable, NULL},
       {"restore", NULL, js_restore, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;

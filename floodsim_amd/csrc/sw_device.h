@@ -109,6 +109,10 @@ struct DevInst {
   uint8_t *act;                 // per strip: advanced in the current step
   const uint8_t *srain;         // per strip: an open cell with a positive rain multiplier
   const int32_t *sopen;         // per strip: open cells it owns
+  // hazard products (null: not tracked): A = (peak d*v, peak force, time deeper than 15 cm, first
+  // time deeper than 15 cm or -1), B = (peak speed, first time deeper than VCUT or -1, worst classes
+  // packed ARR + 8 FD2320 + 64 FD2320-urban, 0)
+  float4 *hzA, *hzB;
   // local time stepping (L > 0)
   int L, tx, ty;                // levels; LTS tiles across and down (8x8 cells)
   uint8_t *lv, *lv0;            // tile levels (smoothed, raw)

@@ -313,6 +313,42 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, size_t row, int X
   if (qh > hdry && !(gld(I.mask + row, X) & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
 }
 
+// Hazard products of a cell after a step of length dt ending at tnow (hazard, hzRecord,
+// kernels.wgsl.js:415-447; the classes of hazard.js:30-66): the worst ARR 2019 H1-H6 and FD2320
+// classes from depth and speed at the same moment, peak d*v and force per width, time deeper than
+// 15 cm and its first time, peak speed and the first time deeper than VCUT. Films under 1 cm are not
+// recorded; speed counts only from VCUT deep.
+constexpr float kHzWet = 0.01f, kHzVcut = 0.05f, kHzThr = 0.15f;
+__device__ __forceinline__ float hz_arr(float d, float v) {
+  const float dv = d * v;
+  if (dv <= 0.3f && d <= 0.3f && v <= 2.f) return 1.f;
+  if (dv <= 0.6f && d <= 0.5f && v <= 2.f) return 2.f;
+  if (dv <= 0.6f && d <= 1.2f && v <= 2.f) return 3.f;
+  if (dv <= 1.f && d <= 2.f && v <= 2.f) return 4.f;
+  if (dv <= 4.f && d <= 4.f && v <= 4.f) return 5.f;
+  return 6.f;
+}
+__device__ __forceinline__ float hz_hr(float hr) { return hr < 0.75f ? 1.f : hr < 1.25f ? 2.f : hr < 2.f ? 3.f : 4.f; }
+
+__device__ __noinline__ void hazard_cell(const DevInst &I, size_t row, int X, float d, float hu, float hv, float dt,
+                                         float tnow) {
+  if (!(d >= kHzWet)) return;
+  const float v = d >= kHzVcut ? sqrtf(hu * hu + hv * hv) / d : 0.f;
+  float4 a = gld(I.hzA + row, X), b = gld(I.hzB + row, X);
+  a.x = fmaxf(a.x, d * v);
+  a.y = fmaxf(a.y, d * v * v + 0.5f * I.g * d * d);
+  if (d > kHzThr) { a.z += dt; if (a.w < 0.f) a.w = tnow; }
+  if (d > kHzVcut) { b.x = fmaxf(b.x, v); if (b.y < 0.f) b.y = tnow; }
+  const float hr = d * (v + 0.5f);
+  const int packed = (int)b.z;
+  const int c0 = max(packed & 7, (int)hz_arr(d, v));
+  const int c1 = max((packed >> 3) & 7, (int)hz_hr(hr));
+  const int c2 = max(packed >> 6, (int)hz_hr(hr + (d > 0.25f ? 1.f : 0.f)));
+  b.z = (float)(c0 + 8 * c1 + 64 * c2);
+  gst(reinterpret_cast<fs_v4 *>(I.hzA + row), X, fs_v4{a.x, a.y, a.z, a.w});
+  gst(reinterpret_cast<fs_v4 *>(I.hzB + row), X, fs_v4{b.x, b.y, b.z, b.w});
+}
+
 // wavefront index for a device-side count of n work items: the blocks that have work are spread
 // round-robin over the 8 XCDs and each XCD takes one contiguous run of items; -1 = no work
 __device__ __forceinline__ int xcd_wave_index_n(int n, int nwaves_per_block) {
@@ -336,9 +372,9 @@ struct FaceRow { float F0, F1, F2, cL, cR, on; };
 // store; cells at m + 1 wait and gather the RK-weighted flux of those faces; the bed-slope term of a
 // stepping cell takes, for a face whose neighbour is finer, its reconstruction from that face's last
 // evaluation (kept in bsx/bsy by the finer record's last stage).
-template <int STAGE, bool LTS>
+template <int STAGE, bool LTS, bool HZ>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
-                                          Part *__restrict__ part, const int cells) {
+                                          Part *__restrict__ part, const int cells, const float tnow) {
   const float rain = I.rec->rainRate;
   const float g = I.g, hdry = I.hdry, hfront = I.hfront, inv_dx = I.inv_dx, dx = I.dx;
   const int so = I.secondOrder, vl = I.limiter;
@@ -575,6 +611,9 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
           }
           if (STAGE == 1) {
             sources_cell(I, row, X, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+            if constexpr (HZ) {
+              if (I.hzA) hazard_cell(I, row, X, qh, qhu, qhv, dt, tnow);
+            }
             if constexpr (LTS) {
               gst(I.sh + row, X, 0.f); gst(I.shu + row, X, 0.f); gst(I.shv + row, X, 0.f); gst(I.sw + row, X, 0.f);
             }
@@ -679,14 +718,23 @@ __device__ __forceinline__ int4 lts_rows(const DevInst &I, int4 wk, int m, int l
   return wk;
 }
 
+// end time of the record at base step kk of the current cycle (the last one ends the cycle exactly;
+// solver-cpu.js:525, kernels.wgsl.js:754)
+__device__ __forceinline__ float lts_record_end_time(const DevInst &I, int kk) {
+  const StepRec *rec = I.rec;
+  return (float)(kk + 1 == (1 << I.L) ? rec->cycT0 + rec->cycD : rec->cycT0 + (double)(kk + 1) * rec->base);
+}
+
 // LTS record of level m (single-instance groups): the parts of the level's units (lts_split), whose
 // number only the device knows; the grid covers kLtsWaveCap per unit and waves without a part exit.
-template <int STAGE>
+template <int STAGE, bool HZ>
 __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                Part *__restrict__ part, int m) {
   const int lane = threadIdx.x & 63;
   const DevInst &I = insts[0];
   if (STAGE == 1 && !I.secondOrder) return;
+  const int kk = m >> 8;   // the record's base step in its cycle (second stage: for the hazard clock)
+  m &= 0xff;
   const float dt = I.rec->ldt[m];
   const int split = lts_split(I, m);
   const int i = xcd_wave_index_n(I.rec->lcount[m] * split, 4);
@@ -700,12 +748,12 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
     if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0};
     return;
   }
-  run_strip<STAGE, true>(I, wk, lane, m, dt, part + i, 0);
+  run_strip<STAGE, true, HZ>(I, wk, lane, m, dt, part + i, 0, lts_record_end_time(I, kk));
 }
 
 // One RK stage over a work list. Global stepping: one wavefront per strip of `work`; a strip the
 // dry-strip rule leaves out writes an empty partial.
-template <int STAGE, bool LTS>
+template <int STAGE, bool LTS, bool HZ = false>
 __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                 int nwork, Part *__restrict__ part, int m) {
   static_assert(!LTS, "local time stepping records use the specialisations below");
@@ -723,7 +771,7 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
     return;
   }
-  const bool stir = run_strip<STAGE, false>(I, wk, lane, 0, dt, part + wid, I.sopen[u]);
+  const bool stir = run_strip<STAGE, false, HZ>(I, wk, lane, 0, dt, part + wid, I.sopen[u], (float)I.rec->tAfter);
   if (STAGE == 1 && I.wet && lane == 0) I.wet[u] = stir ? 1 : 0;
 }
 
@@ -731,16 +779,23 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
 #define FS_LTS_MIN_BLOCKS 1
 #endif
 template <>
-__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<0, true>(
+__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<0, true, false>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
   (void)nwork;
-  lts_stage_body<0>(insts, work, part, m);
+  lts_stage_body<0, false>(insts, work, part, m);
 }
 template <>
-__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<1, true>(
+__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<1, true, false>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
   (void)nwork;
-  lts_stage_body<1>(insts, work, part, m);
+  lts_stage_body<1, false>(insts, work, part, m);
+}
+// with the hazard products (fs_params.hazard)
+template <>
+__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<1, true, true>(
+    const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
+  (void)nwork;
+  lts_stage_body<1, true>(insts, work, part, m);
 }
 
 // ------------------------------------------------------------ local time stepping, per record
@@ -1009,9 +1064,9 @@ __global__ __launch_bounds__(256) void sw_speed(const DevInst *__restrict__ inst
 // first order (secondOrder = false): the Euler stage's U* plus the sources becomes U^{n+1}
 // (step, solver-cpu.js:432-438: one stage, then _sources). LTS: the level-m cells of a record,
 // whose store is then spent (solver-cpu.js:550-557).
-template <bool LTS>
+template <bool LTS, bool HZ>
 __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 wk, int lane, int m, float dt, Part *part,
-                                                   int cells) {
+                                                   int cells, float tnow) {
   double acc_inf = 0.0;
   float smax = 0.f;
   bool stir = false;
@@ -1024,6 +1079,9 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
       if (LTS && tile_level(I, X, y) != m) continue;
       float qh = gld(I.h1 + row, X), qw = gld(I.w1 + row, X), qhu = gld(I.hu1 + row, X), qhv = gld(I.hv1 + row, X);
       sources_cell(I, row, X, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+      if constexpr (HZ) {
+        if (I.hzA) hazard_cell(I, row, X, qh, qhu, qhv, dt, tnow);
+      }
       gst(I.h + row, X, qh); gst(I.w + row, X, qw); gst(I.hu + row, X, qhu); gst(I.hv + row, X, qhv);
       if (LTS) { gst(I.sh + row, X, 0.f); gst(I.shu + row, X, 0.f); gst(I.shv + row, X, 0.f); gst(I.sw + row, X, 0.f); }
       stir = stir || qh != 0.f || qw != 0.f || qhu != 0.f || qhv != 0.f;
@@ -1035,7 +1093,7 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
   return __any(stir);
 }
 
-template <bool LTS>
+template <bool LTS, bool HZ = false>
 __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict__ insts,
                                                        const int4 *__restrict__ work, int nwork,
                                                        Part *__restrict__ part, int m) {
@@ -1052,18 +1110,21 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
       return;
     }
     const float dt = I.rec->dt;
-    const bool stir = euler_finish_strip<false>(I, wk, lane, 0, dt, part + wid, dt > 0.f ? I.sopen[u] : 0);
+    const bool stir = euler_finish_strip<false, HZ>(I, wk, lane, 0, dt, part + wid, dt > 0.f ? I.sopen[u] : 0,
+                                                (float)I.rec->tAfter);
     if (I.wet && dt > 0.f && lane == 0) I.wet[u] = stir ? 1 : 0;
   } else {
     const DevInst &I = insts[0];
     if (I.secondOrder) return;
+    const int kk = m >> 8;
+    m &= 0xff;
     const float dt = I.rec->ldt[m];
     const int split = lts_split(I, m);
     const int i = xcd_wave_index_n(I.rec->lcount[m] * split, 4);
     if (i < 0) return;
     const int u = I.ulist[m * (I.wend - I.wbeg) + i / split];
     const int4 wk = lts_rows(I, work[I.wbeg + u], m, lane, split, i % split);
-    euler_finish_strip<true>(I, wk, lane, m, wk.w > 0 ? dt : 0.f, part + i, 0);
+    euler_finish_strip<true, HZ>(I, wk, lane, m, wk.w > 0 ? dt : 0.f, part + i, 0, lts_record_end_time(I, kk));
   }
 }
 

@@ -19,7 +19,7 @@
  *   readInlets() -> {vol: Float64Array, h: Float32Array, app}, advance(tTarget, K = 64) with the
  *   'GPU solver made no progress (device error?)' error, frame(tEnd, K, readOut) ->
  *   Float32Array (depth, speed per cell) or null, readState() -> {h, hu, hv, hmax, mass},
- *   tileStats(), destroy(); fields t, steps, lastDt, dtAvg, vol, nx, ny, dx, NX, NY, hdry, zref,
+ *   readHazard(), tileStats(), destroy(); fields t, steps, lastDt, dtAvg, vol, nx, ny, dx, NX, NY, hdry, zref,
  *   zh, idx(i, j), lts, ltsOff, tiled, bc, ready (solver-gpu.js:550-693).
  *
  * The device argument is accepted for signature compatibility and ignored: the library drives
@@ -66,6 +66,8 @@ class GPUSolver {
     this.lts = o.lts === true ? 3 : (o.lts | 0);
     this.ltsOff = null;
     this.tiled = o.tiles !== false;
+    // hazard products every step, as the reference always tracks them; hazard: false leaves them out
+    this.hazard = o.hazard !== false;
     const n = o.nx * o.ny;
     const z = f64(o.z, n, 'z');
     let zmin = Infinity;
@@ -80,7 +82,7 @@ class GPUSolver {
       nx: o.nx, ny: o.ny, dx: o.dx, g: this.g, cfl: this.cfl, hdry: this.hdry, hfront: this.hfront,
       dtDry: this.dtDry, secondOrder: this.secondOrder ? 1 : 0, limiter: this.limiter,
       bcWest: BC[this.bc.west], bcEast: BC[this.bc.east], bcSouth: BC[this.bc.south], bcNorth: BC[this.bc.north],
-      lts: this.lts, rows: o.rows | 0, untiled: this.tiled ? 0 : 1,
+      lts: this.lts, rows: o.rows | 0, untiled: this.tiled ? 0 : 1, hazard: this.hazard ? 1 : 0,
     };
     for (const side of ['west', 'east', 'south', 'north']) {
       if (BC[this.bc[side]] === undefined) throw new Error('GPUSolver (HIP): bc.' + side + ' = ' + this.bc[side]);
@@ -240,6 +242,13 @@ class GPUSolver {
       const s = addon.tileStats(this._h);
       return { tiled: this.tiled, tiles: Math.ceil(this.nx / 8) * Math.ceil(this.ny / 8), update: s.active, cells: s.cells };
     });
+  }
+
+  /** Hazard products per interior cell (solver-gpu.js:515-529): dv, force, duration, arrival, vmax,
+   *  tWet (Float32Array; NaN = never) and the worst class per standard, cls.{arr, defra, defraUrban}. */
+  readHazard() {
+    this._raiseDeferred();
+    return this._enqueue(() => addon.readHazard(this._h));
   }
 
   /** Everything the solver holds on the device, with its clock and books (checkpoint.js:128-150). */

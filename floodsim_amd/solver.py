@@ -33,7 +33,7 @@ class fs_params(ctypes.Structure):
         ("hdry", ctypes.c_double), ("hfront", ctypes.c_double), ("dtDry", ctypes.c_double),
         ("secondOrder", ctypes.c_int32), ("limiter", ctypes.c_int32),
         ("bc", ctypes.c_int32 * 4), ("lts", ctypes.c_int32), ("rows", ctypes.c_int32),
-        ("untiled", ctypes.c_int32),
+        ("untiled", ctypes.c_int32), ("hazard", ctypes.c_int32),
     ]
 
 
@@ -55,7 +55,7 @@ EXPORTS = [
     "fs_pool_advance", "fs_pool_step", "fs_pool_read_state", "fs_pool_clock", "fs_pool_bytes_used",
     "fs_pool_kernel_ms", "fs_pool_set_timing", "fs_solver_pool", "fs_version", "fs_copy_probe",
     "fs_pool_dominant_kernel", "fs_lts_levels", "fs_pool_set_rows", "fs_snapshot_bytes", "fs_snapshot",
-    "fs_restore", "fs_pool_snapshot_bytes", "fs_pool_snapshot", "fs_pool_restore",
+    "fs_restore", "fs_pool_snapshot_bytes", "fs_pool_snapshot", "fs_pool_restore", "fs_read_hazard",
 ]
 
 
@@ -121,6 +121,8 @@ def lib(path=None):
     L.fs_restore.argtypes = [P, P, SZ]
     L.fs_pool_snapshot.argtypes = [P, I32, P, SZ]
     L.fs_pool_restore.argtypes = [P, I32, P, SZ]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.fs_read_hazard.argtypes = [P, fp, fp, fp, fp, fp, fp, u8p, u8p, u8p]
     L.fs_solver_pool.restype = P
     L.fs_solver_pool.argtypes = [P]
     _LIB = L
@@ -136,7 +138,7 @@ def _f64(a, n):
 
 
 def make_params(nx, ny, dx, bc=None, secondOrder=True, limiter="minmod", cfl=None, hdry=1e-5, hfront=1e-3,
-                dtDry=1.0, g=9.81, lts=0, rows=0, tiles=True, **_):
+                dtDry=1.0, g=9.81, lts=0, rows=0, tiles=True, hazard=False, **_):
     """fs_params with the defaults of GPUSolver (solver-gpu.js:39-47)."""
     p = fs_params()
     p.nx, p.ny, p.dx, p.g = int(nx), int(ny), float(dx), float(g)
@@ -153,6 +155,7 @@ def make_params(nx, ny, dx, bc=None, secondOrder=True, limiter="minmod", cfl=Non
     p.lts = int(lts or 0)
     p.rows = int(rows or 0)
     p.untiled = 0 if tiles else 1   # tiles=False: every strip every step (the reference's o.tiles)
+    p.hazard = 1 if hazard else 0   # hazard products every step (readHazard)
     return p
 
 
@@ -311,6 +314,19 @@ class GPUSolver:
         self._chk(lib().fs_read_state(self._h, _dp(h), _dp(hu), _dp(hv), _dp(hm), ctypes.byref(m)))
         mass = {k: getattr(m, k) for k, _ in fs_mass._fields_}
         return {"h": h, "hu": hu, "hv": hv, "hmax": hm, "mass": mass}
+
+    def readHazard(self):
+        """Hazard products per interior cell (readHazard, solver-gpu.js:515-529); needs hazard=True."""
+        n = self.nx * self.ny
+        f = {k: np.zeros(n, dtype=np.float32) for k in ("dv", "force", "duration", "arrival", "vmax", "tWet")}
+        c = {k: np.zeros(n, dtype=np.uint8) for k in ("arr", "defra", "defraUrban")}
+        fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        up = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        self._chk(lib().fs_read_hazard(self._h, fp(f["dv"]), fp(f["force"]), fp(f["duration"]), fp(f["arrival"]),
+                                       fp(f["vmax"]), fp(f["tWet"]), up(c["arr"]), up(c["defra"]),
+                                       up(c["defraUrban"])))
+        f["cls"] = c
+        return f
 
     # --- device-state snapshot / restore (checkpoint.js:128-150)
     def snapshot(self):

@@ -48,6 +48,8 @@ typedef struct fs_params {
   int32_t rows;        /* rows per wavefront strip (0 = autotuned) */
   int32_t untiled;     /* 1: advance every strip every step (the reference's tiles: false); 0: skip
                           quiet dry strips (the reference's active tiles, kernels.wgsl.js:639-673) */
+  int32_t hazard;      /* 1: track the hazard products every step (hazard/hzRecord,
+                          kernels.wgsl.js:415-447, which the reference always runs) */
 } fs_params;
 
 /* clock fields of GPUSolver: t, steps, lastDt, dtAvg (solver-gpu.js:578-586) */
@@ -91,6 +93,12 @@ int fs_clock_get(fs_handle s, fs_clock *clk);
 /* LTS tile levels of the last cycle (8x8 tiles, tx*ty bytes, row-major): the reference keeps them
  * in its TL buffer (kernels.wgsl.js:307) and CPUSolver.LV (solver-cpu.js:124) */
 int fs_lts_levels(fs_handle s, uint8_t *levels, int32_t *tx, int32_t *ty);
+/* readHazard (solver-gpu.js:515-529): per interior cell peak d*v (m^2/s), peak force per width (kN/m),
+ * time deeper than 15 cm (s), its first time (s, NaN never), peak speed (m/s), first time deeper than
+ * 5 cm (s, NaN never), worst ARR 2019 H1-H6, FD2320 and FD2320-urban classes (0 never classed); any
+ * output may be NULL. Needs fs_params.hazard = 1. */
+int fs_read_hazard(fs_handle s, float *dv, float *force, float *duration, float *arrival, float *vmax, float *tWet,
+                   uint8_t *arr, uint8_t *defra, uint8_t *defraUrban);
 /* tileStats (:625-629): open cells advanced by the last step (global stepping skips quiet dry
  * strips) and the interior cell total */
 int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total);

@@ -80,6 +80,14 @@ def lib():
         L.or_compute_dt.restype = D
         L.or_compute_dt.argtypes = [P]
         L.or_get_field.argtypes = [P, I, dp]
+        L.or_get_hazard.argtypes = [P, I, dp]
+        L.or_hazard_cell.argtypes = [P, I, D, D, D, D]
+        L.or_hz_arr.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.or_hz_arr.restype = I
+        L.or_hz_hr.argtypes = [ctypes.c_double]
+        L.or_hz_hr.restype = I
+        L.or_hz_force.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.or_hz_force.restype = ctypes.c_double
         L.or_get_levels.restype = I
         L.or_get_levels.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
         L.or_books.argtypes = [P, dp]
@@ -229,6 +237,17 @@ class CPUSolver:
     def field(self, name):
         out = np.empty(self.nx * self.ny)
         lib().or_get_field(self._h, _FIELDS[name], _dp(out))
+        return out
+
+    def readHazard(self):
+        """Hazard products per interior cell (solver-cpu.js:140 readHazard; hazard.js:75-96)."""
+        n = self.nx * self.ny
+        names = ("dv", "force", "duration", "arrival", "vmax", "tWet", "arr", "defra", "defraUrban")
+        out = {}
+        for k, name in enumerate(names):
+            a = np.zeros(n)
+            lib().or_get_hazard(self._h, k, _dp(a))
+            out[name] = a if k < 6 else a.astype(np.uint8)
         return out
 
     def levels(self):

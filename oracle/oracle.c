@@ -69,6 +69,11 @@ typedef struct {
   int *inC;
   double *inQ, *inV;
   double exchanged;
+  /* hazard products per interior cell (HazardTracker, hazard.js:75-96): peak d*v, peak force per
+   * width, time deeper than 15 cm and when first, peak speed and when first deeper than VCUT, and the
+   * worst class per standard */
+  double *hzDv, *hzForce, *hzDur, *hzArr, *hzVmax, *hzWet;
+  uint8_t *hzCls; /* 3 per cell: ARR H1-H6, FD2320, FD2320 urban */
 } or_solver;
 
 static double minmod(double a, double b) {
@@ -106,6 +111,12 @@ or_solver *or_create(const or_params *p, const double *z, const double *manning,
   s->maxDepth = zalloc(N); s->TH = zalloc(N);
   s->rowA = zalloc((size_t)s->NY); s->rowB = zalloc((size_t)s->NY);
   s->lev = -1;
+  {
+    size_t n = (size_t)p->nx * p->ny;
+    s->hzDv = zalloc(n); s->hzForce = zalloc(n); s->hzDur = zalloc(n); s->hzArr = zalloc(n);
+    s->hzVmax = zalloc(n); s->hzWet = zalloc(n); s->hzCls = (uint8_t *)calloc(3 * n, 1);
+    for (size_t k = 0; k < n; k++) { s->hzArr[k] = NAN; s->hzWet[k] = NAN; }
+  }
   for (int j = 0; j < p->ny; j++)
     for (int i = 0; i < p->nx; i++) {
       int c = (j + GH) * s->NX + i + GH, k = j * p->nx + i;
@@ -147,6 +158,7 @@ void or_destroy(or_solver *s) {
                     s->rowA, s->rowB, s->tsp, s->aH, s->aHu, s->aHv, s->inQ, s->inV};
   for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) free(arrs[k]);
   free(s->M); free(s->LV); free(s->TI); free(s->inC);
+  free(s->hzDv); free(s->hzForce); free(s->hzDur); free(s->hzArr); free(s->hzVmax); free(s->hzWet); free(s->hzCls);
   free(s);
 }
 
@@ -485,7 +497,42 @@ static void fixcells(or_solver *s, int sel) {
 }
 
 /* semi-implicit Manning friction, rain, infiltration, max depth (solver-cpu.js:456-483); sel as fixcells */
-static void sources(or_solver *s, double dt, int sel) {
+/* hazard standards (hazard.js:30-66): ARR 2019 / AIDR 7-3 H1-H6 and FD2320 HR = d(v + 0.5) + DF */
+#define HZ_WET 0.01
+#define HZ_VCUT 0.05
+#define HZ_THR 0.15
+int or_hz_arr(double d, double v) {
+  double dv = d * v;
+  if (dv <= 0.3 && d <= 0.3 && v <= 2) return 1;
+  if (dv <= 0.6 && d <= 0.5 && v <= 2) return 2;
+  if (dv <= 0.6 && d <= 1.2 && v <= 2) return 3;
+  if (dv <= 1.0 && d <= 2.0 && v <= 2) return 4;
+  if (dv <= 4.0 && d <= 4.0 && v <= 4) return 5;
+  return 6;
+}
+int or_hz_hr(double hr) { return hr < 0.75 ? 1 : hr < 1.25 ? 2 : hr < 2 ? 3 : 4; }
+/* momentum flux per metre of width, kN/m: rho (d v^2 + g d^2 / 2) / 1000 with rho = 1000 kg/m^3
+ * (hazard.js:71) */
+double or_hz_force(double d, double v, double g) { return d * v * v + 0.5 * g * d * d; }
+
+/* HazardTracker.cell (hazard.js:81-96) for interior cell k after a step of length dt ending at t */
+static void hazard_cell(or_solver *s, int k, double d, double v, double t, double dt) {
+  if (!(d >= HZ_WET)) return;
+  double dv = d * v;
+  if (dv > s->hzDv[k]) s->hzDv[k] = dv;
+  double f = or_hz_force(d, v, s->p.g);
+  if (f > s->hzForce[k]) s->hzForce[k] = f;
+  if (d > HZ_THR) { s->hzDur[k] += dt; if (isnan(s->hzArr[k])) s->hzArr[k] = t; }
+  if (d > HZ_VCUT) { if (v > s->hzVmax[k]) s->hzVmax[k] = v; if (isnan(s->hzWet[k])) s->hzWet[k] = t; }
+  int c[3] = {or_hz_arr(d, v), or_hz_hr(d * (v + 0.5)), or_hz_hr(d * (v + 0.5) + (d > 0.25 ? 1 : 0))};
+  for (int q = 0; q < 3; q++)
+    if (c[q] > s->hzCls[3 * k + q]) s->hzCls[3 * k + q] = (uint8_t)c[q];
+}
+
+/* the tracker on one cell, for known-answer tests of its rules */
+void or_hazard_cell(or_solver *s, int k, double d, double v, double t, double dt) { hazard_cell(s, k, d, v, t, dt); }
+
+static void sources(or_solver *s, double dt, int sel, double tEnd) {
   const int NX = s->NX, NY = s->NY;
   const double g = s->p.g, hdry = s->p.hdry, r0 = s->rainRate, A = s->p.dx * s->p.dx;
 #pragma omp parallel for schedule(static)
@@ -517,6 +564,17 @@ static void sources(or_solver *s, double dt, int sel) {
   for (int j = GH; j < NY - GH; j++) { rv += s->rowA[j]; iv += s->rowB[j]; }
   s->volRain += rv * A;
   s->volInfil += iv * A;
+  /* hazard products after the sources, the step ending at tEnd (solver-cpu.js:484-487) */
+#pragma omp parallel for schedule(static)
+  for (int j = GH; j < NY - GH; j++)
+    for (int i = GH; i < NX - GH; i++) {
+      int c = j * NX + i;
+      if (!s->M[c]) continue;
+      if (sel >= 0 && lvc(s, c) != sel) continue;
+      double d = s->h[c];
+      double v = d >= HZ_VCUT ? sqrt(s->hu[c] * s->hu[c] + s->hv[c] * s->hv[c]) / d : 0;
+      hazard_cell(s, (j - GH) * s->p.nx + (i - GH), d, v, tEnd, dt);
+    }
 }
 
 /* inlet exchange over dt (exec-cpu.mjs:46-55); sel: -1 all inlets, else inlets whose cell is at that level */
@@ -572,7 +630,7 @@ double or_step(or_solver *s, double dtMax) {
     fixcells(s, -1);
     s->volOutflow += dt * q;
   }
-  sources(s, dt, -1);
+  sources(s, dt, -1, s->t + dt);
   if (s->nIn) exchange(s, dt, -1);
   s->t += dt;
   s->steps++;
@@ -580,7 +638,7 @@ double or_step(or_solver *s, double dtMax) {
 }
 
 /* one level-m step of an LTS cycle, dt = 2^m * base step (solver-cpu.js:529-558) */
-static void record(or_solver *s, int m, double dt) {
+static void record(or_solver *s, int m, double dt, double tEnd) {
   double *h = s->h, *hu = s->hu, *hv = s->hv;
   const double hdry = s->p.hdry;
   (void)hdry;
@@ -624,7 +682,7 @@ static void record(or_solver *s, int m, double dt) {
   }
   INTERIOR_M(if (OWN(c)) { s->aH[c] = 0; s->aHu[c] = 0; s->aHv[c] = 0; })
   s->lev = -1;
-  sources(s, dt, m);
+  sources(s, dt, m, tEnd);
   if (s->nIn) exchange(s, dt, m);
 #undef OWN
 #undef COARSE
@@ -672,7 +730,7 @@ static double cycle(or_solver *s, double dtMax) {
   double t0 = s->t;
   for (int k = 0; k < P; k++)
     for (int m = 0; m <= L; m++)
-      if ((k + 1) % (1 << m) == 0) record(s, m, dl * (1 << m));
+      if ((k + 1) % (1 << m) == 0) record(s, m, dl * (1 << m), k == P - 1 ? t0 + D : t0 + (k + 1) * dl);
   s->t = t0 + D;
   s->steps++;
   return D;
@@ -687,6 +745,14 @@ long or_run_steps(or_solver *s, double tEnd, long k) {
   long n = 0;
   while (n < k && s->t < tEnd - 1e-12) { or_step(s, tEnd - s->t); n++; }
   return n;
+}
+
+/* hazard field: 0 d*v, 1 force, 2 duration, 3 arrival (NaN never), 4 peak speed, 5 first time deeper
+ * than VCUT (NaN never), 6 ARR class, 7 FD2320 class, 8 FD2320 urban class; interior nx*ny */
+void or_get_hazard(const or_solver *s, int which, double *out) {
+  size_t n = (size_t)s->p.nx * s->p.ny;
+  const double *src[6] = {s->hzDv, s->hzForce, s->hzDur, s->hzArr, s->hzVmax, s->hzWet};
+  for (size_t k = 0; k < n; k++) out[k] = which < 6 ? src[which][k] : (double)s->hzCls[3 * k + (which - 6)];
 }
 
 /* field: 0 h, 1 hu, 2 hv, 3 maxDepth, 4 z, 5 aH; interior nx*ny row-major */

@@ -27,6 +27,9 @@ async function main() {
   if (s.nInlets) { const r = await s.readInlets(); for (const v of r.vol) inletVol += v; }
   const st = await s.readState();
   const tiles = await s.tileStats();
+  const hz = await s.readHazard();
+  let hzCells = 0;
+  for (let k = 0; k < hz.cls.arr.length; k++) if (hz.cls.arr[k] > 0) hzCells++;
   const t1 = s.t, steps1 = s.steps, lastDt1 = s.lastDt, volRain1 = s.vol.rain;
   // snapshot, run on, restore, run the same span again: the same bits
   const snap = await s.snapshot();
@@ -43,7 +46,7 @@ async function main() {
   process.stdout.write(JSON.stringify({
     t: t1, steps: steps1, lastDt: lastDt1, h: Array.from(st.h), hu: Array.from(st.hu), hv: Array.from(st.hv),
     hmax: Array.from(st.hmax), mass: st.mass, frameSum, inletVol, volRain: volRain1, tiles,
-    snapshotBytes: snap.bytes.length, restoredT: tBack, restoreSame: same,
+    snapshotBytes: snap.bytes.length, hazardCells: hzCells, hazardForceMax: Math.max.apply(null, Array.from(hz.force)), restoredT: tBack, restoreSame: same,
   }) + '\n');
 }
 

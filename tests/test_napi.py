@@ -19,8 +19,8 @@ NODE = shutil.which("node")
 
 pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
-ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readInlets", "readState", "restore",
-               "setInletRates", "setRain", "setRainField", "setState", "snapshot", "tileStats", "version"]
+ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readHazard", "readInlets", "readState",
+               "restore", "setInletRates", "setRain", "setRainField", "setState", "snapshot", "tileStats", "version"]
 
 
 def node(script, *args, timeout=120):
@@ -93,6 +93,7 @@ def test_js_matches_python_binding_and_oracle(tmp_path):
     assert r.returncode == 0, r.stderr
     js = json.loads(r.stdout)
     assert js["restoreSame"] and js["snapshotBytes"] > 0
+    assert js["hazardCells"] > 0 and js["hazardForceMax"] > 0     # hazard products on by default, as the reference
     py, steps = _python_run(c)
     # the same C ABI under both bindings: identical bits
     assert js["steps"] == steps

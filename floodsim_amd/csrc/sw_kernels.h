@@ -406,9 +406,9 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   FaceRow xf1 = {0, 0, 0, 0, 0, 1}, xf2 = {0, 0, 0, 0, 0, 1}, xf3 = {0, 0, 0, 0, 0, 1};
   FaceRow yf3 = {0, 0, 0, 0, 0, 1}, yf4 = {0, 0, 0, 0, 0, 1};
   float th3 = 1.f, th4 = 1.f;
+  // bed-slope terms of rows r-1..r-3 (x) and r-2..r-3 (y); under LTS a stepping cell's term already
+  // holds, for a face whose neighbour is finer, that face's last evaluation
   float sx1 = 0, sx2 = 0, sx3 = 0, sy2 = 0, sy3 = 0;
-  // LTS: the cell's own reconstruction toward each face (h, dz) for rows r-1..r-3 (x) and r-2..r-3 (y)
-  float4 rx1 = {0, 0, 0, 0}, rx2 = {0, 0, 0, 0}, rx3 = {0, 0, 0, 0}, ry2 = {0, 0, 0, 0}, ry3 = {0, 0, 0, 0};
 
   double acc_out = 0.0, acc_clip = 0.0, acc_inf = 0.0;
   float smax = 0.f;
@@ -443,13 +443,24 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         xf0.on = min(p.lv, lvr) == m ? 1.f : 0.f;
       }
     }
+    float sxr = sx0;
     if constexpr (LTS) {
       // a waiting cell keeps its reconstruction toward the faces this record evaluates (last stage)
       const float onW = from_left(xf0.on);
-      if (last && own_col && p.in && p.M && p.lv == m + 1 && r >= y0 && r < y1) {
+      const bool row_in = own_col && p.in && p.M && r >= y0 && r < y1;
+      if (last && row_in && p.lv == m + 1) {
         float2 *b = reinterpret_cast<float2 *>(I.bsx + (size_t)r * pitch);
         if (onW > 0.f) gst(b, 2 * X, make_float2(rx0.x, rx0.y));
         if (xf0.on > 0.f) gst(b, 2 * X + 1, make_float2(rx0.z, rx0.w));
+      }
+      // a stepping cell's bed slope from its face reconstructions (solver-cpu.js:363-368): a face with
+      // a finer neighbour contributes the values of its last evaluation
+      if (row_in && p.lv == m && (onW == 0.f || xf0.on == 0.f)) {
+        const float4 b = gld(I.bsx + (size_t)r * pitch, X);
+        float4 xs = rx0;
+        if (onW == 0.f) { xs.x = b.x; xs.y = b.y; }
+        if (xf0.on == 0.f) { xs.z = b.z; xs.w = b.w; }
+        sxr = hg * (xs.x + xs.z) * (xs.w - xs.y);
       }
     }
 
@@ -468,13 +479,22 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       yf2 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
       if constexpr (LTS) yf2.on = min(lv2, lv1) == m ? 1.f : 0.f;
     }
+    float syr = sy1;
     if constexpr (LTS) {
       const int ry = r - 1;
       const bool onN = min(lv1, p.lv) == m;
-      if (last && own_col && in1 && M1r && lv1 == m + 1 && ry >= y0 && ry < y1) {
+      const bool row_in = own_col && in1 && M1r && ry >= y0 && ry < y1;
+      if (last && row_in && lv1 == m + 1) {
         float2 *b = reinterpret_cast<float2 *>(I.bsy + (size_t)ry * pitch);
         if (yf2.on > 0.f) gst(b, 2 * X, make_float2(ry1.x, ry1.y));
         if (onN) gst(b, 2 * X + 1, make_float2(ry1.z, ry1.w));
+      }
+      if (row_in && lv1 == m && (yf2.on == 0.f || !onN)) {
+        const float4 b = gld(I.bsy + (size_t)ry * pitch, X);
+        float4 ys = ry1;
+        if (yf2.on == 0.f) { ys.x = b.x; ys.y = b.y; }
+        if (!onN) { ys.z = b.z; ys.w = b.w; }
+        syr = hg * (ys.x + ys.z) * (ys.w - ys.y);
       }
     }
 
@@ -534,26 +554,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         const size_t row = (size_t)ry * pitch;
         float bx = sx3, by = sy3;
         if constexpr (LTS) {
-          if (own) {
-            // bed slope from the face reconstructions (solver-cpu.js:363-368): a face with a finer
-            // neighbour contributes the values of its last evaluation
-            float4 xs = rx3, ys = ry3;
-            if (kw == 0.f || ke == 0.f) {
-              const float4 b = gld(I.bsx + row, X);
-              if (kw == 0.f) { xs.x = b.x; xs.y = b.y; }
-              if (ke == 0.f) { xs.z = b.z; xs.w = b.w; }
-            }
-            if (ks == 0.f || kn == 0.f) {
-              const float4 b = gld(I.bsy + row, X);
-              if (ks == 0.f) { ys.x = b.x; ys.y = b.y; }
-              if (kn == 0.f) { ys.z = b.z; ys.w = b.w; }
-            }
-            bx = hg * (xs.x + xs.z) * (xs.w - xs.y);
-            by = hg * (ys.x + ys.z) * (ys.w - ys.y);
-          } else {
-            bx = 0.f;
-            by = 0.f;
-          }
+          if (!own) { bx = 0.f; by = 0.f; }   // a waiting cell gathers fluxes only
         }
         const float dh = -inv_dx * (ke * Fe0 - kw * w0 + kn * Fn0 - ks * Fs0);
         const float dhu = -inv_dx * (ke * (Fe1 + cLe) - kw * (w1 + cRw) + bx + kn * Fn2 - ks * Fs2);
@@ -628,9 +629,9 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     yf4 = yf3; yf3 = yf2;
     th4 = th3; th3 = th2;
     xf3 = xf2; xf2 = xf1; xf1 = xf0;
-    sx3 = sx2; sx2 = sx1; sx1 = sx0;
-    sy3 = sy2; sy2 = sy1;
-    if constexpr (LTS) { rx3 = rx2; rx2 = rx1; rx1 = rx0; ry3 = ry2; ry2 = ry1; lv3 = lv2; lv2 = lv1; lv1 = p.lv; }
+    sx3 = sx2; sx2 = sx1; sx1 = sxr;
+    sy3 = sy2; sy2 = syr;
+    if constexpr (LTS) { lv3 = lv2; lv2 = lv1; lv1 = p.lv; }
     yp_h = yh_p; yp_e = ye_p; yp_n = yn_p; yp_t = yt_p;
     hh3 = h2r; hu3 = hu2; hv3 = hv2; M3 = M2r; in3 = in2;
     h2r = h1r; u2r = u1r; v2r = v1r; z2r = z1r; M2r = M1r; hu2 = hu1; hv2 = hv1; in2 = in1;

@@ -574,15 +574,23 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
           }
         } else {
           float qh = hh3, qhu = hu3, qhv = hv3, qw = gld(Ws + row, X);
-          // RK start state: U^n, plus (LTS) what finer neighbours passed while the cell waited
+          // RK start state: U^n (the window's values in the first stage), plus (LTS) what finer
+          // neighbours passed while the cell waited
           float h0 = 0.f, w0n = 0.f, hu0 = 0.f, hv0 = 0.f;
+          bool spent = false;   // LTS: the cell's store held gathered flux (cleared once used)
           if (STAGE == 1 || LTS) {
-            h0 = gld(I.h + row, X); w0n = gld(I.w + row, X); hu0 = gld(I.hu + row, X); hv0 = gld(I.hv + row, X);
+            if (STAGE == 1) {
+              h0 = gld(I.h + row, X); w0n = gld(I.w + row, X); hu0 = gld(I.hu + row, X); hv0 = gld(I.hv + row, X);
+            } else {
+              h0 = hh3; w0n = qw; hu0 = hu3; hv0 = hv3;
+            }
             if constexpr (LTS) {
               const float sh = gld(I.sh + row, X), sw = gld(I.sw + row, X);
-              hu0 += gld(I.shu + row, X);
-              hv0 += gld(I.shv + row, X);
+              const float shu = gld(I.shu + row, X), shv = gld(I.shv + row, X);
+              hu0 += shu;
+              hv0 += shv;
               kadd(h0, w0n, sh - sw);
+              spent = sh != 0.f || sw != 0.f || shu != 0.f || shv != 0.f;
             }
           }
           if (STAGE == 0) {
@@ -616,7 +624,9 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
               if (I.hzA) hazard_cell(I, row, X, qh, qhu, qhv, dt, tnow);
             }
             if constexpr (LTS) {
-              gst(I.sh + row, X, 0.f); gst(I.shu + row, X, 0.f); gst(I.shv + row, X, 0.f); gst(I.sw + row, X, 0.f);
+              if (spent) {
+                gst(I.sh + row, X, 0.f); gst(I.shu + row, X, 0.f); gst(I.shv + row, X, 0.f); gst(I.sw + row, X, 0.f);
+              }
             }
           }
           gst(Ho + row, X, qh); gst(HUo + row, X, qhu); gst(HVo + row, X, qhv); gst(Wo + row, X, qw);

@@ -54,6 +54,7 @@ struct Slot {
   int L = 0, tx = 0, ty = 0, nunits = 0;
   uint8_t *lv = nullptr, *lv0 = nullptr, *gflag = nullptr, *umask = nullptr;
   float *tsp = nullptr, *tileRf = nullptr;
+  unsigned *speedSlots = nullptr;
   int32_t *tileOpen = nullptr, *ucells = nullptr, *ulist = nullptr;
   float *sh = nullptr, *shu = nullptr, *shv = nullptr, *sw = nullptr;
   float *g0h = nullptr, *g0hu = nullptr, *g0hv = nullptr, *g1h = nullptr, *g1hu = nullptr, *g1hv = nullptr;
@@ -163,6 +164,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
   if (p->lts) {
     const size_t nt = (size_t)((p->nx + 7) / 8) * ((p->ny + 7) / 8), nu = (size_t)units_of(p);
     b += 2 * align256(nt) + align256(nt * 4) * 3;           // lv, lv0, tsp, tileOpen, tileRf
+    b += align256(kSpeedSlots * 4);                          // tile-speed block maxima
     b += 10 * align256(N * 4) + align256(N) + 2 * align256(N * 16);  // store, gathers, flag, bsx, bsy
     b += align256(nu) + 2 * align256(nu * 4 * (kMaxLevels + 1));     // umask, ucells, ulist
   }
@@ -272,7 +274,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.act = S.sact; d.srain = S.srain; d.sopen = S.sopen;
     d.hzA = S.hzA; d.hzB = S.hzB;
     d.L = S.L; d.tx = S.tx; d.ty = S.ty;
-    d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.tileOpen = S.tileOpen; d.tileRf = S.rf ? S.tileRf : nullptr;
+    d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.speedSlots = S.speedSlots; d.tileOpen = S.tileOpen; d.tileRf = S.rf ? S.tileRf : nullptr;
     d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
     d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.g1h = S.g1h; d.g1hu = S.g1hu; d.g1hv = S.g1hv;
     d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
@@ -699,6 +701,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     S.nunits = units_of(p);
     S.lv = (uint8_t *)take(nt); S.lv0 = (uint8_t *)take(nt);
     S.tsp = (float *)take(nt * 4); S.tileOpen = (int32_t *)take(nt * 4); S.tileRf = (float *)take(nt * 4);
+    S.speedSlots = (unsigned *)take(kSpeedSlots * 4);
     S.sh = (float *)take(N * 4); S.shu = (float *)take(N * 4); S.shv = (float *)take(N * 4); S.sw = (float *)take(N * 4);
     S.g0h = (float *)take(N * 4); S.g0hu = (float *)take(N * 4); S.g0hv = (float *)take(N * 4);
     S.g1h = (float *)take(N * 4); S.g1hu = (float *)take(N * 4); S.g1hv = (float *)take(N * 4);
@@ -753,6 +756,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     HIPCHK(P, hipMemcpy(S.tileOpen, to.data(), nt * 4, hipMemcpyHostToDevice));
     HIPCHK(P, hipMemcpy(S.tileRf, tr.data(), nt * 4, hipMemcpyHostToDevice));
     HIPCHK(P, hipMemset(S.lv, 0, nt));
+    HIPCHK(P, hipMemset(S.speedSlots, 0, kSpeedSlots * 4));
     HIPCHK(P, hipMemset(S.sh, 0, (size_t)((char *)S.bsx - (char *)S.sh)));      // store, gathers, flags
     HIPCHK(P, hipMemset(S.bsx, 0, N * 32));
     HIPCHK(P, hipDeviceSynchronize());

@@ -12,6 +12,7 @@ constexpr int kOwn = kLanes - 2 * kHalo;     // columns a wavefront strip owns (
 constexpr int kLtsTile = 8;                  // LTS level tiles (solver-cpu.js:14)
 constexpr int kMaxLevels = 4;                // LTS levels 1..4 (solver-cpu.js:121)
 constexpr double kLtsSafe = 0.9;             // share of its CFL step a tile may take (solver-cpu.js:15)
+constexpr int kSpeedSlots = 4096;            // LTS: slots of the per-block tile-speed maxima
 
 enum { BC_WALL = 0, BC_OPEN = 1, BC_OUTFLOW = 2 };
 
@@ -117,6 +118,7 @@ struct DevInst {
   int L, tx, ty;                // levels; LTS tiles across and down (8x8 cells)
   uint8_t *lv, *lv0;            // tile levels (smoothed, raw)
   float *tsp;                   // tile max wave speed at the cycle start
+  unsigned *speedSlots;         // kSpeedSlots block maxima of the tile speeds (float bits), zero between cycles
   const int32_t *tileOpen;      // open cells per tile
   const float *tileRf;          // rain multiplier summed over the open cells of a tile (rain field only)
   float *sh, *shu, *shv, *sw;   // flux a waiting cell has gathered (Kahan residual sw)

@@ -688,11 +688,12 @@ __device__ __forceinline__ bool strip_begin(const DevInst &I, const int4 wk, int
 // ------------------------------------------------------------ LTS record work
 // A tile takes part in the record of level m when it steps at m, or waits at m + 1 beside a face
 // neighbour at m (the unit rule of sw_lts_units).
+template <bool WAITING_ONLY = false>
 __device__ __forceinline__ bool tile_relevant(const DevInst &I, int a, int b, int m) {
   if (a < 0 || a >= I.tx || b < 0 || b >= I.ty) return false;
   const uint8_t *lv = I.lv;
   const int l = lv[b * I.tx + a];
-  if (l == m) return true;
+  if (l == m) return !WAITING_ONLY;
   if (l != m + 1) return false;
   return (a > 0 && lv[b * I.tx + a - 1] == m) || (a + 1 < I.tx && lv[b * I.tx + a + 1] == m) ||
          (b > 0 && lv[(b - 1) * I.tx + a] == m) || (b + 1 < I.ty && lv[(b + 1) * I.tx + a] == m);
@@ -710,7 +711,9 @@ __device__ __forceinline__ int lts_split(const DevInst &I, int m) {
 }
 
 // The rows one wavefront of a level-m record walks: part `sub` of `split` parts of the unit, cut
-// to the span from its first to its last tile row holding a tile that takes part (w = 0: none)
+// to the span from its first to its last tile row holding a tile that takes part (w = 0: none);
+// WAITING_ONLY: a tile that waits and gathers (the fold of the record's gathers)
+template <bool WAITING_ONLY = false>
 __device__ __forceinline__ int4 lts_rows(const DevInst &I, int4 wk, int m, int lane, int split, int sub) {
   const int tx0 = wk.y / kLtsTile, ty0 = wk.z / kLtsTile;
   const int ntx = min(I.own / kLtsTile, I.tx - tx0);
@@ -718,7 +721,7 @@ __device__ __forceinline__ int4 lts_rows(const DevInst &I, int4 wk, int m, int l
   const int per = ((wk.w + kLtsTile - 1) / kLtsTile + split - 1) / split;  // tile rows per part
   const int r0 = sub * per, r1 = min(nty, r0 + per);
   const int c = lane & 7, r = r0 + (lane >> 3);
-  const bool rel = c < ntx && r < r1 && tile_relevant(I, tx0 + c, ty0 + r, m);
+  const bool rel = c < ntx && r < r1 && tile_relevant<WAITING_ONLY>(I, tx0 + c, ty0 + r, m);
   const uint64_t bits = __ballot(rel);
   int first = -1, last = -1;
   for (int q = 0; q < 8; ++q)
@@ -821,7 +824,7 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
   const int nu = I.wend - I.wbeg;
   const bool two = I.secondOrder;
   if (i >= 0) {
-    const int4 wk = lts_rows(I, work[I.wbeg + I.ulist[m * nu + i / split]], m, lane, split, i % split);
+    const int4 wk = lts_rows<true>(I, work[I.wbeg + I.ulist[m * nu + i / split]], m, lane, split, i % split);
     const int X = wk.y - I.hl + lane;
     if (lane < I.hl || lane >= I.hl + I.own || X >= I.nx) return;
     for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
@@ -845,20 +848,31 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
 // (_cycle, solver-cpu.js:504-527; ltsCycle/ltsLevel/ltsSmooth/ltsList, kernels.wgsl.js:677-741)
 
 // max wave speed per 8x8 tile at the cycle start: one wavefront per tile
+// (and the block's maximum into one of kSpeedSlots slots by atomic max: the cycle kernel reduces
+// the slots; a max does not depend on the order of the atomics, and speeds are non-negative floats,
+// which order as their bit patterns do)
 __global__ __launch_bounds__(256) void sw_lts_speed(const DevInst *__restrict__ insts) {
   const DevInst &I = insts[0];
   const int t = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (t >= I.tx * I.ty) return;
   const int lane = threadIdx.x & 63;
-  const int x = (t % I.tx) * kLtsTile + (lane & 7), y = (t / I.tx) * kLtsTile + (lane >> 3);
   float s = 0.f;
-  if (x < I.nx && y < I.ny) {
-    const size_t k = (size_t)y * I.pitch + x;
-    const float h = I.h[k];
-    if ((I.mask[k] & MASK_OPEN) && h > I.hdry) s = cell_speed(h, I.hu[k], I.hv[k], I.g);
+  if (t < I.tx * I.ty) {
+    const int x = (t % I.tx) * kLtsTile + (lane & 7), y = (t / I.tx) * kLtsTile + (lane >> 3);
+    if (x < I.nx && y < I.ny) {
+      const size_t k = (size_t)y * I.pitch + x;
+      const float h = I.h[k];
+      if ((I.mask[k] & MASK_OPEN) && h > I.hdry) s = cell_speed(h, I.hu[k], I.hv[k], I.g);
+    }
+    s = wave_max(s);
+    if (lane == 0) I.tsp[t] = s;
   }
-  s = wave_max(s);
-  if (lane == 0) I.tsp[t] = s;
+  __shared__ float s_m[4];
+  if (lane == 0) s_m[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float b = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
+    if (b > 0.f) atomicMax(&I.speedSlots[blockIdx.x % kSpeedSlots], __float_as_uint(b));
+  }
 }
 
 // cycle length D = min(2^L 0.9 cfl dx / smax (dtDry when dry), tEnd - t), base step D / 2^L
@@ -867,8 +881,11 @@ __global__ __launch_bounds__(1024) void sw_lts_cycle(const DevInst *__restrict__
   StepRec *rec = I.rec;
   __shared__ float s_m[16];
   float m = 0.f;
-  const int nt = I.tx * I.ty;
-  for (int t = threadIdx.x; t < nt; t += blockDim.x) m = fmaxf(m, I.tsp[t]);
+  // the tile speeds' maximum from sw_lts_speed's slots, which are cleared for the next cycle
+  for (int t = threadIdx.x; t < kSpeedSlots; t += blockDim.x) {
+    m = fmaxf(m, __uint_as_float(I.speedSlots[t]));
+    I.speedSlots[t] = 0u;
+  }
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
   __syncthreads();

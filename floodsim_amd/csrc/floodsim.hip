@@ -182,11 +182,11 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
 int check_params(fs_pool *P, const fs_params *p) {
   if (!p) return fail(P, FS_E_ARG, "params is NULL");
   if (p->nx < 2 || p->ny < 2) return fail(P, FS_E_ARG, "nx and ny must be >= 2 (got %d x %d)", p->nx, p->ny);
-  // every plane is addressed with 32-bit byte offsets: 4-byte planes up to 2^30 cells, the LTS
-  // 16-byte reconstruction planes up to 2^28
-  if ((int64_t)p->nx * p->ny > ((int64_t)1 << (p->lts ? 28 : 30)))
-    return fail(P, FS_E_ARG, "grid too large: %d x %d (at most 2^30 cells, 2^28 with local time stepping)", p->nx,
-                p->ny);
+  // every plane is addressed with 32-bit byte offsets: 4-byte planes up to 2^30 cells, the 16-byte
+  // planes (LTS reconstructions, hazard products) up to 2^28
+  if ((int64_t)p->nx * p->ny > ((int64_t)1 << (p->lts || p->hazard ? 28 : 30)))
+    return fail(P, FS_E_ARG, "grid too large: %d x %d (at most 2^30 cells, 2^28 with local time stepping or "
+                "hazard products)", p->nx, p->ny);
   if (!(p->dx > 0)) return fail(P, FS_E_ARG, "dx must be > 0");
   if (!(p->cfl > 0) || !(p->hdry > 0) || !(p->dtDry > 0)) return fail(P, FS_E_ARG, "cfl, hdry, dtDry must be > 0");
   for (int k = 0; k < 4; ++k)

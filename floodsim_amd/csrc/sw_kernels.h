@@ -212,20 +212,20 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m)
 // case): row-based global loads, no ghost logic
 template <int STAGE, bool LTS>
 __device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, int m) {
-  const size_t row = (size_t)Y * I.pitch;
+  const int k = Y * I.pitch + X;
   Prim p;
   float hu, hv;
   p.lv = 0;
-  if constexpr (LTS) p.lv = gld(I.lv + (size_t)(Y >> 3) * I.tx, X >> 3);
+  if constexpr (LTS) p.lv = gld(I.lv, (Y >> 3) * I.tx + (X >> 3));
   bool useB = STAGE == 1;
   if constexpr (LTS) useB = STAGE == 1 && p.lv == m;
-  if (useB) { p.h = gld(I.h1 + row, X); hu = gld(I.hu1 + row, X); hv = gld(I.hv1 + row, X); }
-  else { p.h = gld(I.h + row, X); hu = gld(I.hu + row, X); hv = gld(I.hv + row, X); }
+  if (useB) { p.h = gld(I.h1, k); hu = gld(I.hu1, k); hv = gld(I.hv1, k); }
+  else { p.h = gld(I.h, k); hu = gld(I.hu, k); hv = gld(I.hv, k); }
   p.hu = hu;
   p.hv = hv;
-  p.z = gld(I.z + row, X);
+  p.z = gld(I.z, k);
   p.in = true;
-  p.M = (gld(I.mask + row, X) & MASK_OPEN) != 0;
+  p.M = (gld(I.mask, k) & MASK_OPEN) != 0;
   const float ih = frcp(p.h);
   const bool wet = p.h > I.hdry;
   p.u = wet ? hu * ih : 0.f;
@@ -284,12 +284,12 @@ __device__ __noinline__ float cell_speed(float h, float hu, float hv, float g) {
 // Friction, rain, infiltration and max depth of one cell after its update, then the cell's wave
 // speed for the next dt (sourcesAt, kernels.wgsl.js:544-590; _sources, solver-cpu.js:456-483).
 // Semi-implicit Manning: (hu, hv) /= 1 + dt g n^2 |u| / h^(4/3).
-__device__ __forceinline__ void sources_cell(const DevInst &I, size_t row, int X, float dt, float rain, float areaf,
+__device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, float rain, float areaf,
                                              float &qh, float &qw, float &qhu, float &qhv, double &acc_inf,
                                              float &smax) {
   const float g = I.g, hdry = I.hdry;
   const float d = qh;
-  const float nn = gld(I.n + row, X);
+  const float nn = gld(I.n, k);
   if (d > hdry && nn > 0.f) {
     const float id = frcp(d);
     const float u = qhu * id, v = qhv * id;
@@ -299,18 +299,18 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, size_t row, int X
     qhu *= kf;
     qhv *= kf;
   }
-  const float rr = (I.flags & FLAG_RF) ? rain * gld(I.rf + row, X) * dt : rain * dt;
+  const float rr = (I.flags & FLAG_RF) ? rain * gld(I.rf, k) * dt : rain * dt;
   float inf = 0.f;
-  if (I.flags & FLAG_INFIL) inf = fminf(gld(I.infil + row, X) * dt, d + rr);
+  if (I.flags & FLAG_INFIL) inf = fminf(gld(I.infil, k) * dt, d + rr);
   float nh = qh, nw = qw;
   kadd(nh, nw, rr - inf);
   if (nh < 0.f) { inf = d - qw + rr; qh = 0.f; qw = 0.f; }
   else { qh = nh; qw = nw; }
   acc_inf += (double)inf * areaf;
   if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
-  const float hm = gld(I.hmax + row, X);
-  if (qh > hm) gst(I.hmax + row, X, qh);
-  if (qh > hdry && !(gld(I.mask + row, X) & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
+  const float hm = gld(I.hmax, k);
+  if (qh > hm) gst(I.hmax, k, qh);
+  if (qh > hdry && !(gld(I.mask, k) & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
 }
 
 // Hazard products of a cell after a step of length dt ending at tnow (hazard, hzRecord,
@@ -330,11 +330,11 @@ __device__ __forceinline__ float hz_arr(float d, float v) {
 }
 __device__ __forceinline__ float hz_hr(float hr) { return hr < 0.75f ? 1.f : hr < 1.25f ? 2.f : hr < 2.f ? 3.f : 4.f; }
 
-__device__ __noinline__ void hazard_cell(const DevInst &I, size_t row, int X, float d, float hu, float hv, float dt,
+__device__ __noinline__ void hazard_cell(const DevInst &I, int k, float d, float hu, float hv, float dt,
                                          float tnow) {
   if (!(d >= kHzWet)) return;
   const float v = d >= kHzVcut ? sqrtf(hu * hu + hv * hv) / d : 0.f;
-  float4 a = gld(I.hzA + row, X), b = gld(I.hzB + row, X);
+  float4 a = gld(I.hzA, k), b = gld(I.hzB, k);
   a.x = fmaxf(a.x, d * v);
   a.y = fmaxf(a.y, d * v * v + 0.5f * I.g * d * d);
   if (d > kHzThr) { a.z += dt; if (a.w < 0.f) a.w = tnow; }
@@ -345,8 +345,8 @@ __device__ __noinline__ void hazard_cell(const DevInst &I, size_t row, int X, fl
   const int c1 = max((packed >> 3) & 7, (int)hz_hr(hr));
   const int c2 = max(packed >> 6, (int)hz_hr(hr + (d > 0.25f ? 1.f : 0.f)));
   b.z = (float)(c0 + 8 * c1 + 64 * c2);
-  gst(reinterpret_cast<fs_v4 *>(I.hzA + row), X, fs_v4{a.x, a.y, a.z, a.w});
-  gst(reinterpret_cast<fs_v4 *>(I.hzB + row), X, fs_v4{b.x, b.y, b.z, b.w});
+  gst(reinterpret_cast<fs_v4 *>(I.hzA), k, fs_v4{a.x, a.y, a.z, a.w});
+  gst(reinterpret_cast<fs_v4 *>(I.hzB), k, fs_v4{b.x, b.y, b.z, b.w});
 }
 
 // wavefront index for a device-side count of n work items: the blocks that have work are spread
@@ -384,6 +384,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   const int y0 = wk.z, y1 = min(wk.z + wk.w, ny);
   const bool last = STAGE == 1 || !so;   // the stage that ends the step (or the record)
   const bool xin = wk.y - I.hl >= 0 && wk.y - I.hl + kLanes <= nx;  // every lane an interior column
+  const bool xedge = wk.y - I.hl <= 0 || wk.y - I.hl + kLanes >= nx;  // a lane on the first or last column
 
   // writes go to B in stage 0 and to A in stage 1; Ws is the residual of the evaluated state
   const float *__restrict__ Ws = STAGE == 0 ? I.w : I.w1;
@@ -449,14 +450,15 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       const float onW = from_left(xf0.on);
       const bool row_in = own_col && p.in && p.M && r >= y0 && r < y1;
       if (last && row_in && p.lv == m + 1) {
-        float2 *b = reinterpret_cast<float2 *>(I.bsx + (size_t)r * pitch);
-        if (onW > 0.f) gst(b, 2 * X, make_float2(rx0.x, rx0.y));
-        if (xf0.on > 0.f) gst(b, 2 * X + 1, make_float2(rx0.z, rx0.w));
+        float2 *b = reinterpret_cast<float2 *>(I.bsx);
+        const int kb = 2 * (r * pitch + X);
+        if (onW > 0.f) gst(b, kb, make_float2(rx0.x, rx0.y));
+        if (xf0.on > 0.f) gst(b, kb + 1, make_float2(rx0.z, rx0.w));
       }
       // a stepping cell's bed slope from its face reconstructions (solver-cpu.js:363-368): a face with
       // a finer neighbour contributes the values of its last evaluation
       if (row_in && p.lv == m && (onW == 0.f || xf0.on == 0.f)) {
-        const float4 b = gld(I.bsx + (size_t)r * pitch, X);
+        const float4 b = gld(I.bsx, r * pitch + X);
         float4 xs = rx0;
         if (onW == 0.f) { xs.x = b.x; xs.y = b.y; }
         if (xf0.on == 0.f) { xs.z = b.z; xs.w = b.w; }
@@ -485,12 +487,13 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       const bool onN = min(lv1, p.lv) == m;
       const bool row_in = own_col && in1 && M1r && ry >= y0 && ry < y1;
       if (last && row_in && lv1 == m + 1) {
-        float2 *b = reinterpret_cast<float2 *>(I.bsy + (size_t)ry * pitch);
-        if (yf2.on > 0.f) gst(b, 2 * X, make_float2(ry1.x, ry1.y));
-        if (onN) gst(b, 2 * X + 1, make_float2(ry1.z, ry1.w));
+        float2 *b = reinterpret_cast<float2 *>(I.bsy);
+        const int kb = 2 * (ry * pitch + X);
+        if (yf2.on > 0.f) gst(b, kb, make_float2(ry1.x, ry1.y));
+        if (onN) gst(b, kb + 1, make_float2(ry1.z, ry1.w));
       }
       if (row_in && lv1 == m && (yf2.on == 0.f || !onN)) {
-        const float4 b = gld(I.bsy + (size_t)ry * pitch, X);
+        const float4 b = gld(I.bsy, ry * pitch + X);
         float4 ys = ry1;
         if (yf2.on == 0.f) { ys.x = b.x; ys.y = b.y; }
         if (!onN) { ys.z = b.z; ys.w = b.w; }
@@ -509,9 +512,9 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         take = take && (lv2 == m || lv2 == m + 1);
         // a waiting cell, or a stepping one in its first stage, holds its water plus its store
         if (take && (lv2 == m + 1 || STAGE == 0)) {
-          const size_t row2 = (size_t)(r - 2) * pitch;
-          float st = gld(I.sh + row2, X) - gld(I.sw + row2, X);
-          if (STAGE == 1 && gld(I.gflag + row2, X)) st += gld(I.g0h + row2, X);  // first-stage gather, not yet folded
+          const int k2 = (r - 2) * pitch + X;
+          float st = gld(I.sh, k2) - gld(I.sw, k2);
+          if (STAGE == 1 && gld(I.gflag, k2)) st += gld(I.g0h, k2);  // first-stage gather, not yet folded
           hv_ = fmaxf(0.f, h2r + st);
         }
       }
@@ -551,7 +554,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         upd = upd && (own || coarse);
       }
       if (upd) {
-        const size_t row = (size_t)ry * pitch;
+        const int k = ry * pitch + X;
         float bx = sx3, by = sy3;
         if constexpr (LTS) {
           if (!own) { bx = 0.f; by = 0.f; }   // a waiting cell gathers fluxes only
@@ -566,27 +569,27 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
           if (kw + ke + ks + kn > 0.f) {
             const float wd = so ? 0.5f * dt : dt;
             if (STAGE == 0) {
-              gst(I.g0h + row, X, wd * dh); gst(I.g0hu + row, X, wd * dhu); gst(I.g0hv + row, X, wd * dhv);
-              gst(I.gflag + row, X, (uint8_t)1);
+              gst(I.g0h, k, wd * dh); gst(I.g0hu, k, wd * dhu); gst(I.g0hv, k, wd * dhv);
+              gst(I.gflag, k, (uint8_t)1);
             } else {
-              gst(I.g1h + row, X, wd * dh); gst(I.g1hu + row, X, wd * dhu); gst(I.g1hv + row, X, wd * dhv);
+              gst(I.g1h, k, wd * dh); gst(I.g1hu, k, wd * dhu); gst(I.g1hv, k, wd * dhv);
             }
           }
         } else {
-          float qh = hh3, qhu = hu3, qhv = hv3, qw = gld(Ws + row, X);
+          float qh = hh3, qhu = hu3, qhv = hv3, qw = gld(Ws, k);
           // RK start state: U^n (the window's values in the first stage), plus (LTS) what finer
           // neighbours passed while the cell waited
           float h0 = 0.f, w0n = 0.f, hu0 = 0.f, hv0 = 0.f;
           bool spent = false;   // LTS: the cell's store held gathered flux (cleared once used)
           if (STAGE == 1 || LTS) {
             if (STAGE == 1) {
-              h0 = gld(I.h + row, X); w0n = gld(I.w + row, X); hu0 = gld(I.hu + row, X); hv0 = gld(I.hv + row, X);
+              h0 = gld(I.h, k); w0n = gld(I.w, k); hu0 = gld(I.hu, k); hv0 = gld(I.hv, k);
             } else {
               h0 = hh3; w0n = qw; hu0 = hu3; hv0 = hv3;
             }
             if constexpr (LTS) {
-              const float sh = gld(I.sh + row, X), sw = gld(I.sw + row, X);
-              const float shu = gld(I.shu + row, X), shv = gld(I.shv + row, X);
+              const float sh = gld(I.sh, k), sw = gld(I.sw, k);
+              const float shu = gld(I.shu, k), shv = gld(I.shv, k);
               hu0 += shu;
               hv0 += shv;
               kadd(h0, w0n, sh - sw);
@@ -610,7 +613,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
           if (qh < 0.f) { acc_clip += -(double)(qh - qw) * areaf; qh = 0.f; qw = 0.f; }
           if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
           // boundary outflow through the domain edge faces (edgeOutAt, kernels.wgsl.js:1020-1030)
-          {
+          // (only strips holding a domain edge column and the edge rows take this wave-uniform branch)
+          if (xedge || ry == 0 || ry == ny - 1) {
             double o = 0.0;
             if (X == 0) o -= (double)w0;
             if (X == nx - 1) o += (double)Fe0;
@@ -619,17 +623,17 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             acc_out += o * ((double)dx * (double)dt * (double)wgt);
           }
           if (STAGE == 1) {
-            sources_cell(I, row, X, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+            sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
             if constexpr (HZ) {
-              if (I.hzA) hazard_cell(I, row, X, qh, qhu, qhv, dt, tnow);
+              if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
             }
             if constexpr (LTS) {
               if (spent) {
-                gst(I.sh + row, X, 0.f); gst(I.shu + row, X, 0.f); gst(I.shv + row, X, 0.f); gst(I.sw + row, X, 0.f);
+                gst(I.sh, k, 0.f); gst(I.shu, k, 0.f); gst(I.shv, k, 0.f); gst(I.sw, k, 0.f);
               }
             }
           }
-          gst(Ho + row, X, qh); gst(HUo + row, X, qhu); gst(HVo + row, X, qhv); gst(Wo + row, X, qw);
+          gst(Ho, k, qh); gst(HUo, k, qhu); gst(HVo, k, qhv); gst(Wo, k, qw);
           stir = stir || qh != 0.f || qw != 0.f || qhu != 0.f || qhv != 0.f;
         }
       }
@@ -789,24 +793,30 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
   if (STAGE == 1 && I.wet && lane == 0) I.wet[u] = stir ? 1 : 0;
 }
 
-#ifndef FS_LTS_MIN_BLOCKS
-#define FS_LTS_MIN_BLOCKS 1
+// waves per SIMD the LTS stage kernels are built for (the second argument of __launch_bounds__ caps
+// the registers so that many fit): both stages at 4 (128 VGPRs; the second stage spills 16 bytes,
+// reloaded about once per row, and runs 1.15x faster than at 3 waves with 135 VGPRs)
+#ifndef FS_LTS_S1_WAVES
+#define FS_LTS_S1_WAVES 4
+#endif
+#ifndef FS_LTS_S2_WAVES
+#define FS_LTS_S2_WAVES 4
 #endif
 template <>
-__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<0, true, false>(
+__global__ __launch_bounds__(256, FS_LTS_S1_WAVES) void sw_stage<0, true, false>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
   (void)nwork;
   lts_stage_body<0, false>(insts, work, part, m);
 }
 template <>
-__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<1, true, false>(
+__global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, false>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
   (void)nwork;
   lts_stage_body<1, false>(insts, work, part, m);
 }
 // with the hazard products (fs_params.hazard)
 template <>
-__global__ __launch_bounds__(256, FS_LTS_MIN_BLOCKS) void sw_stage<1, true, true>(
+__global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, true>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
   (void)nwork;
   lts_stage_body<1, true>(insts, work, part, m);
@@ -827,19 +837,28 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
     const int4 wk = lts_rows<true>(I, work[I.wbeg + I.ulist[m * nu + i / split]], m, lane, split, i % split);
     const int X = wk.y - I.hl + lane;
     if (lane < I.hl || lane >= I.hl + I.own || X >= I.nx) return;
-    for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
-      const size_t k = (size_t)y * I.pitch + X;
-      if (!I.gflag[k]) continue;
-      float sh = I.sh[k], sw = I.sw[k];
-      kadd(sh, sw, I.g0h[k]);
-      float shu = I.shu[k] + I.g0hu[k], shv = I.shv[k] + I.g0hv[k];
-      if (two) {
-        kadd(sh, sw, I.g1h[k]);
-        shu += I.g1hu[k];
-        shv += I.g1hv[k];
+    // the gather flags of 8 rows (one tile row: parts are whole tile rows) are loaded together, so
+    // the walk waits for memory once per tile row rather than once per row
+    const int yend = min(wk.z + wk.w, I.ny);
+    for (int yb = wk.z; yb < yend; yb += kLtsTile) {
+      uint8_t f[kLtsTile];
+#pragma unroll
+      for (int q = 0; q < kLtsTile; ++q) f[q] = yb + q < yend ? gld(I.gflag, (yb + q) * I.pitch + X) : (uint8_t)0;
+#pragma unroll
+      for (int q = 0; q < kLtsTile; ++q) {
+        if (!f[q]) continue;
+        const int k = (yb + q) * I.pitch + X;
+        float sh = gld(I.sh, k), sw = gld(I.sw, k);
+        kadd(sh, sw, gld(I.g0h, k));
+        float shu = gld(I.shu, k) + gld(I.g0hu, k), shv = gld(I.shv, k) + gld(I.g0hv, k);
+        if (two) {
+          kadd(sh, sw, gld(I.g1h, k));
+          shu += gld(I.g1hu, k);
+          shv += gld(I.g1hv, k);
+        }
+        gst(I.sh, k, sh); gst(I.sw, k, sw); gst(I.shu, k, shu); gst(I.shv, k, shv);
+        gst(I.gflag, k, (uint8_t)0);
       }
-      I.sh[k] = sh; I.sw[k] = sw; I.shu[k] = shu; I.shv[k] = shv;
-      I.gflag[k] = 0;
     }
   }
 }
@@ -1032,6 +1051,8 @@ __global__ __launch_bounds__(1024) void sw_lts_record_end(const DevInst *__restr
   const int n = rec->lcount[m] * lts_split(I, m);   // one partial per wavefront of the record
   __shared__ double s_o[16], s_c[16], s_f[16];
   double o = 0.0, c = 0.0, f = 0.0;
+  // unrolled so that several partials are in flight per thread (the sums keep their order)
+#pragma unroll 4
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     const Part a = p1[k];
     o += a.out;
@@ -1102,16 +1123,16 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
   if (dt > 0.f && lane >= I.hl && lane < I.hl + I.own && X < I.nx) {
     const float rain = I.rec->rainRate, areaf = I.dx * I.dx;
     for (int y = wk.z; y < min(wk.z + wk.w, I.ny); ++y) {
-      const size_t row = (size_t)y * I.pitch;
-      if (!(gld(I.mask + row, X) & MASK_OPEN)) continue;
+      const int k = y * I.pitch + X;
+      if (!(gld(I.mask, k) & MASK_OPEN)) continue;
       if (LTS && tile_level(I, X, y) != m) continue;
-      float qh = gld(I.h1 + row, X), qw = gld(I.w1 + row, X), qhu = gld(I.hu1 + row, X), qhv = gld(I.hv1 + row, X);
-      sources_cell(I, row, X, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+      float qh = gld(I.h1, k), qw = gld(I.w1, k), qhu = gld(I.hu1, k), qhv = gld(I.hv1, k);
+      sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
       if constexpr (HZ) {
-        if (I.hzA) hazard_cell(I, row, X, qh, qhu, qhv, dt, tnow);
+        if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
       }
-      gst(I.h + row, X, qh); gst(I.w + row, X, qw); gst(I.hu + row, X, qhu); gst(I.hv + row, X, qhv);
-      if (LTS) { gst(I.sh + row, X, 0.f); gst(I.shu + row, X, 0.f); gst(I.shv + row, X, 0.f); gst(I.sw + row, X, 0.f); }
+      gst(I.h, k, qh); gst(I.w, k, qw); gst(I.hu, k, qhu); gst(I.hv, k, qhv);
+      if (LTS) { gst(I.sh, k, 0.f); gst(I.shu, k, 0.f); gst(I.shv, k, 0.f); gst(I.sw, k, 0.f); }
       stir = stir || qh != 0.f || qw != 0.f || qhu != 0.f || qhv != 0.f;
     }
   }

@@ -20,17 +20,19 @@
 namespace fs {
 
 // ---------------------------------------------------------------- lane shifts (DPP)
+// (bound_ctrl: the lane without a source, 0 or 63, reads 0, a halo lane whose results are unused;
+// the shift can then fold into the instruction that consumes it)
 __device__ __forceinline__ float from_left(float v) {   // lane l receives lane l-1
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float v) {  // lane l receives lane l+1
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ int from_left_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int from_right_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);
 }
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -276,9 +278,12 @@ __device__ __forceinline__ int xcd_wave_index(int nwaves_per_block) {
 }
 
 // |u| + |v| + 2 sqrt(g h): the sum of the directional wave speeds (computeDt, solver-cpu.js:383-394)
-__device__ __noinline__ float cell_speed(float h, float hu, float hv, float g) {
+// (contraction off: every kernel that inlines it computes the same bits, so a speed taken after the
+// host writes state equals the one the second stage takes)
+__device__ __forceinline__ float cell_speed(float h, float hu, float hv, float g) {
+#pragma clang fp contract(off)
   const float ih = frcp(h);
-  return fabsf(hu * ih) + fabsf(hv * ih) + 2.f * fsq(g * h);
+  return (fabsf(hu * ih) + fabsf(hv * ih)) + 2.f * fsq(g * h);
 }
 
 // Friction, rain, infiltration and max depth of one cell after its update, then the cell's wave

@@ -76,6 +76,8 @@ struct Group {
   std::vector<int> withInlets;           // group indices of instances with inlets
   bool anyFirstOrder = false, anySecondOrder = false;
   bool anyHazard = false;                // an instance tracks the hazard products
+  bool anyStepFlux = false;              // an instance takes the step-flux rules (F26)
+  unsigned kinds = 0;                    // second-stage kernel variants the group needs: bit 2*hazard + stepFlux
   bool lts = false;                      // a local-time-stepping instance (alone in its group)
   // captured K-step graph
   // captured K-step graphs, one per power-of-two K (index log2 K)
@@ -243,6 +245,8 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
   G.withInlets.clear();
   G.anyFirstOrder = G.anySecondOrder = false;
   G.anyHazard = false;
+  G.anyStepFlux = false;
+  G.kinds = 0;
   G.lts = false;
   std::vector<int4> work;
   for (size_t gi = 0; gi < slots.size(); ++gi) {
@@ -253,7 +257,10 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.z = S.z; d.n = S.n; d.mask = S.mask; d.hmax = S.hmax; d.infil = S.infil; d.rf = S.rf;
     d.rec = P->drec + slots[gi];
     d.nx = S.nx; d.ny = S.ny; d.pitch = S.pitch;
-    d.flags = (S.infil ? FLAG_INFIL : 0) | (S.rf ? FLAG_RF : 0);
+    d.flags = (S.infil ? FLAG_INFIL : 0) | (S.rf ? FLAG_RF : 0) | (S.p.implicitFriction ? FLAG_IMPF : 0) |
+              (S.p.stepFlux && !S.L ? FLAG_STEPF : 0);   // the step flux applies under global stepping only
+    if (d.flags & FLAG_STEPF) G.anyStepFlux = true;
+    G.kinds |= 1u << ((S.hzA ? 2 : 0) + ((d.flags & FLAG_STEPF) ? 1 : 0));
     d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;
     d.hfront = (float)S.p.hfront; d.cfl = (float)S.p.cfl; d.dtDry = (float)S.p.dtDry;
     d.area = (float)(S.p.dx * S.p.dx);
@@ -342,17 +349,22 @@ int launch_step(fs_pool *P) {
     if (!e0 || !e1 || !e2) return fail(P, FS_E_HIP, "hipEventCreate failed");
     HIPCHK(P, hipEventRecord(e0, P->stream));
   }
-  sw_stage<0, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
+  // one launch per kernel variant the group's instances need (each variant skips the other kinds)
+  const unsigned K = G.kinds;
+  if (K & 0x5u) sw_stage<0, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
+  if (K & 0xau) sw_stage<0, false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
   if (P->timing) HIPCHK(P, hipEventRecord(e1, P->stream));
   // each instance runs its own scheme: the kernel of the other order returns at once for it
   if (G.anySecondOrder) {
-    if (G.anyHazard) sw_stage<1, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    else sw_stage<1, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x1u) sw_stage<1, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x2u) sw_stage<1, false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x4u) sw_stage<1, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x8u) sw_stage<1, false, true, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
   }
   if (P->timing) P->spans.push_back({e1, nullptr, -1, -1});  // completed below with e2
   if (G.anyFirstOrder) {
-    if (G.anyHazard) sw_euler_finish<false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    else sw_euler_finish<false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x3u) sw_euler_finish<false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0xcu) sw_euler_finish<false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
   }
   if (P->timing) { HIPCHK(P, hipEventRecord(e2, P->stream)); P->spans.back().b = e2; }
   for (int gi : G.withInlets) {
@@ -732,7 +744,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   int open = 0;
   for (size_t k = 0; k < N; ++k) {
     const bool o = mask ? mask[k] != 0 : true;
-    S.hmask[k] = o ? MASK_OPEN : 0;
+    S.hmask[k] = (o ? MASK_OPEN : 0) | (tmp[k] > 0.f ? MASK_ROUGH : 0);   // tmp: the FP32 Manning n
     open += o;
   }
   S.inletCells.assign(inlets, inlets + nInlets);

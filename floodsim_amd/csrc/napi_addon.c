@@ -195,6 +195,8 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   p.rows = (int32_t)num_prop(env, a[0], "rows", 0);
   p.untiled = (int32_t)num_prop(env, a[0], "untiled", 0);
   p.hazard = (int32_t)num_prop(env, a[0], "hazard", 0);
+  p.implicitFriction = (int32_t)num_prop(env, a[0], "implicitFriction", 0);
+  p.stepFlux = (int32_t)num_prop(env, a[0], "stepFlux", 0);
   if (p.nx <= 0 || p.ny <= 0) {
     napi_throw_range_error(env, NULL, "create: nx and ny must be positive");
     return NULL;

@@ -39,8 +39,10 @@ __device__ __forceinline__ void gst(float2 *row, int x, float2 v) {
   const fs_v2 t = {v.x, v.y};
   *(FS_GLOBAL fs_v2 *)((FS_GLOBAL char *)row + (uint32_t)x * 8u) = t;
 }
-enum { MASK_OPEN = 1, MASK_INLET = 2 };      // mask byte bits: open (not a wall) cell, inlet cell
-enum { FLAG_INFIL = 1, FLAG_RF = 2 };
+// mask byte bits: open (not a wall) cell, inlet cell, Manning n > 0 (the step-flux rules apply)
+enum { MASK_OPEN = 1, MASK_INLET = 2, MASK_ROUGH = 4 };
+// DevInst flags: infiltration layer, rain field, implicit friction and step flux (F26)
+enum { FLAG_INFIL = 1, FLAG_RF = 2, FLAG_IMPF = 4, FLAG_STEPF = 8 };
 
 // Per-instance mutable record: clock, dt, FP64 books. Written by dt_final only (and the host
 // between launches); the stage kernels read dt and the rain rate from it.

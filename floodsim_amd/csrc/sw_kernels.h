@@ -91,8 +91,13 @@ __device__ __forceinline__ void hllc(float hL, float uL, float vL, float hR, flo
 // difference delta enters: h*_L = max(0, hL - max(0, delta)), h*_R = max(0, hR - max(0, -delta)).
 struct Face { float F0, F1, F2, cL, cR; };
 
+// SF: the step-flux rule (F26, solver-cpu.js:250-273; kernels.wgsl.js:247-257), for faces between two
+// open cells with friction (RL, RR: Manning n > 0): a wet film whose reconstructed depth over the other
+// side's higher face bed is zero passes its own discharge upwind, and the lower cell feels the whole step
+template <bool SF = false>
 __device__ __forceinline__ Face face(float hL, float eL, float uL, float vL, bool ML, float hR, float eR, float uR,
-                                     float vR, bool MR, float dz, float g, float hdry) {
+                                     float vR, bool MR, float dz, float g, float hdry, bool RL = false,
+                                     bool RR = false) {
   if (!MR) { hR = hL; eR = eL; uR = -uL; vR = vL; }
   if (!ML) { hL = hR; eL = eR; uL = -uR; vL = vR; }
   const float delta = (ML && MR) ? (dz + (eR - hR)) - (eL - hL) : 0.f;
@@ -103,6 +108,20 @@ __device__ __forceinline__ Face face(float hL, float eL, float uL, float vL, boo
   const float hg = 0.5f * g;
   f.cL = ML ? hg * (hL * hL - hLs * hLs) : 0.f;
   f.cR = MR ? hg * (hR * hR - hRs * hRs) : 0.f;
+  if constexpr (SF) {
+    if (ML && MR && RL && RR) {
+      // the face beds differ by delta; the surface of the lower side lies max(0, -+delta) - h under the step
+      if (hLs > hdry && hRs <= 0.f && hR > hdry) {
+        const float u = fmaxf(uL, 0.f), q = hLs * u;
+        f.F0 = q; f.F1 = q * u + hg * hLs * hLs; f.F2 = q * vL;
+        f.cR = hg * hR * hR + g * hR * (fmaxf(0.f, -delta) - hR);
+      } else if (hRs > hdry && hLs <= 0.f && hL > hdry) {
+        const float u = fminf(uR, 0.f), q = hRs * u;
+        f.F0 = q; f.F1 = q * u + hg * hRs * hRs; f.F2 = q * vR;
+        f.cL = hg * hL * hL + g * hL * (fmaxf(0.f, delta) - hL);
+      }
+    }
+  }
   if (!ML && !MR) { f.F0 = f.F1 = f.F2 = f.cL = f.cR = 0.f; }
   return f;
 }
@@ -132,7 +151,20 @@ __device__ __forceinline__ float out_vel(float he, float ze, float ne, float hi,
 // and outflow edges copy the edge cell, outflow sets the outward Manning velocity. Ghost beds
 // are the nearest interior bed (zero gradient), ghost cells are open. Corner ghosts never
 // enter an interior cell's stencil and only need to be finite.
-struct Prim { float h, u, v, z, hu, hv; bool M, in; int lv; };
+struct Prim { float h, u, v, z, hu, hv; bool M, in, R; int lv; };   // R: Manning n > 0
+
+// F26 (implicitFriction): the second stage's fluxes see the predictor's velocity after friction implicit
+// in the new speed over the step, k = (1 + sqrt(1 + 4 a |u|)) / 2 (solver-cpu.js:419-423, 443-452;
+// kernels.wgsl.js:139-143); the update itself keeps U*
+__device__ __forceinline__ void pre_friction(const DevInst &I, int k, float h, float dt, float &u, float &v) {
+  const float nn = gld(I.n, k);
+  if (h > I.hdry && nn > 0.f) {
+    const float a = dt * I.g * nn * nn * frcp(fpow(h, 4.f / 3.f));
+    const float ik = frcp(0.5f * (1.f + fsq(1.f + 4.f * a * fsq(u * u + v * v))));
+    u *= ik;
+    v *= ik;
+  }
+}
 
 // level of the LTS tile holding interior cell (x, y)
 __device__ __forceinline__ int tile_level(const DevInst &I, int x, int y) {
@@ -151,7 +183,7 @@ __device__ __forceinline__ void cell_state(const DevInst &I, int x, int y, int m
 }
 
 template <int STAGE, bool LTS>
-__device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m) {
+__device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m, float dt) {
   const int nx = I.nx, ny = I.ny;
   int sx = X, sy = Y;
   float fu = 1.f, fv = 1.f;
@@ -167,24 +199,31 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m)
   Prim p;
   float hu, hv;
   p.lv = 0;
+  bool fromB = STAGE == 1;   // the state is U* (the second stage's evaluation point)
   if constexpr (LTS) {
     p.lv = tile_level(I, zx, zy);  // a ghost takes its edge cell's level (solver-cpu.js:125-128)
     // the state comes from the source cell (a mirrored ghost's may sit in another tile)
     const int ls = (gx || gy) ? tile_level(I, sx, sy) : p.lv;
-    if (STAGE == 1 && ls == m) { p.h = I.h1[k]; hu = I.hu1[k]; hv = I.hv1[k]; }
+    fromB = STAGE == 1 && ls == m;
+    if (fromB) { p.h = I.h1[k]; hu = I.hu1[k]; hv = I.hv1[k]; }
     else { p.h = I.h[k]; hu = I.hu[k]; hv = I.hv[k]; }
   } else {
     cell_state<STAGE, false>(I, sx, sy, m, p.h, hu, hv);
   }
   p.hu = hu * fu;
   p.hv = hv * fv;
-  p.z = I.z[(size_t)zy * I.pitch + zx];
+  const size_t kz = (size_t)zy * I.pitch + zx;   // ghosts take the edge cell's bed and roughness
+  p.z = I.z[kz];
   p.in = !gx && !gy;
-  p.M = p.in ? (I.mask[k] & MASK_OPEN) != 0 : true;
+  const bool srcOpen = (I.mask[k] & MASK_OPEN) != 0;
+  p.M = p.in ? srcOpen : true;
+  p.R = (I.mask[kz] & MASK_ROUGH) != 0;
   const float ih = frcp(p.h);
   const bool wet = p.h > I.hdry;
   p.u = wet ? p.hu * ih : 0.f;
   p.v = wet ? p.hv * ih : 0.f;
+  // a ghost copies its source cell after the source's friction
+  if (STAGE == 1 && (I.flags & FLAG_IMPF) && fromB && srcOpen) pre_friction(I, (int)k, p.h, dt, p.u, p.v);
   if (gx != gy) {  // edge ghost (not a corner): free outfall
     int e = -1, i = -1, side = -1;
     if (X < 0 && I.bc[0] == BC_OUTFLOW) { e = 0; i = 1; side = 0; }
@@ -213,7 +252,7 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m)
 // Primitive of interior cell (X, Y) for a row whose 64 lanes are all interior cells (the common
 // case): row-based global loads, no ghost logic
 template <int STAGE, bool LTS>
-__device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, int m) {
+__device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, int m, float dt) {
   const int k = Y * I.pitch + X;
   Prim p;
   float hu, hv;
@@ -227,11 +266,14 @@ __device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, in
   p.hv = hv;
   p.z = gld(I.z, k);
   p.in = true;
-  p.M = (gld(I.mask, k) & MASK_OPEN) != 0;
+  const uint8_t mk = gld(I.mask, k);
+  p.M = (mk & MASK_OPEN) != 0;
+  p.R = (mk & MASK_ROUGH) != 0;
   const float ih = frcp(p.h);
   const bool wet = p.h > I.hdry;
   p.u = wet ? hu * ih : 0.f;
   p.v = wet ? hv * ih : 0.f;
+  if (STAGE == 1 && (I.flags & FLAG_IMPF) && useB && p.M) pre_friction(I, k, p.h, dt, p.u, p.v);
   return p;
 }
 
@@ -240,11 +282,15 @@ __device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, in
 // wall neighbours are left out of the slope. dza = z_c - z_a, dzb = z_b - z_c.
 struct Slopes { float sh, se, sn, st; };
 
+// SF: a film with friction (Rc) thinner than half the bed's change across the cell is a cascade and
+// stays first order (F26, solver-cpu.js:222-224; kernels.wgsl.js:166-170)
+template <bool SF = false>
 __device__ __forceinline__ Slopes muscl(float hc, float nc, float tc, float ha, float na, float ta, bool Ma,
                                         float dza, float hb, float nb, float tb, bool Mb, float dzb, int secondOrder,
-                                        float hfront, int vanleer) {
+                                        float hfront, int vanleer, bool Rc = false) {
   Slopes s = {0.f, 0.f, 0.f, 0.f};
-  const bool fo = !secondOrder || hc < hfront || (Ma && ha < hfront) || (Mb && hb < hfront);
+  bool fo = !secondOrder || hc < hfront || (Ma && ha < hfront) || (Mb && hb < hfront);
+  if constexpr (SF) fo = fo || (Rc && hc < 0.5f * fabsf((Mb ? dzb : 0.f) + (Ma ? dza : 0.f)));
   if (fo) return s;
   const float dhm = Ma ? hc - ha : 0.f, dhp = Mb ? hb - hc : 0.f;
   s.sh = limit(dhm, dhp, vanleer);
@@ -300,7 +346,8 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, 
     const float u = qhu * id, v = qhv * id;
     const float af = dt * g * nn * nn * frcp(fpow(d, 4.f / 3.f));
     const float sp = fsq(u * u + v * v);
-    const float kf = frcp(1.f + af * sp);
+    // F26 (implicitFriction): implicit in the new speed, k = (1 + sqrt(1 + 4 a |u|)) / 2 (solver-cpu.js:465-467)
+    const float kf = (I.flags & FLAG_IMPF) ? frcp(0.5f * (1.f + fsq(1.f + 4.f * af * sp))) : frcp(1.f + af * sp);
     qhu *= kf;
     qhv *= kf;
   }
@@ -377,9 +424,12 @@ struct FaceRow { float F0, F1, F2, cL, cR, on; };
 // store; cells at m + 1 wait and gather the RK-weighted flux of those faces; the bed-slope term of a
 // stepping cell takes, for a face whose neighbour is finer, its reconstruction from that face's last
 // evaluation (kept in bsx/bsy by the finer record's last stage).
-template <int STAGE, bool LTS, bool HZ>
+// SF: the step-flux rules (F26) for instances that take them (global stepping only)
+template <int STAGE, bool LTS, bool HZ, bool SF = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
                                           Part *__restrict__ part, const int cells, const float tnow) {
+  static_assert(!(SF && LTS), "the step flux applies under global stepping only");
+  const bool sf = SF && (I.flags & FLAG_STEPF);
   const float rain = I.rec->rainRate;
   const float g = I.g, hdry = I.hdry, hfront = I.hfront, inv_dx = I.inv_dx, dx = I.dx;
   const int so = I.secondOrder, vl = I.limiter;
@@ -405,6 +455,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   // ---- register window
   float h1r = 0, u1r = 0, v1r = 0, z1r = 0, h2r = 0, u2r = 0, v2r = 0, z2r = 0;
   bool M1r = false, M2r = false;
+  bool R1r = false, R2r = false;   // SF: Manning n > 0 in rows r-1, r-2
   int lv1 = 0, lv2 = 0, lv3 = 0;
   float hu1 = 0, hv1 = 0, hu2 = 0, hv2 = 0, hu3 = 0, hv3 = 0, hh3 = 0;
   bool M3 = false, in1 = false, in2 = false, in3 = false;
@@ -423,14 +474,16 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   for (int r = y0 - kHalo; r < y1 + kHalo; ++r) {
     // ---------------- row r: primitives
     Prim p;
-    if (xin && r >= 0 && r < ny) p = load_prim_row<STAGE, LTS>(I, X, r, m);
-    else p = load_prim<STAGE, LTS>(I, X, r, m);
+    if (xin && r >= 0 && r < ny) p = load_prim_row<STAGE, LTS>(I, X, r, m, dt);
+    else p = load_prim<STAGE, LTS>(I, X, r, m, dt);
 
     // ---------------- x reconstruction of row r (neighbours by lane shifts)
     const float hl = from_left(p.h), ul = from_left(p.u), vlf = from_left(p.v), zl = from_left(p.z);
     const float hr = from_right(p.h), ur = from_right(p.u), vr = from_right(p.v), zr = from_right(p.z);
     const bool Ml = from_left_i(p.M) != 0, Mr = from_right_i(p.M) != 0;
-    const Slopes sxs = muscl(p.h, p.u, p.v, hl, ul, vlf, Ml, p.z - zl, hr, ur, vr, Mr, zr - p.z, so, hfront, vl);
+    const bool Rr = SF ? from_right_i(p.R) != 0 : false;
+    const Slopes sxs = muscl<SF>(p.h, p.u, p.v, hl, ul, vlf, Ml, p.z - zl, hr, ur, vr, Mr, zr - p.z, so, hfront, vl,
+                                 sf && p.R);
     const float xh_p = p.h + 0.5f * sxs.sh, xe_p = p.h + 0.5f * sxs.se;
     const float xh_m = p.h - 0.5f * sxs.sh, xe_m = p.h - 0.5f * sxs.se;
     const float xu_p = p.u + 0.5f * sxs.sn, xv_p = p.v + 0.5f * sxs.st;
@@ -442,7 +495,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     int lvr = 0;
     {
       const float Rh = from_right(xh_m), Re = from_right(xe_m), Ru = from_right(xu_m), Rv = from_right(xv_m);
-      const Face f = face(xh_p, xe_p, xu_p, xv_p, p.M, Rh, Re, Ru, Rv, Mr, zr - p.z, g, hdry);
+      const Face f = face<SF>(xh_p, xe_p, xu_p, xv_p, p.M, Rh, Re, Ru, Rv, Mr, zr - p.z, g, hdry, sf && p.R, Rr);
       xf0 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
       if constexpr (LTS) {
         lvr = from_right_i(p.lv);
@@ -472,7 +525,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     }
 
     // ---------------- y reconstruction of row r-1 (neighbours: rows r-2 and r in registers)
-    const Slopes sys = muscl(h1r, v1r, u1r, h2r, v2r, u2r, M2r, z1r - z2r, p.h, p.v, p.u, p.M, p.z - z1r, so, hfront, vl);
+    const Slopes sys = muscl<SF>(h1r, v1r, u1r, h2r, v2r, u2r, M2r, z1r - z2r, p.h, p.v, p.u, p.M, p.z - z1r, so, hfront,
+                                 vl, sf && R1r);
     const float yh_p = h1r + 0.5f * sys.sh, ye_p = h1r + 0.5f * sys.se;
     const float yh_m = h1r - 0.5f * sys.sh, ye_m = h1r - 0.5f * sys.se;
     const float yn_p = v1r + 0.5f * sys.sn, yt_p = u1r + 0.5f * sys.st;
@@ -482,7 +536,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     // y face (r-2 | r-1): L = row r-2 toward north, R = row r-1 toward south
     FaceRow yf2;
     {
-      const Face f = face(yp_h, yp_e, yp_n, yp_t, M2r, yh_m, ye_m, yn_m, yt_m, M1r, z1r - z2r, g, hdry);
+      const Face f = face<SF>(yp_h, yp_e, yp_n, yp_t, M2r, yh_m, ye_m, yn_m, yt_m, M1r, z1r - z2r, g, hdry, sf && R2r,
+                              R1r);
       yf2 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
       if constexpr (LTS) yf2.on = min(lv2, lv1) == m ? 1.f : 0.f;
     }
@@ -655,6 +710,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     hh3 = h2r; hu3 = hu2; hv3 = hv2; M3 = M2r; in3 = in2;
     h2r = h1r; u2r = u1r; v2r = v1r; z2r = z1r; M2r = M1r; hu2 = hu1; hv2 = hv1; in2 = in1;
     h1r = p.h; u1r = p.u; v1r = p.v; z1r = p.z; M1r = p.M; hu1 = p.hu; hv1 = p.hv; in1 = p.in;
+    if constexpr (SF) { R2r = R1r; R1r = p.R; }
   }
 
   const double o = wave_sum(acc_out), c = wave_sum(acc_clip), f = wave_sum(acc_inf);
@@ -775,8 +831,11 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
 }
 
 // One RK stage over a work list. Global stepping: one wavefront per strip of `work`; a strip the
-// dry-strip rule leaves out writes an empty partial.
-template <int STAGE, bool LTS, bool HZ = false>
+// dry-strip rule leaves out writes an empty partial. Each kernel variant advances only the instances
+// of its kind (hazard products: second stage only; step flux) and leaves the others' strips alone, so a
+// group of mixed kinds runs one launch per kind present and every instance gets the same machine code,
+// hence the same bits, alone and in any mix.
+template <int STAGE, bool LTS, bool HZ = false, bool SF = false>
 __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                 int nwork, Part *__restrict__ part, int m) {
   static_assert(!LTS, "local time stepping records use the specialisations below");
@@ -785,6 +844,8 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
   if (wid >= nwork) return;
   const int4 wk = work[wid];
   const DevInst &I = insts[wk.x];
+  if (((I.flags & FLAG_STEPF) != 0) != SF) return;
+  if (STAGE == 1 && (I.hzA != nullptr) != HZ) return;
   if (STAGE == 1 && !I.secondOrder) return;  // first order: sw_euler_finish writes this strip's partial
   const float dt = I.rec->dt;
   const int u = wid - I.wbeg;
@@ -794,7 +855,7 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
     return;
   }
-  const bool stir = run_strip<STAGE, false, HZ>(I, wk, lane, 0, dt, part + wid, I.sopen[u], (float)I.rec->tAfter);
+  const bool stir = run_strip<STAGE, false, HZ, SF>(I, wk, lane, 0, dt, part + wid, I.sopen[u], (float)I.rec->tAfter);
   if (STAGE == 1 && I.wet && lane == 0) I.wet[u] = stir ? 1 : 0;
 }
 
@@ -1158,6 +1219,7 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
     const int4 wk = work[wid];
     const DevInst &I = insts[wk.x];
     if (I.secondOrder) return;  // second order: sw_stage<1> writes this strip's partial
+    if ((I.hzA != nullptr) != HZ) return;  // the other variant advances it (same code alone and mixed)
     const int u = wid - I.wbeg;
     if (I.wet && !I.act[u]) {
       if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};

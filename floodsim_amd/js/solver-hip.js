@@ -24,7 +24,8 @@
  *
  * The device argument is accepted for signature compatibility and ignored: the library drives
  * HIP device 0 of the process. Options of the reference outside the shallow-water step
- * (subgrid, sink, gi, barriers, sediment, debris, implicitFriction, stepFlux, stageCells) raise.
+ * (subgrid, sink, gi, barriers, sediment, debris, stageCells) raise. implicitFriction and stepFlux
+ * (F26) are taken as the reference takes them (stepFlux has no effect under local time stepping).
  *
  * Calls on one solver run one at a time: asynchronous calls queue behind each other, and the
  * synchronous setters wait for the queue when it is busy, so they land between the calls they
@@ -36,8 +37,7 @@ const addon = require(path.join(__dirname, 'floodsim_napi.node'));
 
 const GH = 2; // the reference's ghost width, kept for idx(i, j) and zh
 const BC = { wall: 0, open: 1, outflow: 2 };
-const UNSUPPORTED = ['subgrid', 'sink', 'gi', 'barriers', 'sediment', 'debris', 'implicitFriction', 'stepFlux',
-  'stageCells', 'stage'];
+const UNSUPPORTED = ['subgrid', 'sink', 'gi', 'barriers', 'sediment', 'debris', 'stageCells', 'stage'];
 
 function def(v, d) { return v === undefined || v === null ? d : v; }
 
@@ -68,6 +68,8 @@ class GPUSolver {
     this.tiled = o.tiles !== false;
     // hazard products every step, as the reference always tracks them; hazard: false leaves them out
     this.hazard = o.hazard !== false;
+    // F26 options, default off (solver-gpu.js:43)
+    this.implicitFriction = !!o.implicitFriction; this.stepFlux = !!o.stepFlux;
     const n = o.nx * o.ny;
     const z = f64(o.z, n, 'z');
     let zmin = Infinity;
@@ -83,6 +85,7 @@ class GPUSolver {
       dtDry: this.dtDry, secondOrder: this.secondOrder ? 1 : 0, limiter: this.limiter,
       bcWest: BC[this.bc.west], bcEast: BC[this.bc.east], bcSouth: BC[this.bc.south], bcNorth: BC[this.bc.north],
       lts: this.lts, rows: o.rows | 0, untiled: this.tiled ? 0 : 1, hazard: this.hazard ? 1 : 0,
+      implicitFriction: this.implicitFriction ? 1 : 0, stepFlux: this.stepFlux ? 1 : 0,
     };
     for (const side of ['west', 'east', 'south', 'north']) {
       if (BC[this.bc[side]] === undefined) throw new Error('GPUSolver (HIP): bc.' + side + ' = ' + this.bc[side]);

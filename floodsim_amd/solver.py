@@ -34,6 +34,7 @@ class fs_params(ctypes.Structure):
         ("secondOrder", ctypes.c_int32), ("limiter", ctypes.c_int32),
         ("bc", ctypes.c_int32 * 4), ("lts", ctypes.c_int32), ("rows", ctypes.c_int32),
         ("untiled", ctypes.c_int32), ("hazard", ctypes.c_int32),
+        ("implicitFriction", ctypes.c_int32), ("stepFlux", ctypes.c_int32),
     ]
 
 
@@ -138,7 +139,8 @@ def _f64(a, n):
 
 
 def make_params(nx, ny, dx, bc=None, secondOrder=True, limiter="minmod", cfl=None, hdry=1e-5, hfront=1e-3,
-                dtDry=1.0, g=9.81, lts=0, rows=0, tiles=True, hazard=False, **_):
+                dtDry=1.0, g=9.81, lts=0, rows=0, tiles=True, hazard=False, implicitFriction=False,
+                stepFlux=False, **_):
     """fs_params with the defaults of GPUSolver (solver-gpu.js:39-47)."""
     p = fs_params()
     p.nx, p.ny, p.dx, p.g = int(nx), int(ny), float(dx), float(g)
@@ -156,6 +158,8 @@ def make_params(nx, ny, dx, bc=None, secondOrder=True, limiter="minmod", cfl=Non
     p.rows = int(rows or 0)
     p.untiled = 0 if tiles else 1   # tiles=False: every strip every step (the reference's o.tiles)
     p.hazard = 1 if hazard else 0   # hazard products every step (readHazard)
+    p.implicitFriction = 1 if implicitFriction else 0   # F26 (solver-gpu.js:43)
+    p.stepFlux = 1 if stepFlux else 0
     return p
 
 

@@ -50,6 +50,10 @@ typedef struct fs_params {
                           quiet dry strips (the reference's active tiles, kernels.wgsl.js:639-673) */
   int32_t hazard;      /* 1: track the hazard products every step (hazard/hzRecord,
                           kernels.wgsl.js:415-447, which the reference always runs) */
+  int32_t implicitFriction; /* F26 (solver-cpu.js:45, 62; solver-gpu.js:43): Manning friction implicit in the
+                          new speed, and the second stage's fluxes see the predictor after friction */
+  int32_t stepFlux;    /* F26 (solver-cpu.js:47, 63): a film stepping down a bed step taller than itself
+                          passes its own discharge; no effect with local time stepping, as in the reference */
 } fs_params;
 
 /* clock fields of GPUSolver: t, steps, lastDt, dtAvg (solver-gpu.js:578-586) */

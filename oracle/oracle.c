@@ -19,8 +19,10 @@
  *   books             volume, massBalance              solver-cpu.js:156-162, 564-568
  *   inlet exchange    CpuSim.advance                   benchmarks/physics/street/exec-cpu.mjs:46-55
  *                     (per LTS record as the GPU's exchangeL, kernels.wgsl.js:596)
- * Out of scope here (as in the product): flood-barrier weirs, the stepFlux and
- * implicitFriction options (F26), Green-Ampt and GI stores, hazard products.
+ *   F26 options       implicitFriction (_friction, the implicit factor of _sources) and stepFlux
+ *                     (the cascade clause of _recon, _face's step flux, _stepFlux)
+ *                                                      solver-cpu.js:222-225, 250-273, 419-423, 443-452, 465-467, 542-545
+ * Out of scope here (as in the product): flood-barrier weirs, Green-Ampt and GI stores.
  *
  * Loops over rows may run on several OpenMP threads. Every per-cell update is
  * independent; sums (rain, infiltration, clipping) are taken per row and then
@@ -46,6 +48,8 @@ typedef struct {
   int secondOrder, limiter; /* limiter 0 minmod, 1 van Leer */
   int bc[4];                /* west, east, south, north */
   int lts;                  /* 0 = global stepping, else levels 1..4 */
+  int implicitFriction;     /* F26: friction implicit in the new speed, and on the predictor the second stage sees */
+  int stepFlux;             /* F26: a film stepping down a bed step taller than itself passes its own discharge */
 } or_params;
 
 typedef struct {
@@ -55,6 +59,7 @@ typedef struct {
   uint8_t *M;
   double rainRate;
   double *Hp, *Ep, *Up, *Vp, *h0, *hu0, *hv0, *dh, *dhu, *dhv, *FX, *FY, *maxDepth, *TH;
+  double *keepU, *keepV; /* implicitFriction: the predictor's momentum, restored after the second L */
   double *rowA, *rowB; /* per-row partial sums */
   double t;
   long steps;
@@ -109,6 +114,7 @@ or_solver *or_create(const or_params *p, const double *z, const double *manning,
   s->dh = zalloc(N); s->dhu = zalloc(N); s->dhv = zalloc(N);
   s->FX = zalloc(N * 9); s->FY = zalloc(N * 9);
   s->maxDepth = zalloc(N); s->TH = zalloc(N);
+  s->keepU = zalloc(N); s->keepV = zalloc(N);
   s->rowA = zalloc((size_t)s->NY); s->rowB = zalloc((size_t)s->NY);
   s->lev = -1;
   {
@@ -155,7 +161,7 @@ void or_destroy(or_solver *s) {
   if (!s) return;
   double *arrs[] = {s->h, s->hu, s->hv, s->z, s->n, s->infil, s->rainField, s->Hp, s->Ep, s->Up, s->Vp,
                     s->h0, s->hu0, s->hv0, s->dh, s->dhu, s->dhv, s->FX, s->FY, s->maxDepth, s->TH,
-                    s->rowA, s->rowB, s->tsp, s->aH, s->aHu, s->aHv, s->inQ, s->inV};
+                    s->rowA, s->rowB, s->tsp, s->aH, s->aHu, s->aHv, s->inQ, s->inV, s->keepU, s->keepV};
   for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) free(arrs[k]);
   free(s->M); free(s->LV); free(s->TI); free(s->inC);
   free(s->hzDv); free(s->hzForce); free(s->hzDur); free(s->hzArr); free(s->hzVmax); free(s->hzWet); free(s->hzCls);
@@ -301,6 +307,11 @@ static void recon(const or_solver *s, int c, int st, double sg, int isX, double 
   int a = c - st, b = c + st;
   int Ma = s->M[a], Mb = s->M[b];
   if (!fo && ((Ma && Hp[a] < hfront) || (Mb && Hp[b] < hfront))) fo = 1;
+  /* F26: a film thinner than the bed's change across the cell is a cascade, reconstructed first order
+   * (solver-cpu.js:222-224) */
+  if (!fo && s->p.stepFlux && !s->p.lts && s->n[c] > 0 &&
+      Hp[c] < 0.5 * fabs((Mb ? s->z[b] : s->z[c]) - (Ma ? s->z[a] : s->z[c])))
+    fo = 1;
   if (fo) { out[0] = Hp[c]; out[1] = Ep[c]; out[2] = un[c]; out[3] = ut[c]; return; }
   double half = 0.5 * sg;
 #define SL(q) lim(s, (q)[c] - (Ma ? (q)[a] : (q)[c]), (Mb ? (q)[b] : (q)[c]) - (q)[c])
@@ -353,10 +364,28 @@ static void face(const or_solver *s, int a, int b, int st, int isX, double *F) {
   double hR = r[4], zR = r[5] - r[4], uR = r[6], vR = r[7];
   double zs = dmax(zL, zR);
   double hLs = dmax(0, r[1] - zs), hRs = dmax(0, r[5] - zs);
-  const double g = s->p.g;
-  hllc(hLs, uL, vL, hRs, uR, vR, g, s->p.hdry, F);
+  const double g = s->p.g, hdry = s->p.hdry;
+  /* F26 step face (solver-cpu.js:250-254): a wet film on one side whose reconstructed depth over the
+   * other side's higher bed is zero passes its own discharge, upwind (_stepFlux, :269-273) */
+  int cas = 0;
+  if (s->p.stepFlux && !s->p.lts && Ma && Mb && s->n[a] > 0 && s->n[b] > 0) {
+    if (hLs > hdry && hRs <= 0 && hR > hdry) cas = 1;
+    else if (hRs > hdry && hLs <= 0 && hL > hdry) cas = -1;
+  }
+  if (cas == 1) {
+    double u = dmax(uL, 0), q = hLs * u;
+    F[0] = q; F[1] = q * u + 0.5 * g * hLs * hLs; F[2] = q * vL;
+  } else if (cas == -1) {
+    double u = dmin(uR, 0), q = hRs * u;
+    F[0] = q; F[1] = q * u + 0.5 * g * hRs * hRs; F[2] = q * vR;
+  } else {
+    hllc(hLs, uL, vL, hRs, uR, vR, g, hdry, F);
+  }
   F[3] = 0.5 * g * (hL * hL - hLs * hLs);
   F[4] = 0.5 * g * (hR * hR - hRs * hRs);
+  /* the lower cell feels the whole step (solver-cpu.js:261-264) */
+  if (cas == 1) F[4] = 0.5 * g * hR * hR + g * hR * (zs - r[5]);
+  else if (cas == -1) F[3] = 0.5 * g * hL * hL + g * hL * (zs - r[1]);
   F[5] = hL; F[6] = zL; F[7] = hR; F[8] = zR;
   if (!Mb) F[4] = 0;
   if (!Ma) F[3] = 0;
@@ -546,7 +575,8 @@ static void sources(or_solver *s, double dt, int sel, double tEnd) {
       if (d > hdry && s->n[c] > 0) {
         double u = s->hu[c] / d, v = s->hv[c] / d;
         double a = dt * g * s->n[c] * s->n[c] / pow(d, 4.0 / 3.0), sp = sqrt(u * u + v * v);
-        double k = 1 + a * sp;
+        /* F26: fully implicit in the new speed (solver-cpu.js:465-467) */
+        double k = s->p.implicitFriction ? 0.5 * (1 + sqrt(1 + 4 * a * sp)) : 1 + a * sp;
         s->hu[c] /= k; s->hv[c] /= k;
       }
       double r = r0 * (s->rainField ? s->rainField[c] : 1) * dt;
@@ -603,6 +633,37 @@ static void exchange(or_solver *s, double dt, int sel) {
 
 static double cycle(or_solver *s, double dtMax);
 
+/* friction alone over dt, implicit form, on the interior open cells (sel -1) or those of level sel
+ * (_friction, solver-cpu.js:443-452) */
+static void friction(or_solver *s, double dt, int sel) {
+  const double g = s->p.g, hdry = s->p.hdry;
+  double *h = s->h, *hu = s->hu, *hv = s->hv;
+#pragma omp parallel for schedule(static)
+  for (int j = GH; j < s->NY - GH; j++)
+    for (int i = GH; i < s->NX - GH; i++) {
+      int c = j * s->NX + i;
+      if (!s->M[c] || (sel >= 0 && lvc(s, c) != sel)) continue;
+      double d = h[c];
+      if (d <= hdry || !(s->n[c] > 0)) continue;
+      double a = dt * g * s->n[c] * s->n[c] / pow(d, 4.0 / 3.0);
+      double sp = hypot(hu[c], hv[c]) / d;
+      double k = 0.5 * (1 + sqrt(1 + 4 * a * sp));
+      hu[c] /= k;
+      hv[c] /= k;
+    }
+}
+/* the second stage's fluxes see the predictor after friction; its update keeps the predictor
+ * (solver-cpu.js:419-423, 542-545) */
+static void keep_momentum(or_solver *s, int save) {
+  if (save) {
+    memcpy(s->keepU, s->hu, sizeof(double) * s->N);
+    memcpy(s->keepV, s->hv, sizeof(double) * s->N);
+  } else {
+    memcpy(s->hu, s->keepU, sizeof(double) * s->N);
+    memcpy(s->hv, s->keepV, sizeof(double) * s->N);
+  }
+}
+
 /* one global step (solver-cpu.js:406-441) */
 double or_step(or_solver *s, double dtMax) {
   if (s->L) return cycle(s, dtMax);
@@ -617,8 +678,10 @@ double or_step(or_solver *s, double dtMax) {
     double q1 = opL(s, h, hu, hv, dt);
     INTERIOR_M(h[c] += dt * s->dh[c]; hu[c] += dt * s->dhu[c]; hv[c] += dt * s->dhv[c];)
     fixcells(s, -1);
+    if (s->p.implicitFriction) { keep_momentum(s, 1); friction(s, dt, -1); }
     s->stage = 1;
     double q2 = opL(s, h, hu, hv, dt);
+    if (s->p.implicitFriction) keep_momentum(s, 0);
     INTERIOR_M(h[c] = 0.5 * (s->h0[c] + h[c] + dt * s->dh[c]); hu[c] = 0.5 * (s->hu0[c] + hu[c] + dt * s->dhu[c]);
                hv[c] = 0.5 * (s->hv0[c] + hv[c] + dt * s->dhv[c]);)
     fixcells(s, -1);
@@ -662,9 +725,11 @@ static void record(or_solver *s, int m, double dt, double tEnd) {
     GATHER();
     INTERIOR_M(if (OWN(c)) { h[c] = s->h0[c] + dt * s->dh[c]; hu[c] = s->hu0[c] + dt * s->dhu[c]; hv[c] = s->hv0[c] + dt * s->dhv[c]; })
     fixcells(s, m);
+    if (s->p.implicitFriction) { keep_momentum(s, 1); friction(s, dt, m); }
     s->stage = 1;
     double q2 = opL(s, h, hu, hv, dt);
     GATHER();
+    if (s->p.implicitFriction) keep_momentum(s, 0);
     INTERIOR_M(if (OWN(c)) {
       h[c] = 0.5 * (s->h0[c] + h[c] + dt * s->dh[c]);
       hu[c] = 0.5 * (s->hu0[c] + hu[c] + dt * s->dhu[c]);

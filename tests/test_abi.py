@@ -39,10 +39,12 @@ def test_version_string():
 
 def test_params_layout():
     from floodsim_amd.solver import fs_params
-    # int32 nx, ny; 6 doubles; int32 secondOrder, limiter, bc[4], lts, rows, untiled, hazard
-    assert ctypes.sizeof(fs_params) == 8 + 6 * 8 + 10 * 4
+    # int32 nx, ny; 6 doubles; int32 secondOrder, limiter, bc[4], lts, rows, untiled, hazard,
+    # implicitFriction, stepFlux
+    assert ctypes.sizeof(fs_params) == 8 + 6 * 8 + 12 * 4
     assert fs_params.untiled.offset == 8 + 6 * 8 + 8 * 4
     assert fs_params.hazard.offset == 8 + 6 * 8 + 9 * 4
+    assert fs_params.stepFlux.offset == 8 + 6 * 8 + 11 * 4
 
 
 def test_missing_library_raises(tmp_path):

@@ -1,0 +1,150 @@
+"""GPU F26 options (implicitFriction, stepFlux) against the FP64 oracle, itself pinned by
+tests/test_oracle.py to the reference's sheet-flow known answers (tests/run-sheetflow-tests.mjs:22-27).
+
+Bars: max |dh| <= 1 mm and |mass rel| <= 1e-6 as for every parity case, plus the reference GPU suite's
+own sheet-flow check (tests/gpu-suite.js:28-44): GPU depth at 100 m within 3 % of the CPU's and within
+15 % of the kinematic depth."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import refcases as rc
+
+pytestmark = pytest.mark.gpu
+TOL_H = 1e-3
+TOL_MASS = 1e-6
+
+
+@pytest.fixture(scope="module")
+def fsa():
+    import floodsim_amd
+    floodsim_amd.lib()
+    return floodsim_amd
+
+
+@pytest.mark.parametrize("dx,S", [(2, 0.05), (4, 0.15)])
+def test_sheet_flow_f26(fsa, dx, S):
+    n, mmh, L = 0.1, 50, 120
+    nx = int(round(L / dx))
+    z = np.tile(100 - S * (np.arange(nx) + 0.5) * dx, 3)
+    opts = dict(nx=nx, ny=3, dx=dx, z=z, manning=n, bc={"east": "outflow"}, implicitFriction=True, stepFlux=True)
+    g = fsa.GPUSolver(**opts)
+    c = oracle.CPUSolver(**opts, hdry=g.hdry)
+    i_ms = mmh / 1000 / 3600
+    for o in (g, c):
+        o.setDepth(lambda X, Y, Z: 0 * X)
+        o.rainRate = i_ms
+    tc = (L * n / math.sqrt(S)) ** 0.6 / i_ms ** 0.4
+    g.advance(3 * tc + 300)
+    c.run(3 * tc + 300)
+    s = g.readState()
+    hc = c.field("h")
+    d = float(np.max(np.abs(s["h"] - hc)))
+    i = int(100 // dx)
+    hk = (i_ms * (i + 0.5) * dx * n / math.sqrt(S)) ** 0.6
+    hg, hcpu = s["h"][nx + i], hc[nx + i]
+    print(f"\nF26 sheet flow dx {dx} S {S}: GPU/kinematic {hg / hk:.4f} CPU/kinematic {hcpu / hk:.4f} "
+          f"GPU/CPU {hg / hcpu:.5f} max|dh| {d:.2e} m, steps {g.steps}/{c.steps}, mass rel {s['mass']['rel']:.1e}")
+    assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS
+    assert abs(hg / hcpu - 1) <= 0.03 and abs(hg / hk - 1) <= 0.15
+
+
+def urban_opts():
+    sc = rc.cloudburst(80)
+    return sc, dict(nx=sc["nx"], ny=sc["ny"], dx=sc["dx"], z=sc["z"], mask=sc["mask"], manning=sc["manning"],
+                    bc=sc["bc"], infil=sc["infil"])
+
+
+def oracle_urban(opts, sc, opt, scale=1.0):
+    c = oracle.CPUSolver(**opts, hdry=1e-5, **opt)
+    c.setDepth(sc["depth"])
+    c.rainRate = 150 * rc.MMHR * scale
+    c.run(600)
+    return c
+
+
+@pytest.mark.parametrize("opt", [dict(implicitFriction=True), dict(stepFlux=True),
+                                 dict(implicitFriction=True, stepFlux=True)])
+def test_f26_urban(fsa, opt):
+    """The urban cloudburst (src/scenarios.js:8-49) under 150 mm/h for 600 s with the F26 options.
+
+    The step-flux rule switches a face between HLLC and the upwind film discharge, and a cell between
+    first and second order, at thresholds (h* <= 0, h < |dz| / 2): on 2D urban terrain the FP64 answer
+    itself jumps by centimetres in single cells when the rain rate moves by 1e-5 of itself (the
+    reference checks this option on its 1D plane only, gpu-suite.js:28-44). With the step flux on, the
+    bar is that spread: max |dh| within 1.5x the largest change of the FP64 answer under rain x (1 +- 1e-6)
+    and (1 +- 1e-5) (and never above 1 mm where that spread is smaller), with at most 0.5 % of the cells
+    beyond 1 mm; books and outflow as everywhere. Implicit friction alone keeps the plain 1 mm bar."""
+    sc, opts = urban_opts()
+    g = fsa.GPUSolver(**opts, **opt)
+    g.setDepth(sc["depth"])
+    g.rainRate = 150 * rc.MMHR
+    g.advance(600)
+    c = oracle_urban(opts, sc, opt)
+    s = g.readState()
+    h = c.field("h")
+    dh = np.abs(s["h"] - h)
+    d = float(dh.max())
+    bar = TOL_H
+    if opt.get("stepFlux"):
+        spread = max(float(np.max(np.abs(oracle_urban(opts, sc, opt, 1 + e).field("h") - h)))
+                     for e in (1e-6, -1e-6, 1e-5, -1e-5))
+        bar = max(TOL_H, 1.5 * spread)
+        print(f"\n  FP64 spread under rain x (1 +- 1e-6, 1e-5): {spread:.2e} m")
+    print(f"\nF26 {opt} urban: max|dh| {d:.2e} m (bar {bar:.2e}), cells beyond 1 mm {(dh > TOL_H).sum()}/{dh.size}, "
+          f"steps {g.steps}/{c.steps}, mass rel {s['mass']['rel']:.1e}, "
+          f"outflow {s['mass']['outflow']:.3f}/{c.vol['outflow']:.3f}")
+    assert d <= bar and (dh > TOL_H).sum() <= 0.005 * dh.size
+    assert abs(s["mass"]["rel"]) <= TOL_MASS and s["mass"]["clipped"] == 0
+    assert abs(s["mass"]["outflow"] - c.vol["outflow"]) <= 1e-3 * c.vol["outflow"] + 1e-6
+
+
+def test_f26_lts(fsa):
+    """Local time stepping with implicit friction (the record's own cells take the predictor's friction,
+    solver-cpu.js:542-545); stepFlux has no effect under LTS, in the reference as here."""
+    sc, opts = urban_opts()
+    opt = dict(implicitFriction=True, stepFlux=True)
+    g = fsa.GPUSolver(**opts, lts=3, **opt)
+    c = oracle.CPUSolver(**opts, lts=3, hdry=g.hdry, **opt)
+    for o in (g, c):
+        o.setDepth(sc["depth"])
+        o.rainRate = 150 * rc.MMHR
+    for t in np.arange(10, 300 + 1e-9, 10):
+        g.advance(float(t), 8)
+        c.run(float(t))
+    s = g.readState()
+    d = float(np.max(np.abs(s["h"] - c.field("h"))))
+    print(f"\nF26 LTS urban: max|dh| {d:.2e} m, cycles {g.steps}/{c.steps}, mass rel {s['mass']['rel']:.1e}")
+    assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS
+
+
+@pytest.mark.parametrize("partner", [dict(implicitFriction=True, stepFlux=True), dict(hazard=True),
+                                     dict(secondOrder=False, hazard=True)])
+def test_instance_kinds_mixed(fsa, partner):
+    """A group of instances of different kinds (step flux, hazard products, first order) runs one launch
+    per kernel variant, each advancing only its own kind: a plain instance gives the same bits alone and
+    beside an instance of another kind."""
+    sc, opts = urban_opts()
+    plane = dict(nx=60, ny=3, dx=2.0, z=np.tile(100 - 0.05 * (np.arange(60) + 0.5) * 2, 3), manning=0.1,
+                 bc={"east": "outflow"})
+
+    def run(specs):
+        P = fsa.Pool()
+        slots = []
+        for spec in specs:
+            s = P.add(**spec)
+            P.set_state(s, np.zeros(spec["nx"] * spec["ny"]))
+            P.set_rain(s, 120 * rc.MMHR)
+            slots.append(s)
+        P.advance(slots, [300.0] * len(slots))
+        out = [P.read_state(s) for s in slots]
+        P.destroy()
+        return out
+
+    alone = run([opts])[0]
+    mixed = run([opts, dict(plane, **partner)])
+    for k in ("h", "hu", "hv", "hmax"):
+        np.testing.assert_array_equal(alone[k], mixed[0][k])
+    assert abs(mixed[1]["mass"]["rel"]) <= TOL_MASS

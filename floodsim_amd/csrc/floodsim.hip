@@ -630,14 +630,16 @@ int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n
 // strip of interior cell (i, j) under global stepping (work items run x fastest)
 inline int strip_of(const Slot &S, int i, int j) { return (j / S.rows) * S.sxn + i / strip_own(&S.p); }
 
-// per strip: an open cell with a positive rain multiplier (all open cells without a rain field)
+// per strip: bit 0 an open cell with a positive rain multiplier (all open cells without a rain field),
+// bit 1 a stage cell (such strips always step: the stage can flood a dry edge)
 int upload_strip_rain(fs_pool *P, Slot &S) {
   if (!S.nstrips) return FS_OK;
   std::vector<uint8_t> sr(S.nstrips, 0);
   for (int j = 0; j < S.ny; ++j)
     for (int i = 0; i < S.nx; ++i) {
       const size_t k = (size_t)j * S.nx + i;
-      if ((S.hmask[k] & MASK_OPEN) && (S.hrf.empty() || S.hrf[k] > 0.f)) sr[strip_of(S, i, j)] = 1;
+      if ((S.hmask[k] & MASK_OPEN) && (S.hrf.empty() || S.hrf[k] > 0.f)) sr[strip_of(S, i, j)] |= 1;
+      if (S.hmask[k] & MASK_STAGE) sr[strip_of(S, i, j)] |= 2;
     }
   HIPCHK(P, hipMemcpy(S.srain, sr.data(), (size_t)S.nstrips, hipMemcpyHostToDevice));
   return FS_OK;
@@ -810,6 +812,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   r.openCells = open;
   r.levels = p->lts;
   r.dirty = 1;
+  r.stage = -1e30f;   // no stage until setStage
   rc = upload_rec(P, slot);
   if (rc) return rc;
   HIPCHK(P, hipStreamSynchronize(P->stream));
@@ -1022,6 +1025,35 @@ int fs_pool_set_rain(fs_pool_t P, int32_t slot, double rate) {
   return FS_OK;
 }
 
+// water-level boundary: the stage cells (interior edge cells whose outward ghosts hold the stage;
+// solver-gpu.js:125-133) and the stage (setStage, solver-gpu.js:554; stored relative to the bed minimum
+// as solver-gpu.js:296 does)
+int fs_pool_set_stage_cells(fs_pool_t P, int32_t slot, const int32_t *cells, int32_t n) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (n < 0 || (n && !cells)) return fail(P, FS_E_ARG, "stage cells: bad list");
+  const int64_t N = (int64_t)S.nx * S.ny;
+  for (int q = 0; q < n; ++q)
+    if (cells[q] < 0 || cells[q] >= N) return fail(P, FS_E_ARG, "stage cell %d outside the grid", cells[q]);
+  for (auto &b : S.hmask) b &= (uint8_t)~MASK_STAGE;
+  for (int q = 0; q < n; ++q) S.hmask[cells[q]] |= MASK_STAGE;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  HIPCHK(P, hipMemcpy(S.mask, S.hmask.data(), (size_t)N, hipMemcpyHostToDevice));
+  return upload_strip_rain(P, S);
+}
+
+int fs_pool_set_stage(fs_pool_t P, int32_t slot, double z) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  const float st = (float)(z - S.zref);
+  HIPCHK(P, hipMemcpyAsync(&P->drec[slot].stage, &st, sizeof st, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  S.hrec.stage = st;
+  return FS_OK;
+}
+
 int fs_pool_set_inlet_rates(fs_pool_t P, int32_t slot, const double *q, int32_t n) {
   int rc = valid_slot(P, slot);
   if (rc) return rc;
@@ -1214,6 +1246,10 @@ int fs_set_state(fs_handle s, const double *h, const double *hu, const double *h
 }
 
 int fs_set_rain(fs_handle s, double rate) { return s ? fs_pool_set_rain(s->pool, s->slot, rate) : FS_E_ARG; }
+int fs_set_stage_cells(fs_handle s, const int32_t *cells, int32_t n) {
+  return s ? fs_pool_set_stage_cells(s->pool, s->slot, cells, n) : FS_E_ARG;
+}
+int fs_set_stage(fs_handle s, double z) { return s ? fs_pool_set_stage(s->pool, s->slot, z) : FS_E_ARG; }
 
 int fs_set_rain_field(fs_handle s, const double *field) {
   if (!s || !field) return FS_E_ARG;

@@ -310,6 +310,32 @@ static napi_value js_set_inlet_rates(napi_env env, napi_callback_info info) {
   return rc ? fail(env, s->h, rc) : undefined_value(env);
 }
 
+/* setStageCells(h, Int32Array cells) -> fs_set_stage_cells; setStage(h, z) -> fs_set_stage */
+static napi_value js_set_stage_cells(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  void *c;
+  size_t n = 0;
+  if (typed(env, a[1], napi_int32_array, -1, &c, &n, "stageCells")) return NULL;
+  int rc = fs_set_stage_cells(s->h, (const int32_t *)c, (int32_t)n);
+  return rc ? fail(env, s->h, rc) : undefined_value(env);
+}
+
+static napi_value js_set_stage(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  double z = 0;
+  napi_get_value_double(env, a[1], &z);
+  int rc = fs_set_stage(s->h, z);
+  return rc ? fail(env, s->h, rc) : undefined_value(env);
+}
+
 static napi_value js_clock(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
@@ -717,6 +743,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setRain", NULL, js_set_rain, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setRainField", NULL, js_set_rain_field, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setInletRates", NULL, js_set_inlet_rates, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setStageCells", NULL, js_set_stage_cells, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setStage", NULL, js_set_stage, NULL, NULL, NULL, napi_enumerable, NULL},
       {"clock", NULL, js_clock, NULL, NULL, NULL, napi_enumerable, NULL},
       {"tileStats", NULL, js_tile_stats, NULL, NULL, NULL, napi_enumerable, NULL},
       {"levels", NULL, js_levels, NULL, NULL, NULL, napi_enumerable, NULL},

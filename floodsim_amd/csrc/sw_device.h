@@ -39,8 +39,9 @@ __device__ __forceinline__ void gst(float2 *row, int x, float2 v) {
   const fs_v2 t = {v.x, v.y};
   *(FS_GLOBAL fs_v2 *)((FS_GLOBAL char *)row + (uint32_t)x * 8u) = t;
 }
-// mask byte bits: open (not a wall) cell, inlet cell, Manning n > 0 (the step-flux rules apply)
-enum { MASK_OPEN = 1, MASK_INLET = 2, MASK_ROUGH = 4 };
+// mask byte bits: open (not a wall) cell, inlet cell, Manning n > 0 (the step-flux rules apply), stage
+// cell (its outward ghosts hold the stage)
+enum { MASK_OPEN = 1, MASK_INLET = 2, MASK_ROUGH = 4, MASK_STAGE = 8 };
 // DevInst flags: infiltration layer, rain field, implicit friction and step flux (F26)
 enum { FLAG_INFIL = 1, FLAG_RF = 2, FLAG_IMPF = 4, FLAG_STEPF = 8 };
 
@@ -76,6 +77,7 @@ struct StepRec {
   int32_t lcount[kMaxLevels + 1]; // work-list length per level
   int32_t records;                // records run so far (diagnostic)
   int32_t active;                 // open cells the last step advanced (dry strips skipped)
+  float stage;                    // water level of the stage cells' ghosts, m relative to the bed minimum
 };
 
 // one inlet: cell index (y*pitch + x), rate (m^3/s, + into the network, - a point inflow)
@@ -110,7 +112,7 @@ struct DevInst {
   // block is not quiet or rain falls on it, which leaves the results identical to advancing all.
   uint8_t *wet;                 // per strip: not quiet after the last step (+ inlet exchange)
   uint8_t *act;                 // per strip: advanced in the current step
-  const uint8_t *srain;         // per strip: an open cell with a positive rain multiplier
+  const uint8_t *srain;         // per strip: bit 0 an open cell with a positive rain multiplier, bit 1 a stage cell
   const int32_t *sopen;         // per strip: open cells it owns
   // hazard products (null: not tracked): A = (peak d*v, peak force, time deeper than 15 cm, first
   // time deeper than 15 cm or -1), B = (peak speed, first time deeper than VCUT or -1, worst classes

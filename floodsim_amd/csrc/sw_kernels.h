@@ -245,6 +245,26 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m,
       else if (side == 2) p.v = -ov;
       else p.v = ov;
     }
+    // water-level boundary (stageCells, setStage): the ghost of a stage cell holds the surface at the
+    // stage where that stands above the ghost's bed, moving with the edge cell's velocity
+    // (solver-cpu.js:197-202; kernels.wgsl.js:148-154)
+    if (I.mask[kz] & MASK_STAGE) {
+      const float hs = I.rec->stage - p.z;
+      if (hs > I.hdry) {
+        float he, hue, hve;
+        cell_state<STAGE, LTS>(I, zx, zy, m, he, hue, hve);
+        float ue = 0.f, ve = 0.f;
+        if (he > I.hdry) {
+          const float ie = frcp(he);
+          ue = hue * ie;
+          ve = hve * ie;
+        }
+        bool eB = STAGE == 1;
+        if constexpr (LTS) eB = STAGE == 1 && tile_level(I, zx, zy) == m;
+        if (STAGE == 1 && (I.flags & FLAG_IMPF) && eB && (I.mask[kz] & MASK_OPEN)) pre_friction(I, (int)kz, he, dt, ue, ve);
+        p.h = hs; p.u = ue; p.v = ve; p.hu = hs * ue; p.hv = hs * ve;
+      }
+    }
   }
   return p;
 }
@@ -723,7 +743,10 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
 // advance strip u this step: rain falls on it, or a strip of its 3x3 block is not quiet
 // (the stencil of a whole step reaches 6 cells, less than a strip's width and height)
 __device__ __forceinline__ bool strip_active(const DevInst &I, int u) {
-  if (I.rec->rainRate > 0.f && I.srain[u]) return true;
+  // srain bit 0: an open cell with rain; bit 1: a stage cell (always stepping, like the reference's
+  // stage-edge tiles, solver-gpu.js:134-143)
+  const uint8_t sr = I.srain[u];
+  if ((sr & 2) || (I.rec->rainRate > 0.f && (sr & 1))) return true;
   const int sx = I.sxn, sy = (I.wend - I.wbeg) / I.sxn;
   const int ux = u % sx, uy = u / sx;
   bool any = false;

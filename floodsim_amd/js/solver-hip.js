@@ -12,10 +12,12 @@
  * What matches the reference:
  * - constructor options nx, ny, dx, z, manning (number or array), mask, infil (number or array),
  *   rainField, bc {west, east, south, north: 'wall'|'open'|'outflow'}, secondOrder, limiter, cfl,
- *   hdry, hfront, dtDry, g, inlets, lts (true = 3 levels, or 1-4), tiles (solver-gpu.js:35-47);
+ *   hdry, hfront, dtDry, g, inlets, lts (true = 3 levels, or 1-4), tiles, implicitFriction, stepFlux,
+ *   stageCells (solver-gpu.js:35-47, 125-133);
  * - setDepth(fn(x, y, z)) and setVelocity(fn(x, y)) evaluated at cell centres, depth clamped at 0
  *   and 0 in walls, vol.initial the sum of the stored FP32 depths (solver-gpu.js:301-318);
- * - rainRate (m/s), setRainField, setInletRates (m^3/s, + = surface into the network),
+ * - rainRate (m/s), setRainField, setStage(z) (m, the level the stage cells' ghosts hold),
+ *   setInletRates (m^3/s, + = surface into the network),
  *   readInlets() -> {vol: Float64Array, h: Float32Array, app}, advance(tTarget, K = 64) with the
  *   'GPU solver made no progress (device error?)' error, frame(tEnd, K, readOut) ->
  *   Float32Array (depth, speed per cell) or null, readState() -> {h, hu, hv, hmax, mass},
@@ -24,8 +26,8 @@
  *
  * The device argument is accepted for signature compatibility and ignored: the library drives
  * HIP device 0 of the process. Options of the reference outside the shallow-water step
- * (subgrid, sink, gi, barriers, sediment, debris, stageCells) raise. implicitFriction and stepFlux
- * (F26) are taken as the reference takes them (stepFlux has no effect under local time stepping).
+ * (subgrid, sink, gi, barriers, sediment, debris) raise. implicitFriction and stepFlux (F26) are taken
+ * as the reference takes them (stepFlux has no effect under local time stepping).
  *
  * Calls on one solver run one at a time: asynchronous calls queue behind each other, and the
  * synchronous setters wait for the queue when it is busy, so they land between the calls they
@@ -37,7 +39,7 @@ const addon = require(path.join(__dirname, 'floodsim_napi.node'));
 
 const GH = 2; // the reference's ghost width, kept for idx(i, j) and zh
 const BC = { wall: 0, open: 1, outflow: 2 };
-const UNSUPPORTED = ['subgrid', 'sink', 'gi', 'barriers', 'sediment', 'debris', 'stageCells', 'stage'];
+const UNSUPPORTED = ['subgrid', 'sink', 'gi', 'barriers', 'sediment', 'debris'];
 
 function def(v, d) { return v === undefined || v === null ? d : v; }
 
@@ -95,6 +97,8 @@ class GPUSolver {
     const infil = o.infil === undefined || o.infil === null ? null : f64(o.infil, n, 'infil');
     const inlets = this.nInlets ? Int32Array.from(o.inlets) : null;
     this._h = addon.create(params, z, manArr, manS, this._mask, infil, rf, inlets);
+    this.stage = -1e9;
+    if (o.stageCells && o.stageCells.length) addon.setStageCells(this._h, Int32Array.from(o.stageCells));
     this.t = 0; this.steps = 0; this.lastDt = 0; this.dtAvg = 0;
     this.vol = { rain: 0, initial: 0, exchanged: 0 };
     this._rain = 0;
@@ -189,6 +193,12 @@ class GPUSolver {
   setRainField(f) {
     const a = f64(f, this.nx * this.ny, 'rainField');
     this._sync(() => addon.setRainField(this._h, a));
+  }
+
+  /** Water level (m, absolute) the stage cells' ghosts hold from the next step on (solver-gpu.js:554). */
+  setStage(z) {
+    this.stage = +z;
+    this._sync(() => addon.setStage(this._h, this.stage));
   }
 
   /** Exchange rates per inlet (m^3/s, + = surface into the network), held until the next call. */

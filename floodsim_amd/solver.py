@@ -57,6 +57,7 @@ EXPORTS = [
     "fs_pool_kernel_ms", "fs_pool_set_timing", "fs_solver_pool", "fs_version", "fs_copy_probe",
     "fs_pool_dominant_kernel", "fs_lts_levels", "fs_pool_set_rows", "fs_snapshot_bytes", "fs_snapshot",
     "fs_restore", "fs_pool_snapshot_bytes", "fs_pool_snapshot", "fs_pool_restore", "fs_read_hazard",
+    "fs_set_stage_cells", "fs_set_stage", "fs_pool_set_stage_cells", "fs_pool_set_stage",
 ]
 
 
@@ -105,6 +106,10 @@ def lib(path=None):
     L.fs_pool_set_state.argtypes = [P, I32, dp, dp, dp, I32]
     L.fs_pool_set_rain.argtypes = [P, I32, D]
     L.fs_pool_set_inlet_rates.argtypes = [P, I32, dp, I32]
+    L.fs_pool_set_stage_cells.argtypes = [P, I32, ip, I32]
+    L.fs_pool_set_stage.argtypes = [P, I32, D]
+    L.fs_set_stage_cells.argtypes = [P, ip, I32]
+    L.fs_set_stage.argtypes = [P, D]
     L.fs_pool_advance.argtypes = [P, ip, dp, I32, I32, ctypes.POINTER(fs_clock)]
     L.fs_pool_step.argtypes = [P, ip, dp, I32, I32]
     L.fs_pool_read_state.argtypes = [P, I32, dp, dp, dp, dp, ctypes.POINTER(fs_mass)]
@@ -189,7 +194,8 @@ class _Layers:
 class GPUSolver:
     """The reference GPUSolver interface over the HIP library (one instance)."""
 
-    def __init__(self, nx, ny, dx, z, manning=0.0, mask=None, infil=None, rainField=None, inlets=None, **opts):
+    def __init__(self, nx, ny, dx, z, manning=0.0, mask=None, infil=None, rainField=None, inlets=None,
+                 stageCells=None, **opts):
         L = lib()
         self.nx, self.ny, self.dx = int(nx), int(ny), float(dx)
         self.p = make_params(nx, ny, dx, **opts)
@@ -202,6 +208,9 @@ class GPUSolver:
         if rc != 0:
             raise FloodSimError(f"fs_create failed ({rc}): {L.fs_last_error(None).decode()}")
         self._h = h
+        if stageCells is not None and len(stageCells):
+            sk = np.ascontiguousarray(np.asarray(stageCells, dtype=np.int32))
+            self._chk(L.fs_set_stage_cells(h, sk.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(sk)))
         self.nInlets = len(self.layers.inlets)
         self._rain = 0.0
         self._Uh = np.zeros(self.nx * self.ny)
@@ -273,6 +282,10 @@ class GPUSolver:
     def setRainField(self, field):
         f = _f64(field, self.nx * self.ny)
         self._chk(lib().fs_set_rain_field(self._h, _dp(f)))
+
+    def setStage(self, z):
+        """Water level held by the stage cells' ghosts (m, absolute; solver-gpu.js:554)."""
+        self._chk(lib().fs_set_stage(self._h, float(z)))
 
     def setInletRates(self, q):
         a = np.ascontiguousarray(np.asarray(q, dtype=np.float64))
@@ -420,8 +433,9 @@ class Pool:
     def bytes_used(self):
         return lib().fs_pool_bytes_used(self._p)
 
-    def add(self, nx, ny, dx, z, manning=0.0, mask=None, infil=None, rainField=None, inlets=None, **opts):
-        """Admit one instance; raises FloodSimError with code -3 when the HBM budget is spent."""
+    def add(self, nx, ny, dx, z, manning=0.0, mask=None, infil=None, rainField=None, inlets=None, stageCells=None,
+            **opts):
+        """Admit one instance; returns None when the HBM budget is spent."""
         p = make_params(nx, ny, dx, **opts)
         lay = _Layers(int(nx), int(ny), z, manning, mask, infil, rainField, inlets)
         slot = ctypes.c_int32()
@@ -430,7 +444,19 @@ class Pool:
             return None
         self._chk(rc)
         self.meta[slot.value] = (int(nx), int(ny), float(dx), lay)
+        if stageCells is not None and len(stageCells):
+            self.set_stage_cells(slot.value, stageCells)
         return slot.value
+
+    def set_stage_cells(self, slot, cells):
+        """Stage cells of one instance (interior edge cells j*nx+i whose outward ghosts hold the stage)."""
+        sk = np.ascontiguousarray(np.asarray(cells, dtype=np.int32))
+        self._chk(lib().fs_pool_set_stage_cells(self._p, int(slot), sk.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                len(sk)))
+
+    def set_stage(self, slot, z):
+        """Water level of one instance's stage cells' ghosts (m, absolute)."""
+        self._chk(lib().fs_pool_set_stage(self._p, int(slot), float(z)))
 
     def set_rows(self, slot, rows):
         """Rows per wavefront strip of one instance (the autotuned tile shape)."""

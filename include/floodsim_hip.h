@@ -87,6 +87,10 @@ int fs_set_state(fs_handle s, const double *h, const double *hu, const double *h
 int fs_set_rain(fs_handle s, double rate_ms);              /* rainRate property (solver-gpu.js:48, 603) */
 int fs_set_rain_field(fs_handle s, const double *field);   /* setRainField (solver-gpu.js:470-478) */
 int fs_set_inlet_rates(fs_handle s, const double *q, int32_t n);           /* setInletRates (:557) */
+/* the stageCells option (solver-gpu.js:125-133): interior edge cells j*nx+i whose outward ghosts hold
+ * the stage; a new list replaces the old one */
+int fs_set_stage_cells(fs_handle s, const int32_t *cells, int32_t n);
+int fs_set_stage(fs_handle s, double z);                   /* setStage (solver-gpu.js:554), m absolute */
 int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n);        /* readInlets (:564-576) */
 int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk);     /* advance (:589-604) */
 /* frame (:607-622): at most K steps toward tEnd; out (nx*ny*2: depth, speed) may be NULL */
@@ -125,6 +129,8 @@ int fs_pool_set_state(fs_pool_t pool, int32_t slot, const double *h, const doubl
                       int32_t bookInitial);
 int fs_pool_set_rain(fs_pool_t pool, int32_t slot, double rate_ms);
 int fs_pool_set_inlet_rates(fs_pool_t pool, int32_t slot, const double *q, int32_t n);
+int fs_pool_set_stage_cells(fs_pool_t pool, int32_t slot, const int32_t *cells, int32_t n);
+int fs_pool_set_stage(fs_pool_t pool, int32_t slot, double z);
 /* advance the listed slots together until each reaches its own target (K steps per launch group);
  * clks (n entries) may be NULL */
 int fs_pool_advance(fs_pool_t pool, const int32_t *slots, const double *tTargets, int32_t n, int32_t K,

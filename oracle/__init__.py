@@ -75,6 +75,8 @@ def lib():
         L.or_set_state.argtypes = [P, dp, dp, dp]
         L.or_set_rain_rate.argtypes = [P, D]
         L.or_set_inlets.argtypes = [P, ctypes.POINTER(ctypes.c_int), I]
+        L.or_set_stage_cells.argtypes = [P, ctypes.POINTER(ctypes.c_int), I]
+        L.or_set_stage.argtypes = [P, D]
         L.or_set_inlet_rates.argtypes = [P, dp]
         L.or_read_inlets.argtypes = [P, dp, dp]
         L.or_step.restype = D
@@ -121,13 +123,14 @@ class CPUSolver:
     Options: nx, ny, dx, z (nx*ny), manning (number or array), mask (uint8, 0 = wall),
     bc {west,east,south,north}, secondOrder, limiter, cfl, hdry, hfront, dtDry, lts,
     infil (number or array, m/s), rainField (array), inlets (interior indices j*nx+i),
-    implicitFriction, stepFlux (F26; stepFlux has no effect with lts, as in the reference).
+    implicitFriction, stepFlux (F26; stepFlux has no effect with lts, as in the reference),
+    stageCells (interior edge cells whose outward ghosts see the level set by setStage).
     """
 
     def __init__(self, nx, ny, dx, z, manning=0.0, mask=None, bc=None, secondOrder=True,
                  limiter="minmod", cfl=None, hdry=1e-6, hfront=1e-3, dtDry=1.0, lts=0,
                  g=9.81, infil=None, rainField=None, inlets=None, implicitFriction=False, stepFlux=False,
-                 **_ignored):
+                 stageCells=None, **_ignored):
         L = lib()
         self.nx, self.ny, self.dx = int(nx), int(ny), float(dx)
         b = {"west": "wall", "east": "wall", "south": "wall", "north": "wall"}
@@ -167,6 +170,9 @@ class CPUSolver:
             ink = np.ascontiguousarray(np.asarray(inlets, dtype=np.int32))
             self.nInlets = len(ink)
             L.or_set_inlets(self._h, ink.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(ink))
+        if stageCells is not None and len(stageCells):
+            sk = np.ascontiguousarray(np.asarray(stageCells, dtype=np.int32))
+            L.or_set_stage_cells(self._h, sk.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(sk))
         self._rain = 0.0
 
     def __del__(self):
@@ -218,6 +224,10 @@ class CPUSolver:
     def rainRate(self, r):
         self._rain = float(r)
         lib().or_set_rain_rate(self._h, self._rain)
+
+    def setStage(self, z):
+        """The water level the stage cells' ghosts hold (m, absolute; solver-cpu.js:204)."""
+        lib().or_set_stage(self._h, float(z))
 
     def setInletRates(self, q):
         a = np.ascontiguousarray(np.asarray(q, dtype=np.float64))

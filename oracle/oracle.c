@@ -22,6 +22,8 @@
  *   F26 options       implicitFriction (_friction, the implicit factor of _sources) and stepFlux
  *                     (the cascade clause of _recon, _face's step flux, _stepFlux)
  *                                                      solver-cpu.js:222-225, 250-273, 419-423, 443-452, 465-467, 542-545
+ *   water-level edge  stageCells, setStage, the stage ghosts of _fillGhost
+ *                                                      solver-cpu.js:109-117, 197-204
  * Out of scope here (as in the product): flood-barrier weirs, Green-Ampt and GI stores.
  *
  * Loops over rows may run on several OpenMP threads. Every per-cell update is
@@ -60,6 +62,10 @@ typedef struct {
   double rainRate;
   double *Hp, *Ep, *Up, *Vp, *h0, *hu0, *hv0, *dh, *dhu, *dhv, *FX, *FY, *maxDepth, *TH;
   double *keepU, *keepV; /* implicitFriction: the predictor's momentum, restored after the second L */
+  /* water-level boundary: [ghost, edge cell] pairs of the stage cells, and the stage (absolute, m) */
+  int nStage;
+  int *stageG, *stageS;
+  double stageLevel;
   double *rowA, *rowB; /* per-row partial sums */
   double t;
   long steps;
@@ -117,6 +123,7 @@ or_solver *or_create(const or_params *p, const double *z, const double *manning,
   s->keepU = zalloc(N); s->keepV = zalloc(N);
   s->rowA = zalloc((size_t)s->NY); s->rowB = zalloc((size_t)s->NY);
   s->lev = -1;
+  s->stageLevel = -1e9;
   {
     size_t n = (size_t)p->nx * p->ny;
     s->hzDv = zalloc(n); s->hzForce = zalloc(n); s->hzDur = zalloc(n); s->hzArr = zalloc(n);
@@ -163,7 +170,7 @@ void or_destroy(or_solver *s) {
                     s->h0, s->hu0, s->hv0, s->dh, s->dhu, s->dhv, s->FX, s->FY, s->maxDepth, s->TH,
                     s->rowA, s->rowB, s->tsp, s->aH, s->aHu, s->aHv, s->inQ, s->inV, s->keepU, s->keepV};
   for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) free(arrs[k]);
-  free(s->M); free(s->LV); free(s->TI); free(s->inC);
+  free(s->M); free(s->LV); free(s->TI); free(s->inC); free(s->stageG); free(s->stageS);
   free(s->hzDv); free(s->hzForce); free(s->hzDur); free(s->hzArr); free(s->hzVmax); free(s->hzWet); free(s->hzCls);
   free(s);
 }
@@ -284,7 +291,37 @@ static void fillGhost(or_solver *s, double *h, double *hu, double *hv) {
       if (bc[3] == BC_OUTFLOW) hv[c] = h[c] * outVel(s, h, src, src - NX);
     }
 #undef PUT
+  /* water-level boundary: the ghost surface held at the stage when it stands above the ghost's bed, with
+   * the edge cell's velocity (solver-cpu.js:197-202) */
+  for (int m = 0; m < s->nStage; m++) {
+    int c = s->stageG[m], e = s->stageS[m];
+    double hs = s->stageLevel - s->z[c];
+    if (!(hs > s->p.hdry)) continue;
+    double u = h[e] > s->p.hdry ? hu[e] / h[e] : 0, v = h[e] > s->p.hdry ? hv[e] / h[e] : 0;
+    h[c] = hs; hu[c] = hs * u; hv[c] = hs * v;
+  }
 }
+
+/* stageCells (solver-cpu.js:109-117): interior edge cells whose outward ghosts (both layers) see the stage */
+void or_set_stage_cells(or_solver *s, const int *cells, int n) {
+  free(s->stageG); free(s->stageS);
+  s->stageG = (int *)calloc((size_t)(n ? 4 * GH * n : 1), sizeof(int));
+  s->stageS = (int *)calloc((size_t)(n ? 4 * GH * n : 1), sizeof(int));
+  s->nStage = 0;
+  const int nx = s->p.nx, ny = s->p.ny, NX = s->NX;
+  for (int q = 0; q < n; q++) {
+    int i = cells[q] % nx, j = cells[q] / nx, e = (j + GH) * NX + i + GH;
+#define ADD(ii, jj) do { s->stageG[s->nStage] = ((jj) + GH) * NX + (ii) + GH; s->stageS[s->nStage] = e; s->nStage++; } while (0)
+    for (int g2 = 1; g2 <= GH; g2++) {
+      if (i == 0) ADD(-g2, j);
+      if (i == nx - 1) ADD(nx - 1 + g2, j);
+      if (j == 0) ADD(i, -g2);
+      if (j == ny - 1) ADD(i, ny - 1 + g2);
+    }
+#undef ADD
+  }
+}
+void or_set_stage(or_solver *s, double z) { s->stageLevel = z; }
 
 static void prims(or_solver *s, const double *h, const double *hu, const double *hv) {
   const int N = s->N;

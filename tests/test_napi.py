@@ -20,7 +20,8 @@ NODE = shutil.which("node")
 pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
 ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readHazard", "readInlets", "readState",
-               "restore", "setInletRates", "setRain", "setRainField", "setState", "snapshot", "tileStats", "version"]
+               "restore", "setInletRates", "setRain", "setRainField", "setStage", "setStageCells", "setState",
+               "snapshot", "tileStats", "version"]
 
 
 def node(script, *args, timeout=120):
@@ -70,11 +71,16 @@ def _case(nx=48, ny=40, seed=5):
 def _python_run(c):
     import floodsim_amd as fsa
     g = fsa.GPUSolver(nx=c["nx"], ny=c["ny"], dx=c["dx"], z=np.array(c["z"]), manning=c["manning"],
-                      mask=np.array(c["mask"], np.uint8), bc=c["bc"], inlets=c["inlets"])
+                      mask=np.array(c["mask"], np.uint8), bc=c["bc"], inlets=c["inlets"] or None,
+                      stageCells=c.get("stageCells"), implicitFriction=c.get("implicitFriction", False),
+                      stepFlux=c.get("stepFlux", False), hazard=c.get("hazard", True))
     g.setDepth(np.array(c["depth"]))
     g.rainRate = c["rain"]
-    g.setInletRates(c["rates"])
-    for t in c["frames"]:
+    if c["rates"]:
+        g.setInletRates(c["rates"])
+    for f, t in enumerate(c["frames"]):
+        if c.get("stageLevels"):
+            g.setStage(c["stageLevels"][f])
         g.frame(t, 1 << 20, True)
     s = g.readState()
     steps = g.steps
@@ -113,3 +119,41 @@ def test_js_matches_python_binding_and_oracle(tmp_path):
     assert abs(js["mass"]["rel"]) <= 1e-6
     assert js["inletVol"] < 0            # water entered through the inlet (q < 0 is inflow)
     assert abs(js["volRain"] - js["mass"]["rain"]) <= 1e-6 * max(1.0, js["mass"]["rain"])
+
+
+@pytest.mark.gpu
+def test_js_stage_and_f26_case(tmp_path):
+    """The water-level boundary and the F26 options through the JavaScript class: the sea edge of the
+    reference's stage case (gpu-suite.js:148-164) rising 0.4 m, with implicit friction and the step flux;
+    the same bits as the Python binding and within 1 mm of the FP64 oracle."""
+    import oracle
+    nx, ny, dx = 50, 20, 4.0
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = (98 + 0.08 * I + 0.3 * np.sin(J)).reshape(-1)
+    z0 = 99.5
+    frames = [float(t) for t in range(30, 601, 30)]
+    c = dict(nx=nx, ny=ny, dx=dx, z=z.tolist(), manning=0.02, mask=[1] * (nx * ny),
+             bc={k: "outflow" for k in ("west", "east", "south", "north")}, depth=np.maximum(0, z0 - z).tolist(),
+             rain=0.0, inlets=[], rates=[], frames=frames, stageCells=[j * nx for j in range(ny)],
+             stageLevels=[z0 + 0.4 * min(1.0, t / 600) for t in frames], implicitFriction=True, stepFlux=True,
+             hazard=False)
+    p = tmp_path / "stage.json"
+    p.write_text(json.dumps(c))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "run_case.js"), str(p)], cwd=ROOT,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    py, steps = _python_run(c)
+    assert js["steps"] == steps
+    for k in ("h", "hu", "hv", "hmax"):
+        assert np.array_equal(np.array(js[k]), py[k]), k
+    o = oracle.CPUSolver(nx=nx, ny=ny, dx=dx, z=z, manning=0.02, bc=c["bc"], stageCells=c["stageCells"],
+                         implicitFriction=True, stepFlux=True, hdry=1e-5)
+    o.setDepth(np.array(c["depth"]))
+    for f, t in enumerate(frames):
+        o.setStage(c["stageLevels"][f])
+        o.run(t)
+    d = float(np.max(np.abs(np.array(js["h"]) - o.field("h"))))
+    print(f"JS stage case: {js['steps']} steps, max|dh| vs oracle {d:.2e} m, inflow {-js['mass']['outflow']:.2f} "
+          f"m3 (oracle {-o.vol['outflow']:.2f}), mass rel {js['mass']['rel']:.1e}")
+    assert d <= 1e-3 and abs(js["mass"]["rel"]) <= 1e-6 and -js["mass"]["outflow"] > 0

@@ -303,6 +303,119 @@ def test_friction_factor(implicit):
         np.testing.assert_allclose(hu1, h * u / (1 + a * u), rtol=1e-13)
 
 
+OUT_W, OUT_DX, OUT_N, OUT_Q, OUT_DOWN = 5, 10, 0.03, 200, 3
+
+
+def outlet_channel(nx, edge, T=4000):
+    """run-outlet-tests.mjs:35-46: a channel OUT_W cells wide on a level bed, OUT_Q m^3/s into its first
+    column (as inflow inlets, which add q dt / A after each step as the reference's onStep does), its east
+    edge a water level (stageCells + setStage) or a free outfall; mean level per column and east outflow."""
+    ny = OUT_W
+    east = [j * nx + nx - 1 for j in range(ny)]
+    s = oracle.CPUSolver(nx=nx, ny=ny, dx=OUT_DX, z=np.zeros(nx * ny), manning=OUT_N,
+                         bc={"west": "wall", "south": "wall", "north": "wall", "east": "outflow"},
+                         stageCells=east if edge is not None else None, inlets=[j * nx for j in range(ny)])
+    if edge is not None:
+        s.setStage(edge)
+    s.setDepth(lambda X, Y, Z: (OUT_DOWN if edge is None else edge) + 0 * X)
+    s.setInletRates(np.full(ny, -OUT_Q / ny))
+    s.run(T)
+    H = s.field("h").reshape(ny, nx)
+    out = s.field("hu").reshape(ny, nx)[:, nx - 1].sum() * OUT_DX
+    return (lambda i: H[:, i].mean()), out, s
+
+
+def test_stage_boundary_outlet():
+    """run-outlet-tests.mjs:48-55: the full 1600 m river with its gauge level (3 m) at the east edge is
+    steady at the inflow; the closed-form backwater stage 800 m up (level bed, wide channel,
+    d^(13/3) = d0^(13/3) + 13/3 (q n)^2 L, :17-19) matches the full river there; the cut model held at
+    that stage keeps the river's level upstream; a free outfall draws it down."""
+    full, out, s = outlet_channel(160, OUT_DOWN)
+    L, q = 800, OUT_Q / (OUT_W * OUT_DX)
+    edge = (OUT_DOWN ** (13 / 3) + 13 / 3 * (q * OUT_N) ** 2 * L) ** (3 / 13)
+    cut, _, sc = outlet_channel(80, edge)
+    free, _, _ = outlet_channel(80, None)
+    print(f"\noutlet: out {out:.2f} m3/s, backwater {edge:.4f} vs full {full(80):.4f} m, cut {cut(20):.4f} vs full "
+          f"{full(20):.4f} m, free {free(20):.4f} m")
+    assert abs(out - OUT_Q) / OUT_Q < 0.02
+    assert abs(edge - full(80)) < 0.03
+    assert abs(cut(20) - full(20)) < 0.03
+    assert full(20) - free(20) > 0.5
+    assert abs(s.massBalance()["rel"]) < 1e-10 and abs(sc.massBalance()["rel"]) < 1e-10
+
+
+@pytest.mark.parametrize("side", ["wall", "outflow"])
+def test_stage_boundary_box(side):
+    """run-outlet-tests.mjs:58-80 (A19): a box across a floodplain 2 m above a channel, the east edge held
+    at 3 m: with the north and south sides closed the water stays at or above the held level everywhere;
+    with them free outfalls the floodplain drains below it."""
+    nx, ny, lvl = 60, 21, 3.0
+    J = np.repeat(np.arange(ny), nx)
+    z = np.where((J >= 8) & (J <= 12), 0.0, 2.0)
+    chan = [8, 9, 10, 11, 12]
+    s = oracle.CPUSolver(nx=nx, ny=ny, dx=OUT_DX, z=z, manning=OUT_N,
+                         bc={"west": "wall", "east": "outflow", "north": side, "south": side},
+                         stageCells=[j * nx + nx - 1 for j in range(ny)], inlets=[j * nx for j in chan])
+    s.setStage(lvl)
+    s.setDepth(lambda X, Y, Z: np.maximum(0, lvl - Z))
+    s.setInletRates(np.full(len(chan), -OUT_Q / len(chan)))
+    s.run(6000)
+    H = s.field("h").reshape(ny, nx)
+    E = H + z.reshape(ny, nx)
+    wet = H[:, :nx - 1] > 0.01
+    lo = E[:, :nx - 1][wet].min()
+    at_side = E[0, 30]
+    print(f"\nA19 sides {side}: lowest wet level {lo:.3f} m, at the side {at_side:.3f} m (held {lvl} m)")
+    if side == "wall":
+        assert lo >= lvl - 0.01
+    else:
+        assert at_side < lvl - 0.1
+    assert abs(s.massBalance()["rel"]) < 1e-10
+
+
+def stage_sea(opts_extra=None):
+    """gpu-suite.js:148-164: a 50 x 20 sloping shore (bed ~100 m) whose west edge opens to the sea (stage
+    cells), all edges free outfalls; still water at the stage for 300 s, then the stage rises 0.8 m over
+    900 s in 30 s frames."""
+    nx, ny, dx = 50, 20, 4
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = (98 + 0.08 * I + 0.3 * np.sin(J)).reshape(-1)
+    opts = dict(nx=nx, ny=ny, dx=dx, z=z, manning=0.02, bc={k: "outflow" for k in ("west", "east", "south", "north")},
+                stageCells=[j * nx for j in range(ny)])
+    opts.update(opts_extra or {})
+    return opts, 99.5
+
+
+def run_stage_sea(s, z0):
+    s.setStage(z0)
+    s.setDepth(lambda X, Y, Z: np.maximum(0, z0 - Z))
+    yield 300.0
+    for t in range(330, 1801, 30):
+        s.setStage(z0 + 0.8 * min(1, (t - 300) / 900))
+        yield float(t)
+
+
+def test_stage_boundary_sea(golden):
+    """Still water at the stage stays still in FP64 and nothing crosses the boundary; the rising stage then
+    floods in through the boundary only (the books close on the boundary flux), as much as the reference
+    GPU recorded for the same case (results-gpu.json `stage.inflow`, FP32: 5699.31 m^3)."""
+    opts, z0 = stage_sea()
+    s = oracle.CPUSolver(**opts, hdry=1e-5)
+    rest = None
+    for t in run_stage_sea(s, z0):
+        s.run(t)
+        if t == 300.0:
+            h, hu, hv = s.field("h"), s.field("hu"), s.field("hv")
+            wet = h > 1e-3
+            rest = (float(np.max(np.hypot(hu[wet], hv[wet]) / h[wet])), s.vol["outflow"])
+    inflow = -s.vol["outflow"]
+    print(f"\nstage sea: at rest max|u| {rest[0]:.1e} m/s, boundary flux {rest[1]:.1e} m3; rise: inflow {inflow:.2f} m3, "
+          f"mass rel {s.massBalance()['rel']:.1e}")
+    assert rest[0] < 1e-10 and abs(rest[1]) < 1e-8
+    assert abs(inflow - golden["gpu"]["stage"]["inflow"]) < 1e-4 * golden["gpu"]["stage"]["inflow"]
+    assert abs(s.massBalance()["rel"]) < 1e-10
+
+
 def test_symmetry_x_y():
     """A dam break along y is the transpose of the one along x (catches a swapped y-face operand)."""
     nx = 60

@@ -59,6 +59,17 @@ struct Slot {
   float *sh = nullptr, *shu = nullptr, *shv = nullptr, *sw = nullptr;
   float *g0h = nullptr, *g0hu = nullptr, *g0hv = nullptr, *g1h = nullptr, *g1hu = nullptr, *g1hv = nullptr;
   float4 *bsx = nullptr, *bsy = nullptr;
+  // flood barriers: one separate allocation (crest, bidx per cell; faces, ground, flux, books per barrier)
+  char *bmem = nullptr;
+  size_t bbytes = 0;
+  int nbar = 0;
+  float4 *crest = nullptr;
+  int2 *bidx = nullptr;
+  int4 *barFace = nullptr;
+  float *barGround = nullptr, *barFlux = nullptr;
+  double *barBook = nullptr;
+  std::vector<float4> hcrest;            // host copy of the crest plane
+  std::vector<int4> bars;                // per barrier: (i, j, axis, k of its right/upper cell)
   StepRec hrec{};                        // host mirror (refreshed after each launch group)
   double volInitial = 0.0;               // vol.initial (solver-gpu.js:309)
   double exchangedRead = 0.0;            // vol.exchanged: inlet volumes already read (solver-gpu.js:574)
@@ -74,6 +85,7 @@ struct Group {
   int nwork = 0, capWork = 0, capInst = 0;
   std::vector<DevInst> hinst;
   std::vector<int> withInlets;           // group indices of instances with inlets
+  std::vector<int> withBarriers;         // group indices of instances with flood barriers
   bool anyFirstOrder = false, anySecondOrder = false;
   bool anyHazard = false;                // an instance tracks the hazard products
   bool anyStepFlux = false;              // an instance takes the step-flux rules (F26)
@@ -243,6 +255,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
   G.slots = slots;
   G.hinst.clear();
   G.withInlets.clear();
+  G.withBarriers.clear();
   G.anyFirstOrder = G.anySecondOrder = false;
   G.anyHazard = false;
   G.anyStepFlux = false;
@@ -259,8 +272,10 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.nx = S.nx; d.ny = S.ny; d.pitch = S.pitch;
     d.flags = (S.infil ? FLAG_INFIL : 0) | (S.rf ? FLAG_RF : 0) | (S.p.implicitFriction ? FLAG_IMPF : 0) |
               (S.p.stepFlux && !S.L ? FLAG_STEPF : 0);   // the step flux applies under global stepping only
-    if (d.flags & FLAG_STEPF) G.anyStepFlux = true;
-    G.kinds |= 1u << ((S.hzA ? 2 : 0) + ((d.flags & FLAG_STEPF) ? 1 : 0));
+    // extended faces: the step flux or barrier crests (one kernel variant carries both)
+    const bool ext = (d.flags & FLAG_STEPF) || S.nbar;
+    if (ext) G.anyStepFlux = true;
+    G.kinds |= 1u << ((S.hzA ? 2 : 0) + (ext ? 1 : 0));
     d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;
     d.hfront = (float)S.p.hfront; d.cfl = (float)S.p.cfl; d.dtDry = (float)S.p.dtDry;
     d.area = (float)(S.p.dx * S.p.dx);
@@ -286,6 +301,9 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.g1h = S.g1h; d.g1hu = S.g1hu; d.g1hv = S.g1hv;
     d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
     d.umask = S.umask; d.ucells = S.ucells; d.ulist = S.ulist;
+    d.crest = S.crest; d.bidx = S.bidx; d.barFace = S.barFace; d.barGround = S.barGround;
+    d.barFlux = S.barFlux; d.barBook = S.barBook; d.nbar = S.nbar;
+    if (S.nbar) G.withBarriers.push_back((int)gi);
     if (S.L) G.lts = true;
     if (S.hzA) G.anyHazard = true;
     if (S.nin) G.withInlets.push_back((int)gi);
@@ -353,6 +371,8 @@ int launch_step(fs_pool *P) {
   const unsigned K = G.kinds;
   if (K & 0x5u) sw_stage<0, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
   if (K & 0xau) sw_stage<0, false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
+  for (int gi : G.withBarriers)
+    sw_barrier_books<0, false><<<(G.hinst[gi].nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
   if (P->timing) HIPCHK(P, hipEventRecord(e1, P->stream));
   // each instance runs its own scheme: the kernel of the other order returns at once for it
   if (G.anySecondOrder) {
@@ -360,6 +380,8 @@ int launch_step(fs_pool *P) {
     if (K & 0x2u) sw_stage<1, false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
     if (K & 0x4u) sw_stage<1, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
     if (K & 0x8u) sw_stage<1, false, true, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    for (int gi : G.withBarriers)
+      sw_barrier_books<1, false><<<(G.hinst[gi].nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
   }
   if (P->timing) P->spans.push_back({e1, nullptr, -1, -1});  // completed below with e2
   if (G.anyFirstOrder) {
@@ -455,7 +477,12 @@ int launch_cycle(fs_pool *P) {
   for (int k = 0; k < Pn; ++k)
     for (int m = 0; m <= L; ++m) {
       if ((k + 1) % (1 << m)) continue;
-      sw_stage<0, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
+      if (d.nbar) {
+        sw_stage<0, true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
+        sw_barrier_books<0, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
+      } else {
+        sw_stage<0, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
+      }
       hipEvent_t sa = nullptr, sb = nullptr;
       if (P->timing) {
         sa = span_event(P); sb = span_event(P);
@@ -465,8 +492,15 @@ int launch_cycle(fs_pool *P) {
       // the record's base step rides in the level argument's upper bits (its end time: hazard clock)
       const int mk = m | (k << 8);
       if (d.secondOrder) {
-        if (d.hzA) sw_stage<1, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
-        else sw_stage<1, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        if (d.nbar) {
+          if (d.hzA) sw_stage<1, true, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          else sw_stage<1, true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          sw_barrier_books<1, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
+        } else if (d.hzA) {
+          sw_stage<1, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        } else {
+          sw_stage<1, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        }
       } else {
         if (d.hzA) sw_euler_finish<true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
         else sw_euler_finish<true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
@@ -943,8 +977,10 @@ void fs_pool_destroy(fs_pool_t P) {
   if (!P) return;
   (void)hipStreamSynchronize(P->stream);
   drop_graph(P->grp);
-  for (auto &S : P->slots)
+  for (auto &S : P->slots) {
     if (S.used && S.mem) (void)hipFree(S.mem);
+    if (S.used && S.bmem) (void)hipFree(S.bmem);
+  }
   if (P->grp.dinst) (void)hipFree(P->grp.dinst);
   if (P->grp.dwork) (void)hipFree(P->grp.dwork);
   if (P->grp.p1) (void)hipFree(P->grp.p1);
@@ -1002,7 +1038,8 @@ int fs_pool_remove(fs_pool_t P, int32_t slot) {
   Slot &S = P->slots[slot];
   HIPCHK(P, hipStreamSynchronize(P->stream));
   HIPCHK(P, hipFree(S.mem));
-  P->used -= S.bytes;
+  if (S.bmem) HIPCHK(P, hipFree(S.bmem));
+  P->used -= S.bytes + S.bbytes;
   S = Slot();
   P->grp.slots.clear();
   drop_graph(P->grp);
@@ -1051,6 +1088,108 @@ int fs_pool_set_stage(fs_pool_t P, int32_t slot, double z) {
   HIPCHK(P, hipMemcpyAsync(&P->drec[slot].stage, &st, sizeof st, hipMemcpyHostToDevice, P->stream));
   HIPCHK(P, hipStreamSynchronize(P->stream));
   S.hrec.stage = st;
+  return FS_OK;
+}
+
+// flood barriers (o.crest, solver-gpu.js:18-20, 144-151): faces (i, j, axis) between two interior cells,
+// crest z (absolute; non-finite = not deployed), weir coefficient C, ground under the barrier (absolute;
+// non-finite = unknown, no load). A new list replaces the old one and starts its books afresh.
+int fs_pool_set_crests(fs_pool_t P, int32_t slot, const int32_t *fi, const int32_t *fj, const int32_t *axis,
+                       const double *z, const double *C, const double *ground, int32_t n) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (n < 0 || (n && (!fi || !fj || !axis || !z || !C || !ground))) return fail(P, FS_E_ARG, "barriers: bad lists");
+  for (int m = 0; m < n; ++m) {
+    const bool ax = axis[m] != 0;
+    if (fi[m] < (ax ? 0 : 1) || fi[m] >= S.nx || fj[m] < (ax ? 1 : 0) || fj[m] >= S.ny)
+      return fail(P, FS_E_ARG, "barrier %d: face (%d, %d, axis %d) is not between two interior cells", m, fi[m], fj[m],
+                  axis[m]);
+  }
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  if (S.bmem) {
+    HIPCHK(P, hipFree(S.bmem));
+    P->used -= S.bbytes;
+    S.bmem = nullptr; S.bbytes = 0; S.nbar = 0;
+    S.crest = nullptr; S.bidx = nullptr; S.barFace = nullptr; S.barGround = nullptr; S.barFlux = nullptr;
+    S.barBook = nullptr;
+    S.hcrest.clear(); S.bars.clear();
+  }
+  P->grp.slots.clear();   // the instance descriptors change
+  drop_graph(P->grp);
+  if (!n) return FS_OK;
+  const size_t N = (size_t)S.nx * S.ny;
+  const size_t b = align256(N * 16) + align256(N * 8) + align256((size_t)n * 16) + 2 * align256((size_t)n * 4) +
+                   align256((size_t)n * 72);
+  if (P->budget && P->used + b > P->budget) return fail(P, FS_E_BUDGET, "barriers: HBM budget spent");
+  HIPCHK(P, hipMalloc(&S.bmem, b));
+  S.bbytes = b;
+  P->used += b;
+  char *q = S.bmem;
+  auto carve = [&](size_t bytes) { char *r = q; q += align256(bytes); return r; };
+  S.crest = (float4 *)carve(N * 16);
+  S.bidx = (int2 *)carve(N * 8);
+  S.barFace = (int4 *)carve((size_t)n * 16);
+  S.barGround = (float *)carve((size_t)n * 4);
+  S.barFlux = (float *)carve((size_t)n * 4);
+  S.barBook = (double *)carve((size_t)n * 72);
+  S.nbar = n;
+  S.hcrest.assign(N, make_float4(-1e30f, 0.f, -1e30f, 0.f));
+  std::vector<int2> bi(N, make_int2(-1, -1));
+  std::vector<int4> bf(n);
+  std::vector<float> gr(n);
+  std::vector<double> book((size_t)n * 9, 0.0);
+  S.bars.resize(n);
+  for (int m = 0; m < n; ++m) {
+    const int k = fj[m] * S.pitch + fi[m];
+    const bool ax = axis[m] != 0;
+    S.bars[m] = make_int4(fi[m], fj[m], ax ? 1 : 0, k);
+    bf[m] = make_int4(k, k - (ax ? S.pitch : 1), ax ? 1 : 0, 0);
+    gr[m] = std::isfinite(ground[m]) ? (float)(ground[m] - S.zref) : -1e30f;
+    if (ax) { bi[k].y = m; S.hcrest[k].w = (float)C[m]; } else { bi[k].x = m; S.hcrest[k].y = (float)C[m]; }
+    book[9 * (size_t)m + 3] = -1e30;   // level: never wet
+    book[9 * (size_t)m + 8] = -1.0;    // first overtopping: never
+  }
+  HIPCHK(P, hipMemcpy(S.bidx, bi.data(), N * 8, hipMemcpyHostToDevice));
+  HIPCHK(P, hipMemcpy(S.barFace, bf.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+  HIPCHK(P, hipMemcpy(S.barGround, gr.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  HIPCHK(P, hipMemset(S.barFlux, 0, (size_t)n * 4));
+  HIPCHK(P, hipMemcpy(S.barBook, book.data(), (size_t)n * 72, hipMemcpyHostToDevice));
+  return fs_pool_set_crest_levels(P, slot, z);
+}
+
+// setCrestLevels (solver-gpu.js:486-488): new crest per barrier face, in the order given
+int fs_pool_set_crest_levels(fs_pool_t P, int32_t slot, const double *z) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (!S.nbar) return FS_OK;
+  if (!z) return fail(P, FS_E_ARG, "crest levels: NULL");
+  for (int m = 0; m < S.nbar; ++m) {
+    const float c = std::isfinite(z[m]) ? (float)(z[m] - S.zref) : -1e30f;
+    float4 &cr = S.hcrest[S.bars[m].w];
+    if (S.bars[m].z) cr.z = c; else cr.x = c;
+  }
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  HIPCHK(P, hipMemcpy(S.crest, S.hcrest.data(), S.hcrest.size() * 16, hipMemcpyHostToDevice));
+  return FS_OK;
+}
+
+// readBarriers (solver-gpu.js:490-504): 9 per face: plus, minus (m^3), peak (m^2/s), level (m, absolute;
+// -inf = never wet), load (kN/m), tPeak, tLevel, tLoad, tOver (s; NaN = never overtopped)
+int fs_pool_read_barriers(fs_pool_t P, int32_t slot, double *out) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (!S.nbar) return FS_OK;
+  if (!out) return fail(P, FS_E_ARG, "read barriers: NULL");
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  HIPCHK(P, hipMemcpy(out, S.barBook, (size_t)S.nbar * 72, hipMemcpyDeviceToHost));
+  for (int m = 0; m < S.nbar; ++m) {
+    double *b = out + 9 * (size_t)m;
+    b[3] = b[3] > -1e29 ? b[3] + S.zref : -INFINITY;
+    if (b[8] < 0.0) b[8] = NAN;
+  }
   return FS_OK;
 }
 
@@ -1250,6 +1389,14 @@ int fs_set_stage_cells(fs_handle s, const int32_t *cells, int32_t n) {
   return s ? fs_pool_set_stage_cells(s->pool, s->slot, cells, n) : FS_E_ARG;
 }
 int fs_set_stage(fs_handle s, double z) { return s ? fs_pool_set_stage(s->pool, s->slot, z) : FS_E_ARG; }
+int fs_set_crests(fs_handle s, const int32_t *i, const int32_t *j, const int32_t *axis, const double *z,
+                  const double *C, const double *ground, int32_t n) {
+  return s ? fs_pool_set_crests(s->pool, s->slot, i, j, axis, z, C, ground, n) : FS_E_ARG;
+}
+int fs_set_crest_levels(fs_handle s, const double *z) {
+  return s ? fs_pool_set_crest_levels(s->pool, s->slot, z) : FS_E_ARG;
+}
+int fs_read_barriers(fs_handle s, double *out) { return s ? fs_pool_read_barriers(s->pool, s->slot, out) : FS_E_ARG; }
 
 int fs_set_rain_field(fs_handle s, const double *field) {
   if (!s || !field) return FS_E_ARG;

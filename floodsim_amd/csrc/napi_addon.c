@@ -336,6 +336,57 @@ static napi_value js_set_stage(napi_env env, napi_callback_info info) {
   return rc ? fail(env, s->h, rc) : undefined_value(env);
 }
 
+/* setCrests(h, Int32Array i, j, axis, Float64Array z, C, ground) -> fs_set_crests;
+ * setCrestLevels(h, Float64Array z) -> fs_set_crest_levels; readBarriers(h, n) -> Float64Array(9 n) */
+static napi_value js_set_crests(napi_env env, napi_callback_info info) {
+  size_t argc = 7;
+  napi_value a[7];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  void *fi, *fj, *ax, *z, *c, *g;
+  size_t n = 0, n2 = 0;
+  if (typed(env, a[1], napi_int32_array, -1, &fi, &n, "crest i")) return NULL;
+  if (typed(env, a[2], napi_int32_array, (long)n, &fj, &n2, "crest j")) return NULL;
+  if (typed(env, a[3], napi_int32_array, (long)n, &ax, &n2, "crest axis")) return NULL;
+  if (typed(env, a[4], napi_float64_array, (long)n, &z, &n2, "crest z")) return NULL;
+  if (typed(env, a[5], napi_float64_array, (long)n, &c, &n2, "crest C")) return NULL;
+  if (typed(env, a[6], napi_float64_array, (long)n, &g, &n2, "crest ground")) return NULL;
+  int rc = fs_set_crests(s->h, (const int32_t *)fi, (const int32_t *)fj, (const int32_t *)ax, (const double *)z,
+                         (const double *)c, (const double *)g, (int32_t)n);
+  return rc ? fail(env, s->h, rc) : undefined_value(env);
+}
+
+static napi_value js_set_crest_levels(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  void *z;
+  size_t n = 0;
+  if (typed(env, a[1], napi_float64_array, -1, &z, &n, "crest levels")) return NULL;
+  int rc = fs_set_crest_levels(s->h, (const double *)z);
+  return rc ? fail(env, s->h, rc) : undefined_value(env);
+}
+
+static napi_value js_read_barriers(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  int32_t n = 0;
+  napi_get_value_int32(env, a[1], &n);
+  if (n < 0) n = 0;
+  double *out = (double *)calloc((size_t)(n ? 9 * n : 1), sizeof(double));
+  if (!out) { napi_throw_error(env, NULL, "readBarriers: out of memory"); return NULL; }
+  int rc = fs_read_barriers(s->h, out);
+  napi_value r = rc ? NULL : make_f64(env, out, (size_t)9 * n);
+  free(out);
+  return rc ? fail(env, s->h, rc) : r;
+}
+
 static napi_value js_clock(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
@@ -745,6 +796,9 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setInletRates", NULL, js_set_inlet_rates, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setStageCells", NULL, js_set_stage_cells, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setStage", NULL, js_set_stage, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setCrests", NULL, js_set_crests, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setCrestLevels", NULL, js_set_crest_levels, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readBarriers", NULL, js_read_barriers, NULL, NULL, NULL, napi_enumerable, NULL},
       {"clock", NULL, js_clock, NULL, NULL, NULL, napi_enumerable, NULL},
       {"tileStats", NULL, js_tile_stats, NULL, NULL, NULL, napi_enumerable, NULL},
       {"levels", NULL, js_levels, NULL, NULL, NULL, napi_enumerable, NULL},

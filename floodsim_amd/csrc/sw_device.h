@@ -35,6 +35,11 @@ __device__ __forceinline__ float4 gld(const float4 *row, int x) {
   const fs_v4 v = *(const FS_GLOBAL fs_v4 *)((const FS_GLOBAL char *)row + (uint32_t)x * 16u);
   return make_float4(v.x, v.y, v.z, v.w);
 }
+typedef int fs_i2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int2 gld(const int2 *row, int x) {
+  const fs_i2 v = *(const FS_GLOBAL fs_i2 *)((const FS_GLOBAL char *)row + (uint32_t)x * 8u);
+  return make_int2(v.x, v.y);
+}
 __device__ __forceinline__ void gst(float2 *row, int x, float2 v) {
   const fs_v2 t = {v.x, v.y};
   *(FS_GLOBAL fs_v2 *)((FS_GLOBAL char *)row + (uint32_t)x * 8u) = t;
@@ -134,6 +139,18 @@ struct DevInst {
   uint8_t *umask;               // per work unit: levels it takes part in
   int32_t *ucells;              // per work unit and level: own open cells
   int32_t *ulist;               // per level (stride wend - wbeg): work units of the level's records
+  // flood barriers (null: none). Per cell: crest and weir coefficient of its west face (x) and south face
+  // (y), crest relative to the bed minimum (-1e30 = none); the barrier index of those faces (-1 = none).
+  // Per barrier: (cell k of its right/upper side, k of the other side, axis 0 = x face / 1 = y face, 0),
+  // the ground under it (-1e30 = unknown), the theta-scaled mass flux of the last stage, and its books
+  // (plus, minus m^3; peak m^2/s; level m; load kN/m; their times and the first overtopping, s)
+  float4 *crest;
+  int2 *bidx;
+  const int4 *barFace;
+  const float *barGround;
+  float *barFlux;
+  double *barBook;
+  int nbar;
 };
 
 // per wavefront strip and stage: boundary outflow (m^3), clipped (m^3), infiltration (m^3), max

@@ -89,41 +89,79 @@ __device__ __forceinline__ void hllc(float hL, float uL, float vL, float hR, flo
 // kernels.wgsl.js:233-263). L and R are the reconstructed (h, eta - z_cell, un, ut) of the two
 // cells; dz = z_R - z_L of the cell beds. The face beds are z_cell + (e - h); only their
 // difference delta enters: h*_L = max(0, hL - max(0, delta)), h*_R = max(0, hR - max(0, -delta)).
-struct Face { float F0, F1, F2, cL, cR; };
+struct Face { float F0, F1, F2, cL, cR, hLs, hRs; };
 
-// SF: the step-flux rule (F26, solver-cpu.js:250-273; kernels.wgsl.js:247-257), for faces between two
-// open cells with friction (RL, RR: Manning n > 0): a wet film whose reconstructed depth over the other
-// side's higher face bed is zero passes its own discharge upwind, and the lower cell feels the whole step
-template <bool SF = false>
+// EXT (extended faces), for instances with the step flux or flood barriers:
+// - a barrier crest on the face (dcL = crest - z_L cell bed; -1e30 = none) joins the face beds in the
+//   hydrostatic reconstruction, z* = max(zL, zR, crest) (solver-cpu.js:244-246; kernels.wgsl.js:240-242);
+// - the step-flux rule (F26, solver-cpu.js:250-273; kernels.wgsl.js:247-257) for faces between two open
+//   cells with friction (RL, RR: Manning n > 0; both false on a weir face): a wet film whose reconstructed
+//   depth over the other side's higher face bed is zero passes its own discharge upwind, and the lower cell
+//   feels the whole step
+template <bool EXT = false>
 __device__ __forceinline__ Face face(float hL, float eL, float uL, float vL, bool ML, float hR, float eR, float uR,
                                      float vR, bool MR, float dz, float g, float hdry, bool RL = false,
-                                     bool RR = false) {
+                                     bool RR = false, float dcL = -1e30f) {
+  const float dcRel = ML ? dcL : dcL - dz;   // crest over the bed of the cell the left state comes from
   if (!MR) { hR = hL; eR = eL; uR = -uL; vR = vL; }
   if (!ML) { hL = hR; eL = eR; uL = -uR; vL = vR; }
   const float delta = (ML && MR) ? (dz + (eR - hR)) - (eL - hL) : 0.f;
-  const float hLs = fmaxf(0.f, hL - fmaxf(0.f, delta));
-  const float hRs = fmaxf(0.f, hR - fmaxf(0.f, -delta));
+  // z* above the left and right face beds
+  float sL = fmaxf(0.f, delta), sR = fmaxf(0.f, -delta);
+  if constexpr (EXT) {
+    const float cOver = dcRel - (eL - hL);   // crest above the left face bed
+    sL = fmaxf(sL, cOver);
+    sR = fmaxf(sR, cOver - delta);
+  }
+  const float hLs = fmaxf(0.f, hL - sL);
+  const float hRs = fmaxf(0.f, hR - sR);
   Face f;
   hllc(hLs, uL, vL, hRs, uR, vR, g, hdry, f.F0, f.F1, f.F2);
   const float hg = 0.5f * g;
   f.cL = ML ? hg * (hL * hL - hLs * hLs) : 0.f;
   f.cR = MR ? hg * (hR * hR - hRs * hRs) : 0.f;
-  if constexpr (SF) {
+  f.hLs = hLs;
+  f.hRs = hRs;
+  if constexpr (EXT) {
     if (ML && MR && RL && RR) {
-      // the face beds differ by delta; the surface of the lower side lies max(0, -+delta) - h under the step
+      // the surface of the lower side lies z* - (its face bed) - h under the step top
       if (hLs > hdry && hRs <= 0.f && hR > hdry) {
         const float u = fmaxf(uL, 0.f), q = hLs * u;
         f.F0 = q; f.F1 = q * u + hg * hLs * hLs; f.F2 = q * vL;
-        f.cR = hg * hR * hR + g * hR * (fmaxf(0.f, -delta) - hR);
+        f.cR = hg * hR * hR + g * hR * (sR - hR);
       } else if (hRs > hdry && hLs <= 0.f && hL > hdry) {
         const float u = fminf(uR, 0.f), q = hRs * u;
         f.F0 = q; f.F1 = q * u + hg * hRs * hRs; f.F2 = q * vR;
-        f.cL = hg * hL * hL + g * hL * (fmaxf(0.f, delta) - hL);
+        f.cL = hg * hL * hL + g * hL * (sL - hL);
       }
     }
   }
   if (!ML && !MR) { f.F0 = f.F1 = f.F2 = f.cL = f.cR = 0.f; }
   return f;
+}
+
+// Flow over a barrier crest zc whose crest stands above both cell beds (the caller checks): broad-crested
+// weir q = C H1^1.5 with Villemonte submergence (1 - (H2/H1)^1.5)^0.385, capped so one step cannot more than
+// half-close the level difference; the momentum carries the critical velocity over the crest, the pressure
+// is the mean of the two sides' depths over the crest (weir, kernels.wgsl.js:212-230; _weir,
+// solver-cpu.js:277-287). Cell-centred surfaces eA, eB (relative to the bed minimum), depths, tangential
+// velocities; overwrites the face's HLLC flux.
+__device__ __forceinline__ void weir_flux(Face &f, float eA, float hA, float tA, float eB, float hB, float tB, float zc,
+                                          float cw, float g, float hdry, float dx, float dt) {
+  const bool left = eA >= eB;
+  const float eU = left ? eA : eB, hU = left ? hA : hB, tU = left ? tA : tB, eD = left ? eB : eA;
+  const float pres = 0.25f * g * (f.hLs * f.hLs + f.hRs * f.hRs);
+  const float H1 = eU - zc;
+  f.F0 = 0.f; f.F1 = pres; f.F2 = 0.f;
+  if (H1 <= hdry || hU <= hdry) return;
+  const float r = fmaxf(0.f, eD - zc) / H1;
+  if (r >= 1.f) return;
+  float q = cw * fpow(H1, 1.5f) * fpow(1.f - fpow(fmaxf(r, 1e-30f), 1.5f), 0.385f);
+  if (r <= 0.f) q = cw * fpow(H1, 1.5f);
+  if (dt > 0.f) q = fminf(q, (eU - eD) * dx / (4.f * dt));
+  const float un = q / fmaxf(2.f / 3.f * H1, hdry);
+  const float F0 = left ? q : -q;
+  f.F0 = F0; f.F1 = q * un + pres; f.F2 = F0 * tU;
 }
 
 // Kahan-compensated depth: stored h with residual w, true depth h - w (kernels.wgsl.js:265-272)
@@ -444,12 +482,14 @@ struct FaceRow { float F0, F1, F2, cL, cR, on; };
 // store; cells at m + 1 wait and gather the RK-weighted flux of those faces; the bed-slope term of a
 // stepping cell takes, for a face whose neighbour is finer, its reconstruction from that face's last
 // evaluation (kept in bsx/bsy by the finer record's last stage).
-// SF: the step-flux rules (F26) for instances that take them (global stepping only)
-template <int STAGE, bool LTS, bool HZ, bool SF = false>
+// EXT: extended faces, for instances with the step-flux rules (F26; global stepping only) or flood
+// barriers (crests in the face beds, weir faces, the barrier faces' fluxes for their books)
+template <int STAGE, bool LTS, bool HZ, bool EXT = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
                                           Part *__restrict__ part, const int cells, const float tnow) {
-  static_assert(!(SF && LTS), "the step flux applies under global stepping only");
-  const bool sf = SF && (I.flags & FLAG_STEPF);
+  constexpr bool SF = EXT;
+  const bool sf = EXT && (I.flags & FLAG_STEPF);
+  const bool bar = EXT && I.crest != nullptr;
   const float rain = I.rec->rainRate;
   const float g = I.g, hdry = I.hdry, hfront = I.hfront, inv_dx = I.inv_dx, dx = I.dx;
   const int so = I.secondOrder, vl = I.limiter;
@@ -475,7 +515,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   // ---- register window
   float h1r = 0, u1r = 0, v1r = 0, z1r = 0, h2r = 0, u2r = 0, v2r = 0, z2r = 0;
   bool M1r = false, M2r = false;
-  bool R1r = false, R2r = false;   // SF: Manning n > 0 in rows r-1, r-2
+  bool R1r = false, R2r = false;   // EXT: Manning n > 0 in rows r-1, r-2
+  float cy1 = -1e30f, wy1 = 0.f;   // EXT: crest and weir coefficient of row r-1's south faces
   int lv1 = 0, lv2 = 0, lv3 = 0;
   float hu1 = 0, hv1 = 0, hu2 = 0, hv2 = 0, hu3 = 0, hv3 = 0, hh3 = 0;
   bool M3 = false, in1 = false, in2 = false, in3 = false;
@@ -502,6 +543,11 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     const float hr = from_right(p.h), ur = from_right(p.u), vr = from_right(p.v), zr = from_right(p.z);
     const bool Ml = from_left_i(p.M) != 0, Mr = from_right_i(p.M) != 0;
     const bool Rr = SF ? from_right_i(p.R) != 0 : false;
+    // barrier crests of this row: (west face crest, coefficient, south face crest, coefficient)
+    float4 cr = make_float4(-1e30f, 0.f, -1e30f, 0.f);
+    if constexpr (EXT) {
+      if (bar && r >= 0 && r < ny && X >= 0 && X < nx) cr = gld(I.crest, r * pitch + X);
+    }
     const Slopes sxs = muscl<SF>(p.h, p.u, p.v, hl, ul, vlf, Ml, p.z - zl, hr, ur, vr, Mr, zr - p.z, so, hfront, vl,
                                  sf && p.R);
     const float xh_p = p.h + 0.5f * sxs.sh, xe_p = p.h + 0.5f * sxs.se;
@@ -515,7 +561,18 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     int lvr = 0;
     {
       const float Rh = from_right(xh_m), Re = from_right(xe_m), Ru = from_right(xu_m), Rv = from_right(xv_m);
-      const Face f = face<SF>(xh_p, xe_p, xu_p, xv_p, p.M, Rh, Re, Ru, Rv, Mr, zr - p.z, g, hdry, sf && p.R, Rr);
+      float zcx = -1e30f, cwx = 0.f;   // crest of this face (held by lane l + 1 as its west face)
+      bool weirX = false;
+      if constexpr (EXT) {
+        zcx = from_right(cr.x);
+        cwx = from_right(cr.y);
+        weirX = bar && p.M && Mr && zcx > fmaxf(p.z, zr);
+      }
+      Face f = face<EXT>(xh_p, xe_p, xu_p, xv_p, p.M, Rh, Re, Ru, Rv, Mr, zr - p.z, g, hdry, sf && p.R && !weirX, Rr,
+                         zcx - p.z);
+      if constexpr (EXT) {
+        if (weirX) weir_flux(f, p.h + p.z, p.h, p.v, hr + zr, hr, vr, zcx, cwx, g, hdry, dx, dt);
+      }
       xf0 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
       if constexpr (LTS) {
         lvr = from_right_i(p.lv);
@@ -556,8 +613,13 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     // y face (r-2 | r-1): L = row r-2 toward north, R = row r-1 toward south
     FaceRow yf2;
     {
-      const Face f = face<SF>(yp_h, yp_e, yp_n, yp_t, M2r, yh_m, ye_m, yn_m, yt_m, M1r, z1r - z2r, g, hdry, sf && R2r,
-                              R1r);
+      bool weirY = false;
+      if constexpr (EXT) weirY = bar && M2r && M1r && cy1 > fmaxf(z2r, z1r);
+      Face f = face<EXT>(yp_h, yp_e, yp_n, yp_t, M2r, yh_m, ye_m, yn_m, yt_m, M1r, z1r - z2r, g, hdry,
+                         sf && R2r && !weirY, R1r, cy1 - z2r);
+      if constexpr (EXT) {
+        if (weirY) weir_flux(f, h2r + z2r, h2r, u2r, h1r + z1r, h1r, u1r, cy1, wy1, g, hdry, dx, dt);
+      }
       yf2 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
       if constexpr (LTS) yf2.on = min(lv2, lv1) == m ? 1.f : 0.f;
     }
@@ -626,6 +688,15 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       Fe0 *= te; Fe1 *= te; Fe2 *= te;
       Fs0 *= ts; Fs1 *= ts; Fs2 *= ts;
       Fn0 *= tn; Fn1 *= tn; Fn2 *= tn;
+      if constexpr (EXT) {
+        // a barrier on this cell's west or south face: its theta-scaled mass flux for the books (one lane
+        // owns the face, so a plain store)
+        if (bar && own_col && ry >= y0 && ry < y1 && in3) {
+          const int2 bi = gld(I.bidx, ry * pitch + X);
+          if (bi.x >= 0) I.barFlux[bi.x] = w0;
+          if (bi.y >= 0) I.barFlux[bi.y] = Fs0;
+        }
+      }
       bool upd = own_col && ry >= y0 && ry < y1 && in3 && M3;
       bool own = true, coarse = false;
       if constexpr (LTS) {
@@ -730,7 +801,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     hh3 = h2r; hu3 = hu2; hv3 = hv2; M3 = M2r; in3 = in2;
     h2r = h1r; u2r = u1r; v2r = v1r; z2r = z1r; M2r = M1r; hu2 = hu1; hv2 = hv1; in2 = in1;
     h1r = p.h; u1r = p.u; v1r = p.v; z1r = p.z; M1r = p.M; hu1 = p.hu; hv1 = p.hv; in1 = p.in;
-    if constexpr (SF) { R2r = R1r; R1r = p.R; }
+    if constexpr (EXT) { R2r = R1r; R1r = p.R; cy1 = cr.z; wy1 = cr.w; }
   }
 
   const double o = wave_sum(acc_out), c = wave_sum(acc_clip), f = wave_sum(acc_inf);
@@ -829,7 +900,7 @@ __device__ __forceinline__ float lts_record_end_time(const DevInst &I, int kk) {
 
 // LTS record of level m (single-instance groups): the parts of the level's units (lts_split), whose
 // number only the device knows; the grid covers kLtsWaveCap per unit and waves without a part exit.
-template <int STAGE, bool HZ>
+template <int STAGE, bool HZ, bool EXT = false>
 __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                Part *__restrict__ part, int m) {
   const int lane = threadIdx.x & 63;
@@ -850,15 +921,17 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
     if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0};
     return;
   }
-  run_strip<STAGE, true, HZ>(I, wk, lane, m, dt, part + i, 0, lts_record_end_time(I, kk));
+  run_strip<STAGE, true, HZ, EXT>(I, wk, lane, m, dt, part + i, 0, lts_record_end_time(I, kk));
 }
 
 // One RK stage over a work list. Global stepping: one wavefront per strip of `work`; a strip the
 // dry-strip rule leaves out writes an empty partial. Each kernel variant advances only the instances
-// of its kind (hazard products: second stage only; step flux) and leaves the others' strips alone, so a
-// group of mixed kinds runs one launch per kind present and every instance gets the same machine code,
-// hence the same bits, alone and in any mix.
-template <int STAGE, bool LTS, bool HZ = false, bool SF = false>
+// of its kind (hazard products: second stage only; extended faces: step flux or barriers) and leaves the
+// others' strips alone, so a group of mixed kinds runs one launch per kind present and every instance gets
+// the same machine code, hence the same bits, alone and in any mix.
+__device__ __forceinline__ bool ext_faces(const DevInst &I) { return (I.flags & FLAG_STEPF) || I.crest; }
+
+template <int STAGE, bool LTS, bool HZ = false, bool EXT = false>
 __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                 int nwork, Part *__restrict__ part, int m) {
   static_assert(!LTS, "local time stepping records use the specialisations below");
@@ -867,7 +940,7 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
   if (wid >= nwork) return;
   const int4 wk = work[wid];
   const DevInst &I = insts[wk.x];
-  if (((I.flags & FLAG_STEPF) != 0) != SF) return;
+  if (ext_faces(I) != EXT) return;
   if (STAGE == 1 && (I.hzA != nullptr) != HZ) return;
   if (STAGE == 1 && !I.secondOrder) return;  // first order: sw_euler_finish writes this strip's partial
   const float dt = I.rec->dt;
@@ -878,7 +951,7 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
     return;
   }
-  const bool stir = run_strip<STAGE, false, HZ, SF>(I, wk, lane, 0, dt, part + wid, I.sopen[u], (float)I.rec->tAfter);
+  const bool stir = run_strip<STAGE, false, HZ, EXT>(I, wk, lane, 0, dt, part + wid, I.sopen[u], (float)I.rec->tAfter);
   if (STAGE == 1 && I.wet && lane == 0) I.wet[u] = stir ? 1 : 0;
 }
 
@@ -909,6 +982,79 @@ __global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, true>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
   (void)nwork;
   lts_stage_body<1, true>(insts, work, part, m);
+}
+// with flood barriers (extended faces)
+template <>
+__global__ __launch_bounds__(256, FS_LTS_S1_WAVES) void sw_stage<0, true, false, true>(
+    const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
+  (void)nwork;
+  lts_stage_body<0, false, true>(insts, work, part, m);
+}
+template <>
+__global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, false, true>(
+    const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
+  (void)nwork;
+  lts_stage_body<1, false, true>(insts, work, part, m);
+}
+template <>
+__global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, true, true>(
+    const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
+  (void)nwork;
+  lts_stage_body<1, true, true>(insts, work, part, m);
+}
+
+// Barrier books after each RK stage, one thread per barrier face (the barrier kernel,
+// kernels.wgsl.js:366-412; solver-cpu.js:327-343): the theta-scaled flux the face passed this stage is
+// booked while the barrier is up and water stands 2 cm over its crest; in the first stage the peak rate,
+// the highest water and the hydrostatic load on the barrier, each with the step's start time (under local
+// time stepping: a face of the level stepping now, timed at the cycle start). The evaluated state is U^n
+// in the first stage and U* (the stepping level's cells) in the second.
+template <int STAGE, bool LTS>
+__global__ __launch_bounds__(64) void sw_barrier_books(const DevInst *__restrict__ insts, int inst, int m) {
+  const DevInst &I = insts[inst];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= I.nbar) return;
+  if (STAGE == 1 && !I.secondOrder) return;
+  m &= 0xff;
+  const StepRec *rec = I.rec;
+  const float dt = LTS ? rec->ldt[m] : rec->dt;
+  if (!(dt > 0.f)) return;
+  const double tnow = LTS ? rec->cycT0 : rec->t;
+  const int4 bf = I.barFace[q];
+  const int kb = bf.x, ka = bf.y;
+  bool Ba = STAGE == 1, Bb = STAGE == 1;
+  if constexpr (LTS) {
+    const int la = tile_level(I, ka % I.pitch, ka / I.pitch), lb = tile_level(I, kb % I.pitch, kb / I.pitch);
+    if (min(la, lb) != m) return;
+    Ba = STAGE == 1 && la == m;
+    Bb = STAGE == 1 && lb == m;
+  }
+  const float ha = Ba ? I.h1[ka] : I.h[ka], hb = Bb ? I.h1[kb] : I.h[kb];
+  if (ha <= 0.f && hb <= 0.f) return;
+  const float4 cr = I.crest[kb];
+  const float crest = bf.z ? cr.z : cr.x;
+  if (crest < -1e29f) return;
+  const float ea = ha > 0.005f ? ha + I.z[ka] : -1e30f, eb = hb > 0.005f ? hb + I.z[kb] : -1e30f;
+  const float w = fmaxf(ea, eb);
+  const float f = I.barFlux[q];
+  const bool over = w > crest + 0.02f;
+  double *B = I.barBook + 9 * (size_t)q;
+  if (over) {
+    const double v = (double)f * (double)I.dx * (double)dt * (I.secondOrder ? 0.5 : 1.0);
+    if (v > 0.0) B[0] += v;
+    else B[1] -= v;
+    if (B[8] < 0.0 && v != 0.0) B[8] = tnow;
+  }
+  if (STAGE == 0) {
+    if (over && fabsf(f) > B[2]) { B[2] = fabsf(f); B[5] = tnow; }
+    if (w > B[3]) { B[3] = w; B[6] = tnow; }
+    const float gr = I.barGround[q];
+    if (gr > -1e29f) {
+      const float dA = fmaxf(0.f, fminf(ea, crest) - gr), dB = fmaxf(0.f, fminf(eb, crest) - gr);
+      const double load = 0.5 * (double)I.g * fabs((double)dA * dA - (double)dB * dB);
+      if (load > B[4]) { B[4] = load; B[7] = tnow; }
+    }
+  }
 }
 
 // ------------------------------------------------------------ local time stepping, per record

@@ -13,10 +13,11 @@
  * - constructor options nx, ny, dx, z, manning (number or array), mask, infil (number or array),
  *   rainField, bc {west, east, south, north: 'wall'|'open'|'outflow'}, secondOrder, limiter, cfl,
  *   hdry, hfront, dtDry, g, inlets, lts (true = 3 levels, or 1-4), tiles, implicitFriction, stepFlux,
- *   stageCells (solver-gpu.js:35-47, 125-133);
+ *   stageCells, crest (solver-gpu.js:18-20, 35-47, 125-133, 144-151);
  * - setDepth(fn(x, y, z)) and setVelocity(fn(x, y)) evaluated at cell centres, depth clamped at 0
  *   and 0 in walls, vol.initial the sum of the stored FP32 depths (solver-gpu.js:301-318);
  * - rainRate (m/s), setRainField, setStage(z) (m, the level the stage cells' ghosts hold),
+ *   setCrestLevels(zs), readBarriers() -> {plus, minus, peak, level, load, tPeak, tLevel, tLoad, tOver},
  *   setInletRates (m^3/s, + = surface into the network),
  *   readInlets() -> {vol: Float64Array, h: Float32Array, app}, advance(tTarget, K = 64) with the
  *   'GPU solver made no progress (device error?)' error, frame(tEnd, K, readOut) ->
@@ -26,7 +27,7 @@
  *
  * The device argument is accepted for signature compatibility and ignored: the library drives
  * HIP device 0 of the process. Options of the reference outside the shallow-water step
- * (subgrid, sink, gi, barriers, sediment, debris) raise. implicitFriction and stepFlux (F26) are taken
+ * (subgrid, sink, gi, sediment, debris) raise. implicitFriction and stepFlux (F26) are taken
  * as the reference takes them (stepFlux has no effect under local time stepping).
  *
  * Calls on one solver run one at a time: asynchronous calls queue behind each other, and the
@@ -39,7 +40,8 @@ const addon = require(path.join(__dirname, 'floodsim_napi.node'));
 
 const GH = 2; // the reference's ghost width, kept for idx(i, j) and zh
 const BC = { wall: 0, open: 1, outflow: 2 };
-const UNSUPPORTED = ['subgrid', 'sink', 'gi', 'barriers', 'sediment', 'debris'];
+const UNSUPPORTED = ['subgrid', 'sink', 'gi', 'sediment', 'debris'];
+const BARRIER_BOOKS = ['plus', 'minus', 'peak', 'level', 'load', 'tPeak', 'tLevel', 'tLoad', 'tOver'];
 
 function def(v, d) { return v === undefined || v === null ? d : v; }
 
@@ -99,6 +101,14 @@ class GPUSolver {
     this._h = addon.create(params, z, manArr, manS, this._mask, infil, rf, inlets);
     this.stage = -1e9;
     if (o.stageCells && o.stageCells.length) addon.setStageCells(this._h, Int32Array.from(o.stageCells));
+    // flood barriers: faces {i, j, axis, z, C, ground} (solver-gpu.js:18-20, 144-151)
+    this.crest = (o.crest || []).slice();
+    this.nBar = this.crest.length;
+    if (this.nBar) {
+      const col = (k, d, T) => T.from(this.crest.map((f) => (f[k] === undefined || f[k] === null ? d : f[k])));
+      addon.setCrests(this._h, col('i', 0, Int32Array), col('j', 0, Int32Array), col('axis', 0, Int32Array),
+        col('z', -Infinity, Float64Array), col('C', 1.6, Float64Array), col('ground', -Infinity, Float64Array));
+    }
     this.t = 0; this.steps = 0; this.lastDt = 0; this.dtAvg = 0;
     this.vol = { rain: 0, initial: 0, exchanged: 0 };
     this._rain = 0;
@@ -199,6 +209,25 @@ class GPUSolver {
   setStage(z) {
     this.stage = +z;
     this._sync(() => addon.setStage(this._h, this.stage));
+  }
+
+  /** New crest per barrier face, in the order of o.crest (-Infinity = not there; solver-gpu.js:486-488). */
+  setCrestLevels(zs) {
+    const a = Float64Array.from(zs);
+    if (a.length !== this.nBar) throw new RangeError('setCrestLevels: ' + a.length + ' levels for ' + this.nBar + ' faces');
+    this._sync(() => addon.setCrestLevels(this._h, a));
+  }
+
+  /** Barrier books per face since the start (solver-gpu.js:490-504). */
+  readBarriers() {
+    return this._enqueue(() => {
+      const a = addon.readBarriers(this._h, this.nBar), o = {};
+      BARRIER_BOOKS.forEach((k, q) => {
+        o[k] = new Float64Array(this.nBar);
+        for (let m = 0; m < this.nBar; m++) o[k][m] = a[9 * m + q];
+      });
+      return o;
+    });
   }
 
   /** Exchange rates per inlet (m^3/s, + = surface into the network), held until the next call. */

@@ -58,6 +58,8 @@ EXPORTS = [
     "fs_pool_dominant_kernel", "fs_lts_levels", "fs_pool_set_rows", "fs_snapshot_bytes", "fs_snapshot",
     "fs_restore", "fs_pool_snapshot_bytes", "fs_pool_snapshot", "fs_pool_restore", "fs_read_hazard",
     "fs_set_stage_cells", "fs_set_stage", "fs_pool_set_stage_cells", "fs_pool_set_stage",
+    "fs_set_crests", "fs_set_crest_levels", "fs_read_barriers", "fs_pool_set_crests", "fs_pool_set_crest_levels",
+    "fs_pool_read_barriers",
 ]
 
 
@@ -110,6 +112,12 @@ def lib(path=None):
     L.fs_pool_set_stage.argtypes = [P, I32, D]
     L.fs_set_stage_cells.argtypes = [P, ip, I32]
     L.fs_set_stage.argtypes = [P, D]
+    L.fs_set_crests.argtypes = [P, ip, ip, ip, dp, dp, dp, I32]
+    L.fs_set_crest_levels.argtypes = [P, dp]
+    L.fs_read_barriers.argtypes = [P, dp]
+    L.fs_pool_set_crests.argtypes = [P, I32, ip, ip, ip, dp, dp, dp, I32]
+    L.fs_pool_set_crest_levels.argtypes = [P, I32, dp]
+    L.fs_pool_read_barriers.argtypes = [P, I32, dp]
     L.fs_pool_advance.argtypes = [P, ip, dp, I32, I32, ctypes.POINTER(fs_clock)]
     L.fs_pool_step.argtypes = [P, ip, dp, I32, I32]
     L.fs_pool_read_state.argtypes = [P, I32, dp, dp, dp, dp, ctypes.POINTER(fs_mass)]
@@ -191,11 +199,31 @@ class _Layers:
                 len(self.inlets))
 
 
+def _crest_args(crest):
+    """fs_set_crests arguments from the reference's crest list [{i, j, axis, z, C, ground}]."""
+    col = lambda k, d, t: np.ascontiguousarray([d if f.get(k) is None else f.get(k) for f in crest], dtype=t)
+    a = [col("i", 0, np.int32), col("j", 0, np.int32), col("axis", 0, np.int32),
+         col("z", -np.inf, np.float64), col("C", 1.6, np.float64), col("ground", -np.inf, np.float64)]
+    _crest_args.keep = a
+    ip = ctypes.POINTER(ctypes.c_int32)
+    return [a[0].ctypes.data_as(ip), a[1].ctypes.data_as(ip), a[2].ctypes.data_as(ip), _dp(a[3]), _dp(a[4]),
+            _dp(a[5]), len(crest)]
+
+
+def _barrier_books(read, n):
+    out = np.zeros(9 * max(n, 1))
+    if n:
+        read(_dp(out))
+    out = out[:9 * n].reshape(n, 9)
+    return {k: out[:, q].copy() for q, k in enumerate(("plus", "minus", "peak", "level", "load", "tPeak", "tLevel",
+                                                        "tLoad", "tOver"))}
+
+
 class GPUSolver:
     """The reference GPUSolver interface over the HIP library (one instance)."""
 
     def __init__(self, nx, ny, dx, z, manning=0.0, mask=None, infil=None, rainField=None, inlets=None,
-                 stageCells=None, **opts):
+                 stageCells=None, crest=None, **opts):
         L = lib()
         self.nx, self.ny, self.dx = int(nx), int(ny), float(dx)
         self.p = make_params(nx, ny, dx, **opts)
@@ -211,6 +239,10 @@ class GPUSolver:
         if stageCells is not None and len(stageCells):
             sk = np.ascontiguousarray(np.asarray(stageCells, dtype=np.int32))
             self._chk(L.fs_set_stage_cells(h, sk.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(sk)))
+        self.nBar = 0
+        if crest:
+            self.nBar = len(crest)
+            self._chk(L.fs_set_crests(h, *_crest_args(crest)))
         self.nInlets = len(self.layers.inlets)
         self._rain = 0.0
         self._Uh = np.zeros(self.nx * self.ny)
@@ -286,6 +318,16 @@ class GPUSolver:
     def setStage(self, z):
         """Water level held by the stage cells' ghosts (m, absolute; solver-gpu.js:554)."""
         self._chk(lib().fs_set_stage(self._h, float(z)))
+
+    def setCrestLevels(self, zs):
+        """New crest per barrier face, in the order of `crest` (-inf = not there; solver-gpu.js:486-488)."""
+        a = _f64(zs, self.nBar)
+        self._chk(lib().fs_set_crest_levels(self._h, _dp(a)))
+
+    def readBarriers(self):
+        """Barrier books per face (solver-gpu.js:490-504): plus, minus, peak, level, load, tPeak, tLevel,
+        tLoad, tOver."""
+        return _barrier_books(lambda out: self._chk(lib().fs_read_barriers(self._h, out)), self.nBar)
 
     def setInletRates(self, q):
         a = np.ascontiguousarray(np.asarray(q, dtype=np.float64))
@@ -408,6 +450,7 @@ class Pool:
             raise FloodSimError(f"fs_pool_create failed ({rc}): {L.fs_pool_last_error(None).decode()}")
         self._p = h
         self.meta = {}
+        self.nbar = {}
 
     def _chk(self, rc):
         if rc != 0:
@@ -434,7 +477,7 @@ class Pool:
         return lib().fs_pool_bytes_used(self._p)
 
     def add(self, nx, ny, dx, z, manning=0.0, mask=None, infil=None, rainField=None, inlets=None, stageCells=None,
-            **opts):
+            crest=None, **opts):
         """Admit one instance; returns None when the HBM budget is spent."""
         p = make_params(nx, ny, dx, **opts)
         lay = _Layers(int(nx), int(ny), z, manning, mask, infil, rainField, inlets)
@@ -446,7 +489,17 @@ class Pool:
         self.meta[slot.value] = (int(nx), int(ny), float(dx), lay)
         if stageCells is not None and len(stageCells):
             self.set_stage_cells(slot.value, stageCells)
+        if crest:
+            self._chk(lib().fs_pool_set_crests(self._p, slot.value, *_crest_args(crest)))
+            self.nbar[slot.value] = len(crest)
         return slot.value
+
+    def set_crest_levels(self, slot, zs):
+        self._chk(lib().fs_pool_set_crest_levels(self._p, int(slot), _dp(_f64(zs, self.nbar[slot]))))
+
+    def read_barriers(self, slot):
+        return _barrier_books(lambda out: self._chk(lib().fs_pool_read_barriers(self._p, int(slot), out)),
+                              self.nbar.get(slot, 0))
 
     def set_stage_cells(self, slot, cells):
         """Stage cells of one instance (interior edge cells j*nx+i whose outward ghosts hold the stage)."""

@@ -91,6 +91,16 @@ int fs_set_inlet_rates(fs_handle s, const double *q, int32_t n);           /* se
  * the stage; a new list replaces the old one */
 int fs_set_stage_cells(fs_handle s, const int32_t *cells, int32_t n);
 int fs_set_stage(fs_handle s, double z);                   /* setStage (solver-gpu.js:554), m absolute */
+/* the crest option (solver-gpu.js:18-20, 144-151): flood-barrier faces, axis 0 = the face between cells
+ * (i-1, j) and (i, j), axis 1 = between (i, j-1) and (i, j), both interior; crest z (m absolute, non-finite
+ * = not deployed), weir coefficient C (m^0.5/s; the reference's default 1.6), ground (m absolute,
+ * non-finite = unknown); a new list replaces the old one and restarts the books */
+int fs_set_crests(fs_handle s, const int32_t *i, const int32_t *j, const int32_t *axis, const double *z,
+                  const double *C, const double *ground, int32_t n);
+int fs_set_crest_levels(fs_handle s, const double *z);     /* setCrestLevels (solver-gpu.js:486-488) */
+/* readBarriers (solver-gpu.js:490-504): 9 doubles per face: plus, minus (m^3), peak (m^2/s), level (m,
+ * -inf = never wet), load (kN/m), tPeak, tLevel, tLoad, tOver (s, NaN = never overtopped) */
+int fs_read_barriers(fs_handle s, double *out);
 int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n);        /* readInlets (:564-576) */
 int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk);     /* advance (:589-604) */
 /* frame (:607-622): at most K steps toward tEnd; out (nx*ny*2: depth, speed) may be NULL */
@@ -131,6 +141,10 @@ int fs_pool_set_rain(fs_pool_t pool, int32_t slot, double rate_ms);
 int fs_pool_set_inlet_rates(fs_pool_t pool, int32_t slot, const double *q, int32_t n);
 int fs_pool_set_stage_cells(fs_pool_t pool, int32_t slot, const int32_t *cells, int32_t n);
 int fs_pool_set_stage(fs_pool_t pool, int32_t slot, double z);
+int fs_pool_set_crests(fs_pool_t pool, int32_t slot, const int32_t *i, const int32_t *j, const int32_t *axis,
+                       const double *z, const double *C, const double *ground, int32_t n);
+int fs_pool_set_crest_levels(fs_pool_t pool, int32_t slot, const double *z);
+int fs_pool_read_barriers(fs_pool_t pool, int32_t slot, double *out);
 /* advance the listed slots together until each reaches its own target (K steps per launch group);
  * clks (n entries) may be NULL */
 int fs_pool_advance(fs_pool_t pool, const int32_t *slots, const double *tTargets, int32_t n, int32_t K,

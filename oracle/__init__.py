@@ -77,6 +77,10 @@ def lib():
         L.or_set_inlets.argtypes = [P, ctypes.POINTER(ctypes.c_int), I]
         L.or_set_stage_cells.argtypes = [P, ctypes.POINTER(ctypes.c_int), I]
         L.or_set_stage.argtypes = [P, D]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.or_set_crests.argtypes = [P, ip, ip, ip, dp, dp, dp, I]
+        L.or_set_crest_levels.argtypes = [P, dp]
+        L.or_read_barriers.argtypes = [P, dp]
         L.or_set_inlet_rates.argtypes = [P, dp]
         L.or_read_inlets.argtypes = [P, dp, dp]
         L.or_step.restype = D
@@ -124,13 +128,14 @@ class CPUSolver:
     bc {west,east,south,north}, secondOrder, limiter, cfl, hdry, hfront, dtDry, lts,
     infil (number or array, m/s), rainField (array), inlets (interior indices j*nx+i),
     implicitFriction, stepFlux (F26; stepFlux has no effect with lts, as in the reference),
-    stageCells (interior edge cells whose outward ghosts see the level set by setStage).
+    stageCells (interior edge cells whose outward ghosts see the level set by setStage),
+    crest (flood-barrier faces [{i, j, axis, z, C, ground}], solver-cpu.js:43, 98-105).
     """
 
     def __init__(self, nx, ny, dx, z, manning=0.0, mask=None, bc=None, secondOrder=True,
                  limiter="minmod", cfl=None, hdry=1e-6, hfront=1e-3, dtDry=1.0, lts=0,
                  g=9.81, infil=None, rainField=None, inlets=None, implicitFriction=False, stepFlux=False,
-                 stageCells=None, **_ignored):
+                 stageCells=None, crest=None, **_ignored):
         L = lib()
         self.nx, self.ny, self.dx = int(nx), int(ny), float(dx)
         b = {"west": "wall", "east": "wall", "south": "wall", "north": "wall"}
@@ -170,6 +175,16 @@ class CPUSolver:
             ink = np.ascontiguousarray(np.asarray(inlets, dtype=np.int32))
             self.nInlets = len(ink)
             L.or_set_inlets(self._h, ink.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(ink))
+        self.nBar = 0
+        if crest:
+            self.nBar = len(crest)
+            col = lambda k, d, t: np.ascontiguousarray([f.get(k, d) if f.get(k, d) is not None else d for f in crest],
+                                                       dtype=t)
+            self._bi, self._bj, self._ba = col("i", 0, np.int32), col("j", 0, np.int32), col("axis", 0, np.int32)
+            bz, bc, bg = col("z", -np.inf, np.float64), col("C", 1.6, np.float64), col("ground", -np.inf, np.float64)
+            ipt = ctypes.POINTER(ctypes.c_int)
+            L.or_set_crests(self._h, self._bi.ctypes.data_as(ipt), self._bj.ctypes.data_as(ipt),
+                            self._ba.ctypes.data_as(ipt), _dp(bz), _dp(bc), _dp(bg), self.nBar)
         if stageCells is not None and len(stageCells):
             sk = np.ascontiguousarray(np.asarray(stageCells, dtype=np.int32))
             L.or_set_stage_cells(self._h, sk.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(sk))
@@ -224,6 +239,21 @@ class CPUSolver:
     def rainRate(self, r):
         self._rain = float(r)
         lib().or_set_rain_rate(self._h, self._rain)
+
+    def setCrestLevels(self, zs):
+        """New crest per barrier face, in the order of `crest` (-inf = not there; solver-cpu.js:136-138)."""
+        a = np.ascontiguousarray(np.asarray(zs, dtype=np.float64).reshape(self.nBar))
+        lib().or_set_crest_levels(self._h, _dp(a))
+
+    def readBarriers(self):
+        """Barrier books per face (solver-cpu.js:139): plus, minus (m^3), peak (m^2/s), level (m), load (kN/m),
+        tPeak, tLevel, tLoad, tOver (s)."""
+        a = np.zeros(9 * max(self.nBar, 1))
+        if self.nBar:
+            lib().or_read_barriers(self._h, _dp(a))
+        a = a[:9 * self.nBar].reshape(self.nBar, 9)
+        return {k: a[:, q].copy() for q, k in enumerate(("plus", "minus", "peak", "level", "load", "tPeak", "tLevel",
+                                                          "tLoad", "tOver"))}
 
     def setStage(self, z):
         """The water level the stage cells' ghosts hold (m, absolute; solver-cpu.js:204)."""

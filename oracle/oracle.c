@@ -24,7 +24,9 @@
  *                                                      solver-cpu.js:222-225, 250-273, 419-423, 443-452, 465-467, 542-545
  *   water-level edge  stageCells, setStage, the stage ghosts of _fillGhost
  *                                                      solver-cpu.js:109-117, 197-204
- * Out of scope here (as in the product): flood-barrier weirs, Green-Ampt and GI stores.
+ *   flood barriers    crest faces, _weir, setCrestLevels, the barrier books of _L
+ *                                                      solver-cpu.js:98-105, 136-139, 244-258, 277-287, 327-343
+ * Out of scope here (as in the product): Green-Ampt and GI stores.
  *
  * Loops over rows may run on several OpenMP threads. Every per-cell update is
  * independent; sums (rain, infiltration, clipping) are taken per row and then
@@ -66,6 +68,14 @@ typedef struct {
   int nStage;
   int *stageG, *stageS;
   double stageLevel;
+  /* flood barriers: crest (absolute, -INFINITY = none) and weir coefficient per x-face (at the right cell) and
+   * y-face (at the upper cell); the barrier faces (upper/right cell, axis, ground) and their books, 9 per
+   * face: plus, minus, peak, level, load, tPeak, tLevel, tLoad, tOver */
+  double *CX, *CY, *WX, *WY;
+  int nBar;
+  int *barC, *barAxis;
+  double *barGround, *bar;
+  double dtCur; /* the step opL evaluates (the weir's flow cap) */
   double *rowA, *rowB; /* per-row partial sums */
   double t;
   long steps;
@@ -124,6 +134,8 @@ or_solver *or_create(const or_params *p, const double *z, const double *manning,
   s->rowA = zalloc((size_t)s->NY); s->rowB = zalloc((size_t)s->NY);
   s->lev = -1;
   s->stageLevel = -1e9;
+  s->CX = zalloc(N); s->CY = zalloc(N); s->WX = zalloc(N); s->WY = zalloc(N);
+  for (size_t c = 0; c < N; c++) { s->CX[c] = -INFINITY; s->CY[c] = -INFINITY; }
   {
     size_t n = (size_t)p->nx * p->ny;
     s->hzDv = zalloc(n); s->hzForce = zalloc(n); s->hzDur = zalloc(n); s->hzArr = zalloc(n);
@@ -171,6 +183,7 @@ void or_destroy(or_solver *s) {
                     s->rowA, s->rowB, s->tsp, s->aH, s->aHu, s->aHv, s->inQ, s->inV, s->keepU, s->keepV};
   for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) free(arrs[k]);
   free(s->M); free(s->LV); free(s->TI); free(s->inC); free(s->stageG); free(s->stageS);
+  free(s->CX); free(s->CY); free(s->WX); free(s->WY); free(s->barC); free(s->barAxis); free(s->barGround); free(s->bar);
   free(s->hzDv); free(s->hzForce); free(s->hzDur); free(s->hzArr); free(s->hzVmax); free(s->hzWet); free(s->hzCls);
   free(s);
 }
@@ -323,6 +336,37 @@ void or_set_stage_cells(or_solver *s, const int *cells, int n) {
 }
 void or_set_stage(or_solver *s, double z) { s->stageLevel = z; }
 
+/* crest levels per barrier face (absolute; non-finite = not deployed), weir coefficient kept
+ * (setCrestLevels, solver-cpu.js:136-138) */
+void or_set_crest_levels(or_solver *s, const double *z) {
+  for (int m = 0; m < s->nBar; m++) {
+    double zc = isfinite(z[m]) ? z[m] : -INFINITY;
+    if (s->barAxis[m]) s->CY[s->barC[m]] = zc; else s->CX[s->barC[m]] = zc;
+  }
+}
+
+/* o.crest (solver-cpu.js:98-105): faces {i, j, axis, z, C, ground}; axis 0 = the face between (i-1, j) and
+ * (i, j), axis 1 = between (i, j-1) and (i, j); books start empty (level -inf, tOver NaN) */
+void or_set_crests(or_solver *s, const int *i, const int *j, const int *axis, const double *z, const double *C,
+                   const double *ground, int n) {
+  free(s->barC); free(s->barAxis); free(s->barGround); free(s->bar);
+  s->nBar = n;
+  s->barC = (int *)calloc((size_t)(n ? n : 1), sizeof(int));
+  s->barAxis = (int *)calloc((size_t)(n ? n : 1), sizeof(int));
+  s->barGround = zalloc((size_t)n);
+  s->bar = zalloc((size_t)n * 9);
+  for (int m = 0; m < n; m++) {
+    s->barC[m] = (j[m] + GH) * s->NX + i[m] + GH;
+    s->barAxis[m] = axis[m];
+    s->barGround[m] = ground[m];
+    s->bar[9 * m + 3] = -INFINITY;
+    s->bar[9 * m + 8] = NAN;
+    if (axis[m]) s->WY[s->barC[m]] = C[m]; else s->WX[s->barC[m]] = C[m];
+  }
+  or_set_crest_levels(s, z);
+}
+void or_read_barriers(const or_solver *s, double *out) { memcpy(out, s->bar, sizeof(double) * 9 * (size_t)s->nBar); }
+
 static void prims(or_solver *s, const double *h, const double *hu, const double *hv) {
   const int N = s->N;
   const double hdry = s->p.hdry;
@@ -388,8 +432,27 @@ static void hllc(double hL, double uL, double vL, double hR, double uR, double v
   F[0] = H0; F[1] = H1; F[2] = H0 * (SM >= 0 ? vL : vR);
 }
 
-/* face between a (left/below) and b; F = [F0, F1, F2, corrL, corrR, hL, zL, hR, zR] (solver-cpu.js:237-267) */
-static void face(const or_solver *s, int a, int b, int st, int isX, double *F) {
+/* flow over a barrier crest zc: broad-crested weir q = C H1^1.5 with Villemonte submergence
+ * (1 - (H2/H1)^1.5)^0.385, capped so one step cannot more than half-close the level difference; the
+ * momentum carries the critical velocity over the crest, the pressure is the mean of the two sides'
+ * depths over the crest (_weir, solver-cpu.js:277-287) */
+static void weir(const or_solver *s, int a, int b, int isX, double zc, double cw, double hLs, double hRs, double *F) {
+  const double *Hp = s->Hp, *Ep = s->Ep, g = s->p.g, hdry = s->p.hdry, dx = s->p.dx;
+  int left = Ep[a] >= Ep[b], up = left ? a : b, dn = left ? b : a;
+  double pres = 0.25 * g * (hLs * hLs + hRs * hRs), H1 = Ep[up] - zc;
+  F[0] = 0; F[1] = pres; F[2] = 0;
+  if (H1 <= hdry || Hp[up] <= hdry) return;
+  double r = dmax(0, Ep[dn] - zc) / H1;
+  if (r >= 1) return;
+  double q = cw * pow(H1, 1.5) * pow(1 - pow(r, 1.5), 0.385);
+  if (s->dtCur > 0) q = dmin(q, (Ep[up] - Ep[dn]) * dx / (4 * s->dtCur));
+  double un = q / dmax(2.0 / 3.0 * H1, hdry), F0 = left ? q : -q;
+  F[0] = F0; F[1] = q * un + pres; F[2] = F0 * (isX ? s->Vp[up] : s->Up[up]);
+}
+
+/* face between a (left/below) and b with crest zc (-INFINITY: none) and weir coefficient cw;
+ * F = [F0, F1, F2, corrL, corrR, hL, zL, hR, zR] (solver-cpu.js:237-267) */
+static void face(const or_solver *s, int a, int b, int st, int isX, double *F, double zc, double cw) {
   int Ma = s->M[a], Mb = s->M[b];
   if (!Ma && !Mb) { for (int k = 0; k < 9; k++) F[k] = 0; return; }
   double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -399,17 +462,20 @@ static void face(const or_solver *s, int a, int b, int st, int isX, double *F) {
   if (!Ma) { r[0] = r[4]; r[1] = r[5]; r[2] = -r[6]; r[3] = r[7]; }
   double hL = r[0], zL = r[1] - r[0], uL = r[2], vL = r[3];
   double hR = r[4], zR = r[5] - r[4], uR = r[6], vR = r[7];
-  double zs = dmax(zL, zR);
+  double zs = dmax(dmax(zL, zR), zc);
   double hLs = dmax(0, r[1] - zs), hRs = dmax(0, r[5] - zs);
   const double g = s->p.g, hdry = s->p.hdry;
+  int isWeir = Ma && Mb && zc > dmax(s->z[a], s->z[b]);
   /* F26 step face (solver-cpu.js:250-254): a wet film on one side whose reconstructed depth over the
    * other side's higher bed is zero passes its own discharge, upwind (_stepFlux, :269-273) */
   int cas = 0;
-  if (s->p.stepFlux && !s->p.lts && Ma && Mb && s->n[a] > 0 && s->n[b] > 0) {
+  if (s->p.stepFlux && !s->p.lts && !isWeir && Ma && Mb && s->n[a] > 0 && s->n[b] > 0) {
     if (hLs > hdry && hRs <= 0 && hR > hdry) cas = 1;
     else if (hRs > hdry && hLs <= 0 && hL > hdry) cas = -1;
   }
-  if (cas == 1) {
+  if (isWeir) {
+    weir(s, a, b, isX, zc, cw, hLs, hRs, F);
+  } else if (cas == 1) {
     double u = dmax(uL, 0), q = hLs * u;
     F[0] = q; F[1] = q * u + 0.5 * g * hLs * hLs; F[2] = q * vL;
   } else if (cas == -1) {
@@ -433,6 +499,7 @@ static double opL(or_solver *s, double *h, double *hu, double *hv, double dt) {
   const int NX = s->NX, NY = s->NY, lev = s->lev;
   double *FX = s->FX, *FY = s->FY, *TH = s->TH;
   const double dx = s->p.dx, g = s->p.g;
+  s->dtCur = dt;
   fillGhost(s, h, hu, hv);
   prims(s, h, hu, hv);
 #define ON(a, b) (lev < 0 || (lvc(s, a) < lvc(s, b) ? lvc(s, a) : lvc(s, b)) == lev)
@@ -442,13 +509,13 @@ static double opL(or_solver *s, double *h, double *hu, double *hv, double dt) {
   for (int j = GH; j < NY - GH; j++)
     for (int i = GH; i <= NX - GH; i++) {
       int b = j * NX + i;
-      if (ON(b - 1, b)) face(s, b - 1, b, 1, 1, FX + (size_t)b * 9);
+      if (ON(b - 1, b)) face(s, b - 1, b, 1, 1, FX + (size_t)b * 9, s->CX[b], s->WX[b]);
     }
 #pragma omp parallel for schedule(static)
   for (int j = GH; j <= NY - GH; j++)
     for (int i = GH; i < NX - GH; i++) {
       int b = j * NX + i;
-      if (ON(b - NX, b)) face(s, b - NX, b, NX, 0, FY + (size_t)b * 9);
+      if (ON(b - NX, b)) face(s, b - NX, b, NX, 0, FY + (size_t)b * 9, s->CY[b], s->WY[b]);
     }
   /* draining-time limiter */
   const double area = dx * dx;
@@ -481,6 +548,33 @@ static double opL(or_solver *s, double *h, double *hu, double *hv, double dt) {
       double t = TH[f[0] > 0 ? b - NX : b];
       if (t < 1 && ON(b - NX, b)) { f[0] *= t; f[1] *= t; f[2] *= t; }
     }
+  /* barrier books (solver-cpu.js:327-343): the theta-scaled flux each barrier face passes this stage, booked
+   * while the barrier is up and water stands 2 cm over its crest; in the first stage the peak rate, the
+   * highest water and the load, each with the step's start time */
+  {
+    const double wgt = s->p.secondOrder ? 0.5 : 1;
+    for (int m = 0; m < s->nBar; m++) {
+      int b = s->barC[m], a = b - (s->barAxis[m] ? NX : 1);
+      double crest = (s->barAxis[m] ? s->CY : s->CX)[b], *B = s->bar + 9 * m;
+      if ((s->Hp[a] <= 0 && s->Hp[b] <= 0) || crest == -INFINITY || !ON(a, b)) continue;
+      double ea = s->Hp[a] > 0.005 ? s->Ep[a] : -INFINITY, eb = s->Hp[b] > 0.005 ? s->Ep[b] : -INFINITY;
+      double w = dmax(ea, eb);
+      double q = (s->barAxis[m] ? FY : FX)[(size_t)b * 9];
+      int over = w > crest + 0.02;
+      if (over) {
+        double v = q * dx * dt * wgt;
+        if (v > 0) B[0] += v; else B[1] -= v;
+        if (isnan(B[8]) && v != 0) B[8] = s->t;
+      }
+      if (s->stage) continue;
+      if (over && fabs(q) > B[2]) { B[2] = fabs(q); B[5] = s->t; }
+      if (w > B[3]) { B[3] = w; B[6] = s->t; }
+      double gr = s->barGround[m];
+      double dA = dmax(0, dmin(ea, crest) - gr), dB = dmax(0, dmin(eb, crest) - gr);
+      double load = isfinite(gr) ? 0.5 * g * fabs(dA * dA - dB * dB) : 0;
+      if (load > B[4]) { B[4] = load; B[7] = s->t; }
+    }
+  }
   const double inv = 1 / dx;
 #pragma omp parallel for schedule(static)
   for (int j = GH; j < NY - GH; j++)

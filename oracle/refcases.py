@@ -3,6 +3,7 @@
   rng, cloudburst, dambreak, thacker   src/scenarios.js:4-84
   damBreak (Stoker / Ritter)           src/analytic.js:5-27
   thackerPlanar                        src/analytic.js:30-40
+  barrier_case                         the flood-barrier case of tests/gpu-suite.js:242-253
 
 They rebuild the inputs of the recorded fixtures (tests/results-cpu.json) so the
 oracle can be checked against them. Loops follow the JS order where a random
@@ -203,3 +204,19 @@ def thacker_bed(nx, T=None):
     x = (np.arange(nx) + 0.5) * dx
     X, Y = np.meshgrid(x, x)
     return T.bed(X, Y).reshape(-1), dx, T
+
+
+def barrier_case():
+    """gpu-suite.js:242-253: a wall of barrier faces across a sloping 64 x 48 basin (crest 5.4 m, C 1.6)
+    and a gap-free diagonal leg (crest 4.9 m, C 1.44), a dam-break start, the leg raised 0.5 m at 30 s."""
+    nx, ny, dx = 64, 48, 2
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = (5 - 0.02 * I + 0.01 * np.sin(J / 3)).reshape(-1)
+    crest = [dict(i=20, j=j, axis=0, z=5.4, C=1.6, ground=4.6) for j in range(ny)]
+    for i in range(36, 50):
+        crest += [dict(i=i, j=i - 20, axis=1, z=4.9, C=1.44, ground=4.3),
+                  dict(i=i + 1, j=i - 20, axis=0, z=4.9, C=1.44, ground=4.3)]
+    opts = dict(nx=nx, ny=ny, dx=dx, z=z, manning=0.03, crest=crest,
+                bc={"west": "wall", "east": "outflow", "south": "wall", "north": "wall"})
+    lift = [f["z"] if m < ny else f["z"] + 0.5 for m, f in enumerate(crest)]
+    return opts, lift, (lambda X, Y, Z: np.where(X < 30, 6 - Z, 0))

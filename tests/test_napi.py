@@ -20,8 +20,8 @@ NODE = shutil.which("node")
 pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
 ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readHazard", "readInlets", "readState",
-               "restore", "setInletRates", "setRain", "setRainField", "setStage", "setStageCells", "setState",
-               "snapshot", "tileStats", "version"]
+               "readBarriers", "restore", "setCrestLevels", "setCrests", "setInletRates", "setRain", "setRainField", "setStage",
+               "setStageCells", "setState", "snapshot", "tileStats", "version"]
 
 
 def node(script, *args, timeout=120):

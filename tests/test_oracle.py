@@ -416,6 +416,30 @@ def test_stage_boundary_sea(golden):
     assert abs(s.massBalance()["rel"]) < 1e-10
 
 
+def test_barriers(golden):
+    """The volume over the barriers, summed over faces and directions, is the reference CPU's to 1e-10
+    (results-gpu.json `barriers.overVolCPU` = 860.666935813937 m^3); the books close."""
+    opts, lift, init = rc.barrier_case()
+    c = oracle.CPUSolver(**opts, hdry=1e-5)
+    c.setDepth(init)
+    c.run(30)
+    c.setCrestLevels(lift)
+    c.run(90)
+    b = c.readBarriers()
+    over = b["plus"].sum() + b["minus"].sum()
+    print(f"\nbarriers: over {over:.9f} m3 (reference CPU {golden['gpu']['barriers']['overVolCPU']:.9f}), "
+          f"{c.steps} steps, mass rel {c.massBalance()['rel']:.1e}")
+    assert rel(over, golden["gpu"]["barriers"]["overVolCPU"]) < 1e-10
+    assert abs(c.massBalance()["rel"]) < 1e-12
+    # a crest below both beds changes nothing: no barrier at all gives the same state
+    o2 = dict(opts, crest=[dict(f, z=-10.0) for f in opts["crest"]])
+    a, n = oracle.CPUSolver(**o2, hdry=1e-5), oracle.CPUSolver(**dict(opts, crest=None), hdry=1e-5)
+    for s in (a, n):
+        s.setDepth(init)
+        s.run(20)
+    np.testing.assert_array_equal(a.field("h"), n.field("h"))
+
+
 def test_symmetry_x_y():
     """A dam break along y is the transpose of the one along x (catches a swapped y-face operand)."""
     nx = 60

@@ -70,6 +70,7 @@ struct Slot {
   double *barBook = nullptr;
   std::vector<float4> hcrest;            // host copy of the crest plane
   std::vector<int4> bars;                // per barrier: (i, j, axis, k of its right/upper cell)
+  int stageCells = 0;                    // cells flagged MASK_STAGE
   StepRec hrec{};                        // host mirror (refreshed after each launch group)
   double volInitial = 0.0;               // vol.initial (solver-gpu.js:309)
   double exchangedRead = 0.0;            // vol.exchanged: inlet volumes already read (solver-gpu.js:574)
@@ -271,9 +272,10 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.rec = P->drec + slots[gi];
     d.nx = S.nx; d.ny = S.ny; d.pitch = S.pitch;
     d.flags = (S.infil ? FLAG_INFIL : 0) | (S.rf ? FLAG_RF : 0) | (S.p.implicitFriction ? FLAG_IMPF : 0) |
-              (S.p.stepFlux && !S.L ? FLAG_STEPF : 0);   // the step flux applies under global stepping only
-    // extended faces: the step flux or barrier crests (one kernel variant carries both)
-    const bool ext = (d.flags & FLAG_STEPF) || S.nbar;
+              (S.p.stepFlux && !S.L ? FLAG_STEPF : 0) |   // the step flux applies under global stepping only
+              (S.stageCells ? FLAG_STAGE : 0);
+    // extended faces and edges: the step flux, barrier crests or stage cells (one kernel variant)
+    const bool ext = (d.flags & (FLAG_STEPF | FLAG_STAGE)) || S.nbar;
     if (ext) G.anyStepFlux = true;
     G.kinds |= 1u << ((S.hzA ? 2 : 0) + (ext ? 1 : 0));
     d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;
@@ -477,9 +479,10 @@ int launch_cycle(fs_pool *P) {
   for (int k = 0; k < Pn; ++k)
     for (int m = 0; m <= L; ++m) {
       if ((k + 1) % (1 << m)) continue;
-      if (d.nbar) {
+      const bool ext = d.nbar || (d.flags & FLAG_STAGE);
+      if (ext) {
         sw_stage<0, true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
-        sw_barrier_books<0, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
+        if (d.nbar) sw_barrier_books<0, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
       } else {
         sw_stage<0, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
       }
@@ -492,10 +495,10 @@ int launch_cycle(fs_pool *P) {
       // the record's base step rides in the level argument's upper bits (its end time: hazard clock)
       const int mk = m | (k << 8);
       if (d.secondOrder) {
-        if (d.nbar) {
+        if (ext) {
           if (d.hzA) sw_stage<1, true, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
           else sw_stage<1, true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
-          sw_barrier_books<1, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
+          if (d.nbar) sw_barrier_books<1, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
         } else if (d.hzA) {
           sw_stage<1, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
         } else {
@@ -1075,6 +1078,9 @@ int fs_pool_set_stage_cells(fs_pool_t P, int32_t slot, const int32_t *cells, int
     if (cells[q] < 0 || cells[q] >= N) return fail(P, FS_E_ARG, "stage cell %d outside the grid", cells[q]);
   for (auto &b : S.hmask) b &= (uint8_t)~MASK_STAGE;
   for (int q = 0; q < n; ++q) S.hmask[cells[q]] |= MASK_STAGE;
+  S.stageCells = n;
+  P->grp.slots.clear();   // the instance's kernel variant changes
+  drop_graph(P->grp);
   HIPCHK(P, hipStreamSynchronize(P->stream));
   HIPCHK(P, hipMemcpy(S.mask, S.hmask.data(), (size_t)N, hipMemcpyHostToDevice));
   return upload_strip_rain(P, S);

@@ -47,8 +47,8 @@ __device__ __forceinline__ void gst(float2 *row, int x, float2 v) {
 // mask byte bits: open (not a wall) cell, inlet cell, Manning n > 0 (the step-flux rules apply), stage
 // cell (its outward ghosts hold the stage)
 enum { MASK_OPEN = 1, MASK_INLET = 2, MASK_ROUGH = 4, MASK_STAGE = 8 };
-// DevInst flags: infiltration layer, rain field, implicit friction and step flux (F26)
-enum { FLAG_INFIL = 1, FLAG_RF = 2, FLAG_IMPF = 4, FLAG_STEPF = 8 };
+// DevInst flags: infiltration layer, rain field, implicit friction and step flux (F26), stage cells
+enum { FLAG_INFIL = 1, FLAG_RF = 2, FLAG_IMPF = 4, FLAG_STEPF = 8, FLAG_STAGE = 16 };
 
 // Per-instance mutable record: clock, dt, FP64 books. Written by dt_final only (and the host
 // between launches); the stage kernels read dt and the rain rate from it.

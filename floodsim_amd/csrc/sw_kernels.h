@@ -220,7 +220,9 @@ __device__ __forceinline__ void cell_state(const DevInst &I, int x, int y, int m
   else { h = I.h[k]; hu = I.hu[k]; hv = I.hv[k]; }
 }
 
-template <int STAGE, bool LTS>
+// EXT: the stage cells' ghosts (kept out of the plain kernels: their code in this edge path alone costs
+// the plain second stage registers and spills)
+template <int STAGE, bool LTS, bool EXT = false>
 __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m, float dt) {
   const int nx = I.nx, ny = I.ny;
   int sx = X, sy = Y;
@@ -286,7 +288,7 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m,
     // water-level boundary (stageCells, setStage): the ghost of a stage cell holds the surface at the
     // stage where that stands above the ghost's bed, moving with the edge cell's velocity
     // (solver-cpu.js:197-202; kernels.wgsl.js:148-154)
-    if (I.mask[kz] & MASK_STAGE) {
+    if (EXT && (I.mask[kz] & MASK_STAGE)) {
       const float hs = I.rec->stage - p.z;
       if (hs > I.hdry) {
         float he, hue, hve;
@@ -482,8 +484,8 @@ struct FaceRow { float F0, F1, F2, cL, cR, on; };
 // store; cells at m + 1 wait and gather the RK-weighted flux of those faces; the bed-slope term of a
 // stepping cell takes, for a face whose neighbour is finer, its reconstruction from that face's last
 // evaluation (kept in bsx/bsy by the finer record's last stage).
-// EXT: extended faces, for instances with the step-flux rules (F26; global stepping only) or flood
-// barriers (crests in the face beds, weir faces, the barrier faces' fluxes for their books)
+// EXT: extended faces and edges, for instances with the step-flux rules (F26; global stepping only), flood
+// barriers (crests in the face beds, weir faces, the barrier faces' fluxes for their books) or stage cells
 template <int STAGE, bool LTS, bool HZ, bool EXT = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
                                           Part *__restrict__ part, const int cells, const float tnow) {
@@ -536,7 +538,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     // ---------------- row r: primitives
     Prim p;
     if (xin && r >= 0 && r < ny) p = load_prim_row<STAGE, LTS>(I, X, r, m, dt);
-    else p = load_prim<STAGE, LTS>(I, X, r, m, dt);
+    else p = load_prim<STAGE, LTS, EXT>(I, X, r, m, dt);
 
     // ---------------- x reconstruction of row r (neighbours by lane shifts)
     const float hl = from_left(p.h), ul = from_left(p.u), vlf = from_left(p.v), zl = from_left(p.z);
@@ -929,7 +931,7 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
 // of its kind (hazard products: second stage only; extended faces: step flux or barriers) and leaves the
 // others' strips alone, so a group of mixed kinds runs one launch per kind present and every instance gets
 // the same machine code, hence the same bits, alone and in any mix.
-__device__ __forceinline__ bool ext_faces(const DevInst &I) { return (I.flags & FLAG_STEPF) || I.crest; }
+__device__ __forceinline__ bool ext_faces(const DevInst &I) { return (I.flags & (FLAG_STEPF | FLAG_STAGE)) || I.crest; }
 
 template <int STAGE, bool LTS, bool HZ = false, bool EXT = false>
 __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,

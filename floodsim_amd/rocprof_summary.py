@@ -8,10 +8,15 @@ statistics are taken over that window.
     python -m floodsim_amd.rocprof_summary trace  <k_kernel_trace.csv> --kernel 'sw_stage<1' --last 1240
     python -m floodsim_amd.rocprof_summary traffic <fetch counter_collection.csv> <write counter_collection.csv> \
         --kernel 'sw_stage<1' --last 1240 --copy-bytes 2147483648
+    python -m floodsim_amd.rocprof_summary counters <counter_collection.csv> --last 1240 --clock-ghz 2.4
 
 `traffic` turns FETCH_SIZE and WRITE_SIZE (KiB per dispatch) into HBM bytes per dispatch of the
 kernel, each scaled by the ratio of the known size to what the same counter reads for the copy
 kernel (fs_copy_kernel, `copy-bytes` read and written per dispatch) in the same run.
+
+`counters` gives, per kernel, the mean of every collected counter over its last N dispatches and,
+from SQ_INSTS_VALU, the time its VALU instructions take at the full issue rate of the chip (one
+wave64 instruction per 4 cycles on each of 1024 SIMDs at `clock-ghz`, packed FP32 not used).
 """
 import argparse
 import csv
@@ -74,6 +79,26 @@ def traffic(fetch_path, write_path, kernel, last, copy_bytes):
                 read_bytes=read_b, write_bytes=write_b, bytes_per_launch=read_b + write_b, launches=last)
 
 
+def counters(path, last, clock_ghz, simds=1024):
+    """Mean counter values per dispatch over each kernel's last `last` dispatches."""
+    by = {}
+    for r in _rows(path):
+        by.setdefault(_short(r["Kernel_Name"]), []).append(r)
+    out = {}
+    for k, rs in by.items():
+        disp = sorted({int(r["Dispatch_Id"]) for r in rs})[-last:]
+        keep = set(disp)
+        tot = {}
+        for r in rs:
+            if int(r["Dispatch_Id"]) in keep:
+                tot[r["Counter_Name"]] = tot.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        per = {c: v / len(disp) for c, v in tot.items()}
+        if "SQ_INSTS_VALU" in per:
+            per["valu_issue_ms"] = per["SQ_INSTS_VALU"] * 4.0 / (simds * clock_ghz * 1e9) * 1e3
+        out[k] = dict(dispatches=len(disp), per_dispatch=per)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -87,6 +112,10 @@ def main(argv=None):
     b.add_argument("--kernel", required=True)
     b.add_argument("--last", type=int, required=True)
     b.add_argument("--copy-bytes", type=float, default=float(2 << 30))
+    c = sub.add_parser("counters")
+    c.add_argument("path")
+    c.add_argument("--last", type=int, required=True)
+    c.add_argument("--clock-ghz", type=float, default=2.4)
     args = ap.parse_args(argv)
     if args.cmd == "trace":
         res = trace_window(args.path, args.kernel, args.last)
@@ -95,6 +124,9 @@ def main(argv=None):
         for k in res["kernels"]:
             print(f"{k['kernel'][:58]:58s} {k['calls']:6d} {k['mean_us']:10.1f} {k['min_us']:9.1f} {k['max_us']:9.1f} "
                   f"{k['total_ms']:9.2f} {100 * k['share']:5.1f}%")
+    elif args.cmd == "counters":
+        json.dump(counters(args.path, args.last, args.clock_ghz), sys.stdout, indent=1)
+        print()
     else:
         res = traffic(args.fetch, args.write, args.kernel, args.last, args.copy_bytes)
         json.dump(res, sys.stdout, indent=1)

@@ -81,6 +81,8 @@ def lib():
         L.or_set_crests.argtypes = [P, ip, ip, ip, dp, dp, dp, I]
         L.or_set_crest_levels.argtypes = [P, dp]
         L.or_read_barriers.argtypes = [P, dp]
+        L.or_set_ga.argtypes = [P, dp, dp, dp]
+        L.or_set_gi.argtypes = [P, dp, dp, dp]
         L.or_set_inlet_rates.argtypes = [P, dp]
         L.or_read_inlets.argtypes = [P, dp, dp]
         L.or_step.restype = D
@@ -118,7 +120,7 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-_FIELDS = {"h": 0, "hu": 1, "hv": 2, "maxDepth": 3, "hmax": 3, "z": 4, "aH": 5}
+_FIELDS = {"h": 0, "hu": 1, "hv": 2, "maxDepth": 3, "hmax": 3, "z": 4, "aH": 5, "gaF": 6, "giStore": 7}
 
 
 class CPUSolver:
@@ -129,13 +131,15 @@ class CPUSolver:
     infil (number or array, m/s), rainField (array), inlets (interior indices j*nx+i),
     implicitFriction, stepFlux (F26; stepFlux has no effect with lts, as in the reference),
     stageCells (interior edge cells whose outward ghosts see the level set by setStage),
-    crest (flood-barrier faces [{i, j, axis, z, C, ground}], solver-cpu.js:43, 98-105).
+    crest (flood-barrier faces [{i, j, axis, z, C, ground}], solver-cpu.js:43, 98-105),
+    ga {psiD, fmax, perv} (Green-Ampt soil) and gi {rate, cap, drain} (GI stores), per cell
+    (solver-cpu.js:94-96).
     """
 
     def __init__(self, nx, ny, dx, z, manning=0.0, mask=None, bc=None, secondOrder=True,
                  limiter="minmod", cfl=None, hdry=1e-6, hfront=1e-3, dtDry=1.0, lts=0,
                  g=9.81, infil=None, rainField=None, inlets=None, implicitFriction=False, stepFlux=False,
-                 stageCells=None, crest=None, **_ignored):
+                 stageCells=None, crest=None, ga=None, gi=None, **_ignored):
         L = lib()
         self.nx, self.ny, self.dx = int(nx), int(ny), float(dx)
         b = {"west": "wall", "east": "wall", "south": "wall", "north": "wall"}
@@ -185,6 +189,12 @@ class CPUSolver:
             ipt = ctypes.POINTER(ctypes.c_int)
             L.or_set_crests(self._h, self._bi.ctypes.data_as(ipt), self._bj.ctypes.data_as(ipt),
                             self._ba.ctypes.data_as(ipt), _dp(bz), _dp(bc), _dp(bg), self.nBar)
+        if ga is not None:
+            a = [np.ascontiguousarray(np.asarray(ga[k], dtype=np.float64).reshape(n)) for k in ("psiD", "fmax", "perv")]
+            L.or_set_ga(self._h, *[_dp(x) for x in a])
+        if gi is not None:
+            a = [np.ascontiguousarray(np.asarray(gi[k], dtype=np.float64).reshape(n)) for k in ("rate", "cap", "drain")]
+            L.or_set_gi(self._h, *[_dp(x) for x in a])
         if stageCells is not None and len(stageCells):
             sk = np.ascontiguousarray(np.asarray(stageCells, dtype=np.int32))
             L.or_set_stage_cells(self._h, sk.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(sk))

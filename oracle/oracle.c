@@ -26,7 +26,7 @@
  *                                                      solver-cpu.js:109-117, 197-204
  *   flood barriers    crest faces, _weir, setCrestLevels, the barrier books of _L
  *                                                      solver-cpu.js:98-105, 136-139, 244-258, 277-287, 327-343
- * Out of scope here (as in the product): Green-Ampt and GI stores.
+ *   Green-Ampt, GI    the ga / gi options and their terms of _sources  solver-cpu.js:94-96, 466-474
  *
  * Loops over rows may run on several OpenMP threads. Every per-cell update is
  * independent; sums (rain, infiltration, clipping) are taken per row and then
@@ -76,6 +76,9 @@ typedef struct {
   int *barC, *barAxis;
   double *barGround, *bar;
   double dtCur; /* the step opL evaluates (the weir's flow cap) */
+  /* Green-Ampt soil (psiD, Fmax, pervious share; F infiltrated so far) and GI stores (rate, capacity, drain;
+   * stored depth), per padded cell, zero where absent */
+  double *gaPsiD, *gaFmax, *gaPerv, *gaF, *giRate, *giCap, *giDrain, *giStore;
   double *rowA, *rowB; /* per-row partial sums */
   double t;
   long steps;
@@ -135,6 +138,8 @@ or_solver *or_create(const or_params *p, const double *z, const double *manning,
   s->lev = -1;
   s->stageLevel = -1e9;
   s->CX = zalloc(N); s->CY = zalloc(N); s->WX = zalloc(N); s->WY = zalloc(N);
+  s->gaPsiD = zalloc(N); s->gaFmax = zalloc(N); s->gaPerv = zalloc(N); s->gaF = zalloc(N);
+  s->giRate = zalloc(N); s->giCap = zalloc(N); s->giDrain = zalloc(N); s->giStore = zalloc(N);
   for (size_t c = 0; c < N; c++) { s->CX[c] = -INFINITY; s->CY[c] = -INFINITY; }
   {
     size_t n = (size_t)p->nx * p->ny;
@@ -184,6 +189,8 @@ void or_destroy(or_solver *s) {
   for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) free(arrs[k]);
   free(s->M); free(s->LV); free(s->TI); free(s->inC); free(s->stageG); free(s->stageS);
   free(s->CX); free(s->CY); free(s->WX); free(s->WY); free(s->barC); free(s->barAxis); free(s->barGround); free(s->bar);
+  free(s->gaPsiD); free(s->gaFmax); free(s->gaPerv); free(s->gaF);
+  free(s->giRate); free(s->giCap); free(s->giDrain); free(s->giStore);
   free(s->hzDv); free(s->hzForce); free(s->hzDur); free(s->hzArr); free(s->hzVmax); free(s->hzWet); free(s->hzCls);
   free(s);
 }
@@ -366,6 +373,23 @@ void or_set_crests(or_solver *s, const int *i, const int *j, const int *axis, co
   or_set_crest_levels(s, z);
 }
 void or_read_barriers(const or_solver *s, double *out) { memcpy(out, s->bar, sizeof(double) * 9 * (size_t)s->nBar); }
+
+/* o.ga (solver-cpu.js:96): Green-Ampt soil where perv > 0 (Fmax capped at 1e6); o.gi (:95): GI stores where
+ * cap > 0; per interior cell */
+void or_set_ga(or_solver *s, const double *psiD, const double *fmax, const double *perv) {
+  for (int j = 0; j < s->p.ny; j++)
+    for (int i = 0; i < s->p.nx; i++) {
+      int c = (j + GH) * s->NX + i + GH, k = j * s->p.nx + i;
+      if (perv[k] > 0) { s->gaPsiD[c] = psiD[k]; s->gaFmax[c] = dmin(fmax[k], 1e6); s->gaPerv[c] = perv[k]; }
+    }
+}
+void or_set_gi(or_solver *s, const double *rate, const double *cap, const double *drain) {
+  for (int j = 0; j < s->p.ny; j++)
+    for (int i = 0; i < s->p.nx; i++) {
+      int c = (j + GH) * s->NX + i + GH, k = j * s->p.nx + i;
+      if (cap[k] > 0) { s->giRate[c] = rate[k]; s->giCap[c] = cap[k]; s->giDrain[c] = drain[k]; }
+    }
+}
 
 static void prims(or_solver *s, const double *h, const double *hu, const double *hv) {
   const int N = s->N;
@@ -712,7 +736,21 @@ static void sources(or_solver *s, double dt, int sel, double tEnd) {
       }
       double r = r0 * (s->rainField ? s->rainField[c] : 1) * dt;
       double cap = s->infil[c] * dt;
+      /* Green-Ampt: capacity K (1 + psi dtheta / F), none once the soil above the water table is full
+       * (solver-cpu.js:466) */
+      if (s->gaPerv[c] > 0)
+        cap = s->gaF[c] >= s->gaFmax[c] ? 0
+                                        : dmin(cap * (1 + s->gaPsiD[c] / dmax(s->gaF[c], 1e-3)),
+                                               (s->gaFmax[c] - s->gaF[c]) * s->gaPerv[c]);
       double inf = dmin(cap, d + r);
+      if (s->gaPerv[c] > 0) s->gaF[c] += inf / s->gaPerv[c];
+      /* GI store: drains linearly, then takes what it can at its rate up to its capacity (:469-472) */
+      if (s->giCap[c] > 0) {
+        double st = dmax(0, s->giStore[c] - s->giDrain[c] * dt);
+        double g2 = dmin(dmin(s->giRate[c] * dt, d + r - inf), s->giCap[c] - st);
+        s->giStore[c] = st + dmax(0, g2);
+        if (g2 > 0) inf += g2;
+      }
       s->h[c] = dmax(0, d + r - inf);
       rv += r;
       iv += d + r - s->h[c];
@@ -954,7 +992,7 @@ void or_get_hazard(const or_solver *s, int which, double *out) {
 /* field: 0 h, 1 hu, 2 hv, 3 maxDepth, 4 z, 5 aH; interior nx*ny row-major */
 void or_get_field(const or_solver *s, int which, double *out) {
   const double *src = which == 0 ? s->h : which == 1 ? s->hu : which == 2 ? s->hv : which == 3 ? s->maxDepth
-                    : which == 4 ? s->z : s->aH;
+                    : which == 4 ? s->z : which == 6 ? s->gaF : which == 7 ? s->giStore : s->aH;
   for (int j = 0; j < s->p.ny; j++)
     for (int i = 0; i < s->p.nx; i++) out[j * s->p.nx + i] = src ? src[(j + GH) * s->NX + i + GH] : 0;
 }

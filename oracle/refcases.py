@@ -4,6 +4,7 @@
   damBreak (Stoker / Ritter)           src/analytic.js:5-27
   thackerPlanar                        src/analytic.js:30-40
   barrier_case                         the flood-barrier case of tests/gpu-suite.js:242-253
+  gi_case, ga_case                     the GI and Green-Ampt cases of tests/gpu-suite.js:188-193, 332-345
 
 They rebuild the inputs of the recorded fixtures (tests/results-cpu.json) so the
 oracle can be checked against them. Loops follow the JS order where a random
@@ -220,3 +221,33 @@ def barrier_case():
                 bc={"west": "wall", "east": "outflow", "south": "wall", "north": "wall"})
     lift = [f["z"] if m < ny else f["z"] + 0.5 for m, f in enumerate(crest)]
     return opts, lift, (lambda X, Y, Z: np.where(X < 30, 6 - Z, 0))
+
+
+MMHR_ = 1 / 1000 / 3600
+
+
+def gi_case():
+    """tests/gpu-suite.js:188-193: GI stores (100 mm/h intake, 20 mm capacity, 5 mm/h drain) on the west
+    half of a 60 x 40 slope; 60 mm/h rain for 1800 s, then none until 3600 s."""
+    nx, ny, dx = 60, 40, 2
+    I, _ = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = (0.01 * I).reshape(-1)
+    half = (I < nx / 2).reshape(-1)
+    gi = dict(rate=np.where(half, 100 * MMHR_, 0.0), cap=np.where(half, 0.02, 0.0), drain=np.where(half, 5 * MMHR_, 0.0))
+    return dict(nx=nx, ny=ny, dx=dx, z=z, manning=0.03, bc={"west": "outflow"}, gi=gi), [(60 * MMHR_, 1800.0),
+                                                                                         (0.0, 3600.0)]
+
+
+def ga_case():
+    """tests/gpu-suite.js:332-345: Green-Ampt on a 60 x 30 slope with a valley, two soils (K 3e-6 / 1e-6 m/s,
+    psi dtheta 0.03 / 0.08 m) and a shallow water table on half (Fmax 0.02 m); 120 mm/h for 1800 s, then none
+    until 2700 s."""
+    nx, ny, dx = 60, 30, 2
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = (5 - 0.03 * I + 0.06 * np.abs(J - 14.5)).reshape(-1)
+    K = np.where(I < 30, 3e-6, 1e-6).reshape(-1)
+    ga = dict(psiD=np.where(I < 30, 0.03, 0.08).reshape(-1), fmax=np.where(J < 15, 0.02, 1e3).reshape(-1),
+              perv=np.full(nx * ny, 0.8))
+    return dict(nx=nx, ny=ny, dx=dx, z=z, manning=0.04, infil=K, ga=ga,
+                bc={"west": "wall", "east": "outflow", "south": "wall", "north": "wall"}), [(120 * MMHR_, 1800.0),
+                                                                                              (0.0, 2700.0)]

@@ -12,7 +12,7 @@ import sys
 
 CPU_KEYS = ["stoker", "ritter", "lakeSubmerged", "lakeIsland", "thacker", "rainMass", "urban", "valleyDam"]
 GPU_KEYS = ["stoker", "ritter", "lakeSubmerged", "lakeIsland", "thacker", "thackerMass", "rainMass", "urban",
-            "valleyDam", "inlets", "tiles", "manyTiles", "rerun", "lts", "hazard", "stage", "barriers"]
+            "valleyDam", "inlets", "tiles", "manyTiles", "rerun", "lts", "hazard", "stage", "barriers", "gi", "greenAmptScour"]
 
 
 def main(ref):

@@ -440,6 +440,43 @@ def test_barriers(golden):
     np.testing.assert_array_equal(a.field("h"), n.field("h"))
 
 
+def run_schedule(s, schedule):
+    for rate, t in schedule:
+        s.rainRate = rate
+        s.run(t)
+
+
+def test_gi_stores(golden):
+    """GI stores fill to capacity and drain: the water they hold at the end equals the reference CPU's
+    recorded 94.63455931826601 m^3 (results-gpu.json `gi.cpuStored`) to 1e-12; the books close."""
+    opts, sched = rc.gi_case()
+    c = oracle.CPUSolver(**opts, hdry=1e-5)
+    c.setDepth(lambda X, Y, Z: 0 * X)
+    run_schedule(c, sched)
+    store = c.field("giStore").sum() * opts["dx"] ** 2
+    print(f"\nGI: stored {store:.12f} m3 (reference CPU {golden['gpu']['gi']['cpuStored']:.12f}), "
+          f"mass rel {c.massBalance()['rel']:.1e}")
+    assert rel(store, golden["gpu"]["gi"]["cpuStored"]) < 1e-12
+    assert abs(c.massBalance()["rel"]) < 1e-12
+    assert c.field("giStore").max() <= 0.02 + 1e-15
+
+
+def test_green_ampt(golden):
+    """Green-Ampt infiltration: the volume infiltrated equals the reference CPU's recorded
+    126.84261015153905 m^3 (results-gpu.json `greenAmptScour.infilCpu`) to 1e-12; the soil under the
+    shallow water table stops at its Fmax."""
+    opts, sched = rc.ga_case()
+    c = oracle.CPUSolver(**opts, hdry=1e-5)
+    c.setDepth(lambda X, Y, Z: 0 * X)
+    run_schedule(c, sched)
+    print(f"\nGreen-Ampt: infiltrated {c.vol['infil']:.12f} m3 (reference CPU "
+          f"{golden['gpu']['greenAmptScour']['infilCpu']:.12f}), mass rel {c.massBalance()['rel']:.1e}")
+    assert rel(c.vol["infil"], golden["gpu"]["greenAmptScour"]["infilCpu"]) < 1e-12
+    assert abs(c.massBalance()["rel"]) < 1e-12
+    F = c.field("gaF").reshape(opts["ny"], opts["nx"])
+    assert F[:15].max() <= 0.02 + 1e-12 and F[15:].max() > 0.02
+
+
 def test_symmetry_x_y():
     """A dam break along y is the transpose of the one along x (catches a swapped y-face operand)."""
     nx = 60

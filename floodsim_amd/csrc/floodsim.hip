@@ -70,6 +70,10 @@ struct Slot {
   double *barBook = nullptr;
   std::vector<float4> hcrest;            // host copy of the crest plane
   std::vector<int4> bars;                // per barrier: (i, j, axis, k of its right/upper cell)
+  // Green-Ampt soil and GI stores: one separate allocation of up to two float4 planes
+  char *smem = nullptr;
+  size_t sbytes = 0;
+  float4 *soil = nullptr, *gi = nullptr;
   int stageCells = 0;                    // cells flagged MASK_STAGE
   StepRec hrec{};                        // host mirror (refreshed after each launch group)
   double volInitial = 0.0;               // vol.initial (solver-gpu.js:309)
@@ -275,7 +279,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
               (S.p.stepFlux && !S.L ? FLAG_STEPF : 0) |   // the step flux applies under global stepping only
               (S.stageCells ? FLAG_STAGE : 0);
     // extended faces and edges: the step flux, barrier crests or stage cells (one kernel variant)
-    const bool ext = (d.flags & (FLAG_STEPF | FLAG_STAGE)) || S.nbar;
+    const bool ext = (d.flags & (FLAG_STEPF | FLAG_STAGE)) || S.nbar || S.soil || S.gi;
     if (ext) G.anyStepFlux = true;
     G.kinds |= 1u << ((S.hzA ? 2 : 0) + (ext ? 1 : 0));
     d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;
@@ -294,7 +298,8 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.own = own;
     d.rows = rows;
     d.sxn = S.sxn;
-    d.wet = S.nstrips && !S.p.untiled ? S.swet : nullptr;
+    // GI stores drain every step, dry or not: an instance with them steps every strip
+    d.wet = S.nstrips && !S.p.untiled && !S.gi ? S.swet : nullptr;
     d.act = S.sact; d.srain = S.srain; d.sopen = S.sopen;
     d.hzA = S.hzA; d.hzB = S.hzB;
     d.L = S.L; d.tx = S.tx; d.ty = S.ty;
@@ -305,6 +310,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     d.umask = S.umask; d.ucells = S.ucells; d.ulist = S.ulist;
     d.crest = S.crest; d.bidx = S.bidx; d.barFace = S.barFace; d.barGround = S.barGround;
     d.barFlux = S.barFlux; d.barBook = S.barBook; d.nbar = S.nbar;
+    d.soil = S.soil; d.gi = S.gi;
     if (S.nbar) G.withBarriers.push_back((int)gi);
     if (S.L) G.lts = true;
     if (S.hzA) G.anyHazard = true;
@@ -387,8 +393,10 @@ int launch_step(fs_pool *P) {
   }
   if (P->timing) P->spans.push_back({e1, nullptr, -1, -1});  // completed below with e2
   if (G.anyFirstOrder) {
-    if (K & 0x3u) sw_euler_finish<false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    if (K & 0xcu) sw_euler_finish<false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x1u) sw_euler_finish<false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x2u) sw_euler_finish<false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x4u) sw_euler_finish<false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x8u) sw_euler_finish<false, true, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
   }
   if (P->timing) { HIPCHK(P, hipEventRecord(e2, P->stream)); P->spans.back().b = e2; }
   for (int gi : G.withInlets) {
@@ -479,7 +487,7 @@ int launch_cycle(fs_pool *P) {
   for (int k = 0; k < Pn; ++k)
     for (int m = 0; m <= L; ++m) {
       if ((k + 1) % (1 << m)) continue;
-      const bool ext = d.nbar || (d.flags & FLAG_STAGE);
+      const bool ext = d.nbar || (d.flags & FLAG_STAGE) || d.soil || d.gi;
       if (ext) {
         sw_stage<0, true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
         if (d.nbar) sw_barrier_books<0, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
@@ -505,8 +513,14 @@ int launch_cycle(fs_pool *P) {
           sw_stage<1, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
         }
       } else {
-        if (d.hzA) sw_euler_finish<true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
-        else sw_euler_finish<true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        if (ext) {
+          if (d.hzA) sw_euler_finish<true, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          else sw_euler_finish<true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        } else if (d.hzA) {
+          sw_euler_finish<true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        } else {
+          sw_euler_finish<true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+        }
       }
       if (P->timing) {
         HIPCHK(P, hipEventRecord(sb, P->stream));
@@ -983,6 +997,7 @@ void fs_pool_destroy(fs_pool_t P) {
   for (auto &S : P->slots) {
     if (S.used && S.mem) (void)hipFree(S.mem);
     if (S.used && S.bmem) (void)hipFree(S.bmem);
+    if (S.used && S.smem) (void)hipFree(S.smem);
   }
   if (P->grp.dinst) (void)hipFree(P->grp.dinst);
   if (P->grp.dwork) (void)hipFree(P->grp.dwork);
@@ -1042,7 +1057,8 @@ int fs_pool_remove(fs_pool_t P, int32_t slot) {
   HIPCHK(P, hipStreamSynchronize(P->stream));
   HIPCHK(P, hipFree(S.mem));
   if (S.bmem) HIPCHK(P, hipFree(S.bmem));
-  P->used -= S.bytes + S.bbytes;
+  if (S.smem) HIPCHK(P, hipFree(S.smem));
+  P->used -= S.bytes + S.bbytes + S.sbytes;
   S = Slot();
   P->grp.slots.clear();
   drop_graph(P->grp);
@@ -1162,6 +1178,55 @@ int fs_pool_set_crests(fs_pool_t P, int32_t slot, const int32_t *fi, const int32
   HIPCHK(P, hipMemset(S.barFlux, 0, (size_t)n * 4));
   HIPCHK(P, hipMemcpy(S.barBook, book.data(), (size_t)n * 72, hipMemcpyHostToDevice));
   return fs_pool_set_crest_levels(P, slot, z);
+}
+
+// Green-Ampt soil (o.ga: psi dtheta, Fmax, pervious share, where perv > 0; Fmax capped at 1e6 m) and GI
+// stores (o.gi: intake rate, capacity, drain rate, where cap > 0), per interior cell (solver-cpu.js:94-96);
+// both planes live in one allocation of their own, the infiltrated depth F and the stored depth start at 0
+int soil_planes(fs_pool *P, Slot &S) {
+  if (S.smem) return FS_OK;
+  const size_t N = (size_t)S.nx * S.ny, b = 2 * align256(N * 16);
+  if (P->budget && P->used + b > P->budget) return fail(P, FS_E_BUDGET, "soil/GI planes: HBM budget spent");
+  HIPCHK(P, hipMalloc(&S.smem, b));
+  S.sbytes = b;
+  P->used += b;
+  return FS_OK;
+}
+
+int fs_pool_set_ga(fs_pool_t P, int32_t slot, const double *psiD, const double *fmax, const double *perv) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (!psiD || !fmax || !perv) return fail(P, FS_E_ARG, "Green-Ampt: NULL layer");
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  if ((rc = soil_planes(P, S))) return rc;
+  const size_t N = (size_t)S.nx * S.ny;
+  std::vector<float4> v(N, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (size_t k = 0; k < N; ++k)
+    if (perv[k] > 0) v[k] = make_float4((float)psiD[k], (float)std::min(fmax[k], 1e6), (float)perv[k], 0.f);
+  S.soil = (float4 *)S.smem;
+  HIPCHK(P, hipMemcpy(S.soil, v.data(), N * 16, hipMemcpyHostToDevice));
+  P->grp.slots.clear();   // the instance's kernel variant changes
+  drop_graph(P->grp);
+  return FS_OK;
+}
+
+int fs_pool_set_gi(fs_pool_t P, int32_t slot, const double *rate, const double *cap, const double *drain) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  Slot &S = P->slots[slot];
+  if (!rate || !cap || !drain) return fail(P, FS_E_ARG, "GI: NULL layer");
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  if ((rc = soil_planes(P, S))) return rc;
+  const size_t N = (size_t)S.nx * S.ny;
+  std::vector<float4> v(N, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (size_t k = 0; k < N; ++k)
+    if (cap[k] > 0) v[k] = make_float4((float)rate[k], (float)cap[k], (float)drain[k], 0.f);
+  S.gi = (float4 *)(S.smem + align256(N * 16));
+  HIPCHK(P, hipMemcpy(S.gi, v.data(), N * 16, hipMemcpyHostToDevice));
+  P->grp.slots.clear();
+  drop_graph(P->grp);
+  return FS_OK;
 }
 
 // setCrestLevels (solver-gpu.js:486-488): new crest per barrier face, in the order given
@@ -1303,6 +1368,13 @@ int fs_pool_read_state(fs_pool_t P, int32_t slot, double *h, double *hu, double 
     m.exchanged = S.exchangedRead + pending;
     m.clipped = r.clipped;
     m.initial = S.volInitial;
+    m.giStored = 0.0;   // water the GI stores hold (m^3; solver-gpu.js:683-689 mass.giStored)
+    if (S.gi) {
+      std::vector<float4> g(N);
+      HIPCHK(P, hipMemcpy(g.data(), S.gi, N * 16, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < N; ++k) m.giStored += (double)g[k].w;
+      m.giStored *= A;
+    }
     m.expected = S.volInitial + r.rain - r.infil - r.outflow - m.exchanged;
     // scale as readState (solver-gpu.js:685)
     const double scale = std::max(1e-30, std::max(std::fabs(m.expected),
@@ -1403,6 +1475,12 @@ int fs_set_crest_levels(fs_handle s, const double *z) {
   return s ? fs_pool_set_crest_levels(s->pool, s->slot, z) : FS_E_ARG;
 }
 int fs_read_barriers(fs_handle s, double *out) { return s ? fs_pool_read_barriers(s->pool, s->slot, out) : FS_E_ARG; }
+int fs_set_ga(fs_handle s, const double *psiD, const double *fmax, const double *perv) {
+  return s ? fs_pool_set_ga(s->pool, s->slot, psiD, fmax, perv) : FS_E_ARG;
+}
+int fs_set_gi(fs_handle s, const double *rate, const double *cap, const double *drain) {
+  return s ? fs_pool_set_gi(s->pool, s->slot, rate, cap, drain) : FS_E_ARG;
+}
 
 int fs_set_rain_field(fs_handle s, const double *field) {
   if (!s || !field) return FS_E_ARG;

@@ -387,6 +387,25 @@ static napi_value js_read_barriers(napi_env env, napi_callback_info info) {
   return rc ? fail(env, s->h, rc) : r;
 }
 
+/* setGa(h, Float64Array psiD, fmax, perv) -> fs_set_ga; setGi(h, Float64Array rate, cap, drain) -> fs_set_gi */
+static napi_value js_set_layers3(napi_env env, napi_callback_info info, int gi) {
+  size_t argc = 4;
+  napi_value a[4];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  void *p[3];
+  size_t n0 = 0, n = 0;
+  if (typed(env, a[1], napi_float64_array, -1, &p[0], &n0, gi ? "gi.rate" : "ga.psiD")) return NULL;
+  if (typed(env, a[2], napi_float64_array, (long)n0, &p[1], &n, gi ? "gi.cap" : "ga.fmax")) return NULL;
+  if (typed(env, a[3], napi_float64_array, (long)n0, &p[2], &n, gi ? "gi.drain" : "ga.perv")) return NULL;
+  int rc = gi ? fs_set_gi(s->h, (const double *)p[0], (const double *)p[1], (const double *)p[2])
+              : fs_set_ga(s->h, (const double *)p[0], (const double *)p[1], (const double *)p[2]);
+  return rc ? fail(env, s->h, rc) : undefined_value(env);
+}
+static napi_value js_set_ga(napi_env env, napi_callback_info info) { return js_set_layers3(env, info, 0); }
+static napi_value js_set_gi(napi_env env, napi_callback_info info) { return js_set_layers3(env, info, 1); }
+
 static napi_value js_clock(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
@@ -553,7 +572,7 @@ static void work_complete(napi_env env, napi_status status, void *data) {
         set_num(env, m, "volume", w->mass.volume);
         set_num(env, m, "rain", w->mass.rain);
         set_num(env, m, "infil", w->mass.infil);
-        set_num(env, m, "giStored", 0.0);
+        set_num(env, m, "giStored", w->mass.giStored);
         set_num(env, m, "outflow", w->mass.outflow);
         set_num(env, m, "exchanged", w->mass.exchanged);
         set_num(env, m, "clipped", w->mass.clipped);
@@ -797,6 +816,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setStageCells", NULL, js_set_stage_cells, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setStage", NULL, js_set_stage, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setCrests", NULL, js_set_crests, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setGa", NULL, js_set_ga, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setGi", NULL, js_set_gi, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setCrestLevels", NULL, js_set_crest_levels, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readBarriers", NULL, js_read_barriers, NULL, NULL, NULL, napi_enumerable, NULL},
       {"clock", NULL, js_clock, NULL, NULL, NULL, napi_enumerable, NULL},

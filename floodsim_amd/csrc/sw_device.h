@@ -151,6 +151,9 @@ struct DevInst {
   float *barFlux;
   double *barBook;
   int nbar;
+  // Green-Ampt soil (psi dtheta, Fmax, pervious share, F infiltrated so far) and GI stores (intake rate,
+  // capacity, drain rate, stored depth) per cell; null: none
+  float4 *soil, *gi;
 };
 
 // per wavefront strip and stage: boundary outflow (m^3), clipped (m^3), infiltration (m^3), max

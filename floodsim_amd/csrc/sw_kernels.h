@@ -395,6 +395,11 @@ __device__ __forceinline__ float cell_speed(float h, float hu, float hv, float g
 // Friction, rain, infiltration and max depth of one cell after its update, then the cell's wave
 // speed for the next dt (sourcesAt, kernels.wgsl.js:544-590; _sources, solver-cpu.js:456-483).
 // Semi-implicit Manning: (hu, hv) /= 1 + dt g n^2 |u| / h^(4/3).
+// EXT: Green-Ampt soil and GI stores (solver-cpu.js:466-474): the infiltration capacity K dt becomes
+// K dt (1 + psi dtheta / F), none once the soil above the water table is full, and F grows by what went in
+// over the pervious share; a GI store drains linearly, then takes what it can at its intake rate up to its
+// capacity, which is booked as infiltration
+template <bool EXT = false>
 __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, float rain, float areaf,
                                              float &qh, float &qw, float &qhu, float &qhv, double &acc_inf,
                                              float &smax) {
@@ -413,11 +418,41 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, 
   }
   const float rr = (I.flags & FLAG_RF) ? rain * gld(I.rf, k) * dt : rain * dt;
   float inf = 0.f;
-  if (I.flags & FLAG_INFIL) inf = fminf(gld(I.infil, k) * dt, d + rr);
+  if constexpr (!EXT) {
+    if (I.flags & FLAG_INFIL) inf = fminf(gld(I.infil, k) * dt, d + rr);
+  }
+  float4 ga = make_float4(0.f, 0.f, 0.f, 0.f), gi = make_float4(0.f, 0.f, 0.f, 0.f);
+  float soilIn = 0.f;
+  if constexpr (EXT) {
+    float cap = (I.flags & FLAG_INFIL) ? gld(I.infil, k) * dt : 0.f;
+    if (I.soil) {
+      ga = gld(I.soil, k);
+      if (ga.z > 0.f) cap = ga.w >= ga.y ? 0.f : fminf(cap * (1.f + ga.x / fmaxf(ga.w, 1e-3f)), (ga.y - ga.w) * ga.z);
+    }
+    inf = fminf(cap, d + rr);
+    soilIn = inf;
+    if (I.gi) {
+      gi = gld(I.gi, k);
+      if (gi.y > 0.f) {
+        const float st = fmaxf(0.f, gi.w - gi.z * dt);
+        const float g2 = fminf(fminf(gi.x * dt, d + rr - inf), gi.y - st);
+        gi.w = st + fmaxf(0.f, g2);
+        if (g2 > 0.f) inf += g2;
+      }
+    }
+  }
   float nh = qh, nw = qw;
   kadd(nh, nw, rr - inf);
   if (nh < 0.f) { inf = d - qw + rr; qh = 0.f; qw = 0.f; }
   else { qh = nh; qw = nw; }
+  if constexpr (EXT) {
+    // the soil takes its share of what actually left the cell (FP32 rounding can clip the sum)
+    if (I.soil && ga.z > 0.f) {
+      ga.w += fminf(soilIn, inf) / ga.z;
+      gst(reinterpret_cast<fs_v4 *>(I.soil), k, fs_v4{ga.x, ga.y, ga.z, ga.w});
+    }
+    if (I.gi && gi.y > 0.f) gst(reinterpret_cast<fs_v4 *>(I.gi), k, fs_v4{gi.x, gi.y, gi.z, gi.w});
+  }
   acc_inf += (double)inf * areaf;
   if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
   const float hm = gld(I.hmax, k);
@@ -776,7 +811,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             acc_out += o * ((double)dx * (double)dt * (double)wgt);
           }
           if (STAGE == 1) {
-            sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+            sources_cell<EXT>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
             if constexpr (HZ) {
               if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
             }
@@ -931,7 +966,9 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
 // of its kind (hazard products: second stage only; extended faces: step flux or barriers) and leaves the
 // others' strips alone, so a group of mixed kinds runs one launch per kind present and every instance gets
 // the same machine code, hence the same bits, alone and in any mix.
-__device__ __forceinline__ bool ext_faces(const DevInst &I) { return (I.flags & (FLAG_STEPF | FLAG_STAGE)) || I.crest; }
+__device__ __forceinline__ bool ext_faces(const DevInst &I) {
+  return (I.flags & (FLAG_STEPF | FLAG_STAGE)) || I.crest || I.soil || I.gi;
+}
 
 template <int STAGE, bool LTS, bool HZ = false, bool EXT = false>
 __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
@@ -1350,7 +1387,7 @@ __global__ __launch_bounds__(256) void sw_speed(const DevInst *__restrict__ inst
 // first order (secondOrder = false): the Euler stage's U* plus the sources becomes U^{n+1}
 // (step, solver-cpu.js:432-438: one stage, then _sources). LTS: the level-m cells of a record,
 // whose store is then spent (solver-cpu.js:550-557).
-template <bool LTS, bool HZ>
+template <bool LTS, bool HZ, bool EXT = false>
 __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 wk, int lane, int m, float dt, Part *part,
                                                    int cells, float tnow) {
   double acc_inf = 0.0;
@@ -1364,7 +1401,7 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
       if (!(gld(I.mask, k) & MASK_OPEN)) continue;
       if (LTS && tile_level(I, X, y) != m) continue;
       float qh = gld(I.h1, k), qw = gld(I.w1, k), qhu = gld(I.hu1, k), qhv = gld(I.hv1, k);
-      sources_cell(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+      sources_cell<EXT>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
       if constexpr (HZ) {
         if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
       }
@@ -1379,7 +1416,7 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
   return __any(stir);
 }
 
-template <bool LTS, bool HZ = false>
+template <bool LTS, bool HZ = false, bool EXT = false>
 __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict__ insts,
                                                        const int4 *__restrict__ work, int nwork,
                                                        Part *__restrict__ part, int m) {
@@ -1390,15 +1427,15 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
     const int4 wk = work[wid];
     const DevInst &I = insts[wk.x];
     if (I.secondOrder) return;  // second order: sw_stage<1> writes this strip's partial
-    if ((I.hzA != nullptr) != HZ) return;  // the other variant advances it (same code alone and mixed)
+    if ((I.hzA != nullptr) != HZ || ext_faces(I) != EXT) return;  // another variant advances it
     const int u = wid - I.wbeg;
     if (I.wet && !I.act[u]) {
       if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
       return;
     }
     const float dt = I.rec->dt;
-    const bool stir = euler_finish_strip<false, HZ>(I, wk, lane, 0, dt, part + wid, dt > 0.f ? I.sopen[u] : 0,
-                                                (float)I.rec->tAfter);
+    const bool stir = euler_finish_strip<false, HZ, EXT>(I, wk, lane, 0, dt, part + wid, dt > 0.f ? I.sopen[u] : 0,
+                                                     (float)I.rec->tAfter);
     if (I.wet && dt > 0.f && lane == 0) I.wet[u] = stir ? 1 : 0;
   } else {
     const DevInst &I = insts[0];
@@ -1411,7 +1448,7 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
     if (i < 0) return;
     const int u = I.ulist[m * (I.wend - I.wbeg) + i / split];
     const int4 wk = lts_rows(I, work[I.wbeg + u], m, lane, split, i % split);
-    euler_finish_strip<true, HZ>(I, wk, lane, m, wk.w > 0 ? dt : 0.f, part + i, 0, lts_record_end_time(I, kk));
+    euler_finish_strip<true, HZ, EXT>(I, wk, lane, m, wk.w > 0 ? dt : 0.f, part + i, 0, lts_record_end_time(I, kk));
   }
 }
 

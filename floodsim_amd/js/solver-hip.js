@@ -13,7 +13,8 @@
  * - constructor options nx, ny, dx, z, manning (number or array), mask, infil (number or array),
  *   rainField, bc {west, east, south, north: 'wall'|'open'|'outflow'}, secondOrder, limiter, cfl,
  *   hdry, hfront, dtDry, g, inlets, lts (true = 3 levels, or 1-4), tiles, implicitFriction, stepFlux,
- *   stageCells, crest (solver-gpu.js:18-20, 35-47, 125-133, 144-151);
+ *   stageCells, crest, ga {psiD, fmax, perv}, gi {rate, cap, drain} (solver-gpu.js:18-20, 35-47, 125-133,
+ *   144-151; solver-cpu.js:94-96);
  * - setDepth(fn(x, y, z)) and setVelocity(fn(x, y)) evaluated at cell centres, depth clamped at 0
  *   and 0 in walls, vol.initial the sum of the stored FP32 depths (solver-gpu.js:301-318);
  * - rainRate (m/s), setRainField, setStage(z) (m, the level the stage cells' ghosts hold),
@@ -27,7 +28,7 @@
  *
  * The device argument is accepted for signature compatibility and ignored: the library drives
  * HIP device 0 of the process. Options of the reference outside the shallow-water step
- * (subgrid, sink, gi, sediment, debris) raise. implicitFriction and stepFlux (F26) are taken
+ * (subgrid, sink, sediment, debris) raise. implicitFriction and stepFlux (F26) are taken
  * as the reference takes them (stepFlux has no effect under local time stepping).
  *
  * Calls on one solver run one at a time: asynchronous calls queue behind each other, and the
@@ -40,7 +41,7 @@ const addon = require(path.join(__dirname, 'floodsim_napi.node'));
 
 const GH = 2; // the reference's ghost width, kept for idx(i, j) and zh
 const BC = { wall: 0, open: 1, outflow: 2 };
-const UNSUPPORTED = ['subgrid', 'sink', 'gi', 'sediment', 'debris'];
+const UNSUPPORTED = ['subgrid', 'sink', 'sediment', 'debris'];
 const BARRIER_BOOKS = ['plus', 'minus', 'peak', 'level', 'load', 'tPeak', 'tLevel', 'tLoad', 'tOver'];
 
 function def(v, d) { return v === undefined || v === null ? d : v; }
@@ -101,6 +102,9 @@ class GPUSolver {
     this._h = addon.create(params, z, manArr, manS, this._mask, infil, rf, inlets);
     this.stage = -1e9;
     if (o.stageCells && o.stageCells.length) addon.setStageCells(this._h, Int32Array.from(o.stageCells));
+    // Green-Ampt soil and GI stores per cell (solver-cpu.js:94-96)
+    if (o.ga) addon.setGa(this._h, f64(o.ga.psiD, n, 'ga.psiD'), f64(o.ga.fmax, n, 'ga.fmax'), f64(o.ga.perv, n, 'ga.perv'));
+    if (o.gi) addon.setGi(this._h, f64(o.gi.rate, n, 'gi.rate'), f64(o.gi.cap, n, 'gi.cap'), f64(o.gi.drain, n, 'gi.drain'));
     // flood barriers: faces {i, j, axis, z, C, ground} (solver-gpu.js:18-20, 144-151)
     this.crest = (o.crest || []).slice();
     this.nBar = this.crest.length;

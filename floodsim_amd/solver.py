@@ -45,7 +45,8 @@ class fs_clock(ctypes.Structure):
 
 class fs_mass(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in
-                ("volume", "rain", "infil", "outflow", "exchanged", "clipped", "expected", "rel", "relExClip", "initial")]
+                ("volume", "rain", "infil", "outflow", "exchanged", "clipped", "expected", "rel", "relExClip", "initial",
+                 "giStored")]
 
 
 EXPORTS = [
@@ -59,7 +60,7 @@ EXPORTS = [
     "fs_restore", "fs_pool_snapshot_bytes", "fs_pool_snapshot", "fs_pool_restore", "fs_read_hazard",
     "fs_set_stage_cells", "fs_set_stage", "fs_pool_set_stage_cells", "fs_pool_set_stage",
     "fs_set_crests", "fs_set_crest_levels", "fs_read_barriers", "fs_pool_set_crests", "fs_pool_set_crest_levels",
-    "fs_pool_read_barriers",
+    "fs_pool_read_barriers", "fs_set_ga", "fs_set_gi", "fs_pool_set_ga", "fs_pool_set_gi",
 ]
 
 
@@ -118,6 +119,10 @@ def lib(path=None):
     L.fs_pool_set_crests.argtypes = [P, I32, ip, ip, ip, dp, dp, dp, I32]
     L.fs_pool_set_crest_levels.argtypes = [P, I32, dp]
     L.fs_pool_read_barriers.argtypes = [P, I32, dp]
+    L.fs_set_ga.argtypes = [P, dp, dp, dp]
+    L.fs_set_gi.argtypes = [P, dp, dp, dp]
+    L.fs_pool_set_ga.argtypes = [P, I32, dp, dp, dp]
+    L.fs_pool_set_gi.argtypes = [P, I32, dp, dp, dp]
     L.fs_pool_advance.argtypes = [P, ip, dp, I32, I32, ctypes.POINTER(fs_clock)]
     L.fs_pool_step.argtypes = [P, ip, dp, I32, I32]
     L.fs_pool_read_state.argtypes = [P, I32, dp, dp, dp, dp, ctypes.POINTER(fs_mass)]
@@ -223,7 +228,7 @@ class GPUSolver:
     """The reference GPUSolver interface over the HIP library (one instance)."""
 
     def __init__(self, nx, ny, dx, z, manning=0.0, mask=None, infil=None, rainField=None, inlets=None,
-                 stageCells=None, crest=None, **opts):
+                 stageCells=None, crest=None, ga=None, gi=None, **opts):
         L = lib()
         self.nx, self.ny, self.dx = int(nx), int(ny), float(dx)
         self.p = make_params(nx, ny, dx, **opts)
@@ -243,6 +248,11 @@ class GPUSolver:
         if crest:
             self.nBar = len(crest)
             self._chk(L.fs_set_crests(h, *_crest_args(crest)))
+        n = self.nx * self.ny
+        if ga is not None:
+            self._chk(L.fs_set_ga(h, *[_dp(_f64(ga[k], n)) for k in ("psiD", "fmax", "perv")]))
+        if gi is not None:
+            self._chk(L.fs_set_gi(h, *[_dp(_f64(gi[k], n)) for k in ("rate", "cap", "drain")]))
         self.nInlets = len(self.layers.inlets)
         self._rain = 0.0
         self._Uh = np.zeros(self.nx * self.ny)
@@ -477,7 +487,7 @@ class Pool:
         return lib().fs_pool_bytes_used(self._p)
 
     def add(self, nx, ny, dx, z, manning=0.0, mask=None, infil=None, rainField=None, inlets=None, stageCells=None,
-            crest=None, **opts):
+            crest=None, ga=None, gi=None, **opts):
         """Admit one instance; returns None when the HBM budget is spent."""
         p = make_params(nx, ny, dx, **opts)
         lay = _Layers(int(nx), int(ny), z, manning, mask, infil, rainField, inlets)
@@ -492,6 +502,11 @@ class Pool:
         if crest:
             self._chk(lib().fs_pool_set_crests(self._p, slot.value, *_crest_args(crest)))
             self.nbar[slot.value] = len(crest)
+        n = int(nx) * int(ny)
+        if ga is not None:
+            self._chk(lib().fs_pool_set_ga(self._p, slot.value, *[_dp(_f64(ga[k], n)) for k in ("psiD", "fmax", "perv")]))
+        if gi is not None:
+            self._chk(lib().fs_pool_set_gi(self._p, slot.value, *[_dp(_f64(gi[k], n)) for k in ("rate", "cap", "drain")]))
         return slot.value
 
     def set_crest_levels(self, slot, zs):

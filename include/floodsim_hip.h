@@ -68,6 +68,7 @@ typedef struct fs_clock {
 /* readState().mass of GPUSolver (solver-gpu.js:683-689), FP64 books */
 typedef struct fs_mass {
   double volume, rain, infil, outflow, exchanged, clipped, expected, rel, relExClip, initial;
+  double giStored;     /* water the GI stores hold, m^3 (counted in infil, as the reference books it) */
 } fs_mass;
 
 typedef struct fs_pool *fs_pool_t;
@@ -101,6 +102,11 @@ int fs_set_crest_levels(fs_handle s, const double *z);     /* setCrestLevels (so
 /* readBarriers (solver-gpu.js:490-504): 9 doubles per face: plus, minus (m^3), peak (m^2/s), level (m,
  * -inf = never wet), load (kN/m), tPeak, tLevel, tLoad, tOver (s, NaN = never overtopped) */
 int fs_read_barriers(fs_handle s, double *out);
+/* the ga option (solver-cpu.js:96): Green-Ampt soil per cell (psi dtheta m, Fmax m, pervious share; cells
+ * with perv > 0); the infiltration capacity K of the infil layer becomes K (1 + psi dtheta / F) */
+int fs_set_ga(fs_handle s, const double *psiD, const double *fmax, const double *perv);
+/* the gi option (solver-cpu.js:95): GI stores per cell (intake m/s, capacity m, drain m/s; cells with cap > 0) */
+int fs_set_gi(fs_handle s, const double *rate, const double *cap, const double *drain);
 int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n);        /* readInlets (:564-576) */
 int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk);     /* advance (:589-604) */
 /* frame (:607-622): at most K steps toward tEnd; out (nx*ny*2: depth, speed) may be NULL */
@@ -145,6 +151,8 @@ int fs_pool_set_crests(fs_pool_t pool, int32_t slot, const int32_t *i, const int
                        const double *z, const double *C, const double *ground, int32_t n);
 int fs_pool_set_crest_levels(fs_pool_t pool, int32_t slot, const double *z);
 int fs_pool_read_barriers(fs_pool_t pool, int32_t slot, double *out);
+int fs_pool_set_ga(fs_pool_t pool, int32_t slot, const double *psiD, const double *fmax, const double *perv);
+int fs_pool_set_gi(fs_pool_t pool, int32_t slot, const double *rate, const double *cap, const double *drain);
 /* advance the listed slots together until each reaches its own target (K steps per launch group);
  * clks (n entries) may be NULL */
 int fs_pool_advance(fs_pool_t pool, const int32_t *slots, const double *tTargets, int32_t n, int32_t K,

@@ -279,7 +279,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
               (S.p.stepFlux && !S.L ? FLAG_STEPF : 0) |   // the step flux applies under global stepping only
               (S.stageCells ? FLAG_STAGE : 0);
     // extended faces and edges: the step flux, barrier crests or stage cells (one kernel variant)
-    const bool ext = (d.flags & (FLAG_STEPF | FLAG_STAGE)) || S.nbar || S.soil || S.gi;
+    const bool ext = (d.flags & (FLAG_STEPF | FLAG_STAGE)) || S.nbar || S.soil || S.gi || S.p.limiter;
     if (ext) G.anyStepFlux = true;
     G.kinds |= 1u << ((S.hzA ? 2 : 0) + (ext ? 1 : 0));
     d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;
@@ -487,7 +487,7 @@ int launch_cycle(fs_pool *P) {
   for (int k = 0; k < Pn; ++k)
     for (int m = 0; m <= L; ++m) {
       if ((k + 1) % (1 << m)) continue;
-      const bool ext = d.nbar || (d.flags & FLAG_STAGE) || d.soil || d.gi;
+      const bool ext = d.nbar || (d.flags & FLAG_STAGE) || d.soil || d.gi || d.limiter;
       if (ext) {
         sw_stage<0, true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
         if (d.nbar) sw_barrier_books<0, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);

@@ -520,7 +520,8 @@ struct FaceRow { float F0, F1, F2, cL, cR, on; };
 // stepping cell takes, for a face whose neighbour is finer, its reconstruction from that face's last
 // evaluation (kept in bsx/bsy by the finer record's last stage).
 // EXT: extended faces and edges, for instances with the step-flux rules (F26; global stepping only), flood
-// barriers (crests in the face beds, weir faces, the barrier faces' fluxes for their books) or stage cells
+// barriers (crests in the face beds, weir faces, the barrier faces' fluxes for their books), stage cells,
+// Green-Ampt soil or GI stores, or the van Leer limiter
 template <int STAGE, bool LTS, bool HZ, bool EXT = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
                                           Part *__restrict__ part, const int cells, const float tnow) {
@@ -529,7 +530,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   const bool bar = EXT && I.crest != nullptr;
   const float rain = I.rec->rainRate;
   const float g = I.g, hdry = I.hdry, hfront = I.hfront, inv_dx = I.inv_dx, dx = I.dx;
-  const int so = I.secondOrder, vl = I.limiter;
+  // the van Leer limiter runs in the extended variant only: the plain kernels are minmod at compile time
+  const int so = I.secondOrder, vl = EXT ? I.limiter : 0;
   const int nx = I.nx, ny = I.ny, pitch = I.pitch;
   const int X = wk.y - I.hl + lane;
   const bool own_col = lane >= I.hl && lane < I.hl + I.own && X < nx;
@@ -967,7 +969,7 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
 // others' strips alone, so a group of mixed kinds runs one launch per kind present and every instance gets
 // the same machine code, hence the same bits, alone and in any mix.
 __device__ __forceinline__ bool ext_faces(const DevInst &I) {
-  return (I.flags & (FLAG_STEPF | FLAG_STAGE)) || I.crest || I.soil || I.gi;
+  return (I.flags & (FLAG_STEPF | FLAG_STAGE)) || I.crest || I.soil || I.gi || I.limiter;
 }
 
 template <int STAGE, bool LTS, bool HZ = false, bool EXT = false>

@@ -279,7 +279,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
               (S.p.stepFlux && !S.L ? FLAG_STEPF : 0) |   // the step flux applies under global stepping only
               (S.stageCells ? FLAG_STAGE : 0);
     // extended faces and edges: the step flux, barrier crests or stage cells (one kernel variant)
-    const bool ext = (d.flags & (FLAG_STEPF | FLAG_STAGE)) || S.nbar || S.soil || S.gi || S.p.limiter;
+    const bool ext = (d.flags & (FLAG_IMPF | FLAG_STEPF | FLAG_STAGE)) || S.nbar || S.soil || S.gi || S.p.limiter;
     if (ext) G.anyStepFlux = true;
     G.kinds |= 1u << ((S.hzA ? 2 : 0) + (ext ? 1 : 0));
     d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;

@@ -263,7 +263,7 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m,
   p.u = wet ? p.hu * ih : 0.f;
   p.v = wet ? p.hv * ih : 0.f;
   // a ghost copies its source cell after the source's friction
-  if (STAGE == 1 && (I.flags & FLAG_IMPF) && fromB && srcOpen) pre_friction(I, (int)k, p.h, dt, p.u, p.v);
+  if (STAGE == 1 && (EXT || LTS) && (I.flags & FLAG_IMPF) && fromB && srcOpen) pre_friction(I, (int)k, p.h, dt, p.u, p.v);
   if (gx != gy) {  // edge ghost (not a corner): free outfall
     int e = -1, i = -1, side = -1;
     if (X < 0 && I.bc[0] == BC_OUTFLOW) { e = 0; i = 1; side = 0; }
@@ -311,7 +311,7 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m,
 
 // Primitive of interior cell (X, Y) for a row whose 64 lanes are all interior cells (the common
 // case): row-based global loads, no ghost logic
-template <int STAGE, bool LTS>
+template <int STAGE, bool LTS, bool EXT = false>
 __device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, int m, float dt) {
   const int k = Y * I.pitch + X;
   Prim p;
@@ -333,7 +333,7 @@ __device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, in
   const bool wet = p.h > I.hdry;
   p.u = wet ? hu * ih : 0.f;
   p.v = wet ? hv * ih : 0.f;
-  if (STAGE == 1 && (I.flags & FLAG_IMPF) && useB && p.M) pre_friction(I, k, p.h, dt, p.u, p.v);
+  if (STAGE == 1 && (EXT || LTS) && (I.flags & FLAG_IMPF) && useB && p.M) pre_friction(I, k, p.h, dt, p.u, p.v);
   return p;
 }
 
@@ -399,7 +399,9 @@ __device__ __forceinline__ float cell_speed(float h, float hu, float hv, float g
 // K dt (1 + psi dtheta / F), none once the soil above the water table is full, and F grows by what went in
 // over the pervious share; a GI store drains linearly, then takes what it can at its intake rate up to its
 // capacity, which is booked as infiltration
-template <bool EXT = false>
+// IMPF: the variant takes implicit friction (F26): the extended variants, and every LTS variant (measured:
+// taking the test out of the plain LTS kernels slows their schedule)
+template <bool EXT = false, bool IMPF = EXT>
 __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, float rain, float areaf,
                                              float &qh, float &qw, float &qhu, float &qhv, double &acc_inf,
                                              float &smax) {
@@ -412,7 +414,7 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, 
     const float af = dt * g * nn * nn * frcp(fpow(d, 4.f / 3.f));
     const float sp = fsq(u * u + v * v);
     // F26 (implicitFriction): implicit in the new speed, k = (1 + sqrt(1 + 4 a |u|)) / 2 (solver-cpu.js:465-467)
-    const float kf = (I.flags & FLAG_IMPF) ? frcp(0.5f * (1.f + fsq(1.f + 4.f * af * sp))) : frcp(1.f + af * sp);
+    const float kf = (IMPF && (I.flags & FLAG_IMPF)) ? frcp(0.5f * (1.f + fsq(1.f + 4.f * af * sp))) : frcp(1.f + af * sp);
     qhu *= kf;
     qhv *= kf;
   }
@@ -521,7 +523,8 @@ struct FaceRow { float F0, F1, F2, cL, cR, on; };
 // evaluation (kept in bsx/bsy by the finer record's last stage).
 // EXT: extended faces and edges, for instances with the step-flux rules (F26; global stepping only), flood
 // barriers (crests in the face beds, weir faces, the barrier faces' fluxes for their books), stage cells,
-// Green-Ampt soil or GI stores, or the van Leer limiter
+// Green-Ampt soil or GI stores, implicit friction (F26; global stepping: under LTS every variant takes it)
+// or the van Leer limiter
 template <int STAGE, bool LTS, bool HZ, bool EXT = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
                                           Part *__restrict__ part, const int cells, const float tnow) {
@@ -574,7 +577,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   for (int r = y0 - kHalo; r < y1 + kHalo; ++r) {
     // ---------------- row r: primitives
     Prim p;
-    if (xin && r >= 0 && r < ny) p = load_prim_row<STAGE, LTS>(I, X, r, m, dt);
+    if (xin && r >= 0 && r < ny) p = load_prim_row<STAGE, LTS, EXT>(I, X, r, m, dt);
     else p = load_prim<STAGE, LTS, EXT>(I, X, r, m, dt);
 
     // ---------------- x reconstruction of row r (neighbours by lane shifts)
@@ -813,7 +816,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             acc_out += o * ((double)dx * (double)dt * (double)wgt);
           }
           if (STAGE == 1) {
-            sources_cell<EXT>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+            sources_cell<EXT, EXT || LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
             if constexpr (HZ) {
               if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
             }
@@ -969,7 +972,7 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
 // others' strips alone, so a group of mixed kinds runs one launch per kind present and every instance gets
 // the same machine code, hence the same bits, alone and in any mix.
 __device__ __forceinline__ bool ext_faces(const DevInst &I) {
-  return (I.flags & (FLAG_STEPF | FLAG_STAGE)) || I.crest || I.soil || I.gi || I.limiter;
+  return (I.flags & (FLAG_IMPF | FLAG_STEPF | FLAG_STAGE)) || I.crest || I.soil || I.gi || I.limiter;
 }
 
 template <int STAGE, bool LTS, bool HZ = false, bool EXT = false>
@@ -1403,7 +1406,7 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
       if (!(gld(I.mask, k) & MASK_OPEN)) continue;
       if (LTS && tile_level(I, X, y) != m) continue;
       float qh = gld(I.h1, k), qw = gld(I.w1, k), qhu = gld(I.hu1, k), qhv = gld(I.hv1, k);
-      sources_cell<EXT>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+      sources_cell<EXT, EXT || LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
       if constexpr (HZ) {
         if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
       }

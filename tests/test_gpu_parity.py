@@ -325,3 +325,34 @@ def test_pool_alone_vs_mixed(fsa):
         for k in ("h", "hu", "hv", "hmax"):
             np.testing.assert_array_equal(a[k], m[k])
         assert abs(a["mass"]["rel"]) <= TOL_MASS
+
+
+@pytest.mark.parametrize("lts", [0, 3])
+def test_rain_field(fsa, lts):
+    """A per-cell rain multiplier (rainField; solver-cpu.js:470, kernels.wgsl.js:28, 561) on the urban
+    cloudburst, global and local time stepping (the LTS books sum the multiplier per tile and level):
+    against the oracle, and the rain booked equal to rate x T x sum of the multiplier over open cells."""
+    sc = rc.cloudburst(80)
+    n = sc["nx"] * sc["ny"]
+    rf = np.random.default_rng(5).uniform(0.0, 2.0, n)
+    rf[: n // 7] = 0.0
+    opts = dict(nx=sc["nx"], ny=sc["ny"], dx=sc["dx"], z=sc["z"], mask=sc["mask"], manning=sc["manning"],
+                bc=sc["bc"], infil=sc["infil"], rainField=rf, lts=lts)
+    g = fsa.GPUSolver(**opts)
+    c = oracle.CPUSolver(**opts, hdry=g.hdry)
+    rate = 150 * rc.MMHR
+    for o in (g, c):
+        o.setDepth(sc["depth"])
+        o.rainRate = rate
+    for t in np.arange(10, 300 + 1e-9, 10):
+        g.advance(float(t), 8)
+        c.run(float(t))
+    s = g.readState()
+    d = float(np.max(np.abs(s["h"] - c.field("h"))))
+    report(f"rain field lts {lts}", g, c, s, d)
+    open_ = np.asarray(sc["mask"]).reshape(-1) != 0
+    expect = rate * 300 * rf[open_].sum() * sc["dx"] ** 2
+    print(f"  rain booked {s['mass']['rain']:.6f} m3, oracle {c.vol['rain']:.6f}, closed form {expect:.6f}")
+    assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS and s["mass"]["clipped"] == 0
+    assert abs(s["mass"]["rain"] - expect) <= 1e-6 * expect
+    assert abs(c.vol["rain"] - expect) <= 1e-12 * expect

@@ -168,6 +168,40 @@ def test_rain_on_grid(golden):
     assert abs(mb["rel"]) < 1e-12 and mb["clipped"] == 0
 
 
+def rain_field_basin(lts=0, field="random"):
+    nx, ny, dx = 40, 30, 2.0
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = 0.2 * np.sin(I / 5) + 0.1 * np.cos(J / 4)
+    r = np.random.default_rng(3).uniform(0, 2, (ny, nx))
+    r[5:9, 10:20] = 0
+    rf = {"random": r, "ones": np.ones((ny, nx)), "none": None}[field]
+    s = oracle.CPUSolver(nx=nx, ny=ny, dx=dx, z=z, manning=0.03, lts=lts, rainField=rf)
+    s.setDepth(lambda X, Y, Z: 0 * X)
+    s.rainRate = 1e-4
+    return s, r, dx
+
+
+@pytest.mark.parametrize("lts", [0, 3])
+def test_rain_field(lts):
+    """The per-cell rain multiplier (rainField, solver-cpu.js:470). The reference records no non-uniform field: the checks
+    are what the mass books fix. In a walled basin the first step from dry holds no flow, so each cell
+    holds its multiplier x rate x dt; the rain booked over 600 s is rate x 600 x sum(r) dx^2 and the
+    books close; a field of ones is bit for bit no field."""
+    s, r, dx = rain_field_basin(lts)
+    s.run(0.5)
+    assert s.steps == 1
+    assert np.array_equal(s.field("h"), r.reshape(-1) * 1e-4 * 0.5)
+    s.run(600)
+    assert rel(s.vol["rain"], 1e-4 * 600 * r.sum() * dx * dx) < 1e-12
+    assert abs(s.massBalance()["rel"]) < 1e-12
+    ones, _, _ = rain_field_basin(lts, "ones")
+    none, _, _ = rain_field_basin(lts, "none")
+    for o in (ones, none):
+        o.run(300)
+    for k in ("h", "hu", "hv"):
+        assert np.array_equal(ones.field(k), none.field(k))
+
+
 @pytest.mark.parametrize("key,gen,rate,tEnd", [("urban", lambda: rc.cloudburst(80), 150 * rc.MMHR, 900),
                                               ("valleyDam", lambda: rc.dambreak(48), 0, 120)])
 def test_app_scenarios(golden, key, gen, rate, tEnd):

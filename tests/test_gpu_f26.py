@@ -121,7 +121,7 @@ def test_f26_lts(fsa):
 
 
 @pytest.mark.parametrize("partner", [dict(implicitFriction=True, stepFlux=True), dict(hazard=True),
-                                     dict(secondOrder=False, hazard=True),
+                                     dict(secondOrder=False, hazard=True), dict(limiter="vanleer", hazard=True),
                                      dict(crest=[dict(i=30, j=j, axis=0, z=97.2, C=1.6) for j in range(3)])])
 def test_instance_kinds_mixed(fsa, partner):
     """A group of instances of different kinds (step flux, hazard products, first order) runs one launch

@@ -696,6 +696,25 @@ int upload_strip_rain(fs_pool *P, Slot &S) {
   return FS_OK;
 }
 
+// LTS: open cells and rain multiplier (the host copy of the rain field; 1 without one) summed per 8x8 tile,
+// for the per-level books of each cycle
+int upload_tile_tables(fs_pool *P, Slot &S) {
+  const size_t nt = (size_t)S.tx * S.ty;
+  std::vector<int32_t> to(nt, 0);
+  std::vector<float> tr(nt, 0.f);
+  for (int j = 0; j < S.ny; ++j)
+    for (int i = 0; i < S.nx; ++i) {
+      const size_t k = (size_t)j * S.nx + i;
+      if (S.hmask[k] & MASK_OPEN) {
+        to[(size_t)(j / 8) * S.tx + i / 8] += 1;
+        tr[(size_t)(j / 8) * S.tx + i / 8] += S.hrf.empty() ? 1.f : S.hrf[k];
+      }
+    }
+  HIPCHK(P, hipMemcpy(S.tileOpen, to.data(), nt * 4, hipMemcpyHostToDevice));
+  HIPCHK(P, hipMemcpy(S.tileRf, tr.data(), nt * 4, hipMemcpyHostToDevice));
+  return FS_OK;
+}
+
 // hazard products before the first step: nothing recorded, no arrival, no first wetting
 __global__ __launch_bounds__(256) void sw_hz_init(float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
@@ -808,18 +827,14 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     for (size_t k = 0; k < N; ++k) tmp[k] = (float)infil[k];
     HIPCHK(P, hipMemcpy(S.infil, tmp.data(), N * 4, hipMemcpyHostToDevice));
   }
+  if (rainField) {
+    S.hrf.resize(N);
+    for (size_t k = 0; k < N; ++k) S.hrf[k] = (float)rainField[k];
+  }
   if (S.L) {
     const size_t nt = (size_t)S.tx * S.ty;
-    std::vector<int32_t> to(nt, 0);
-    std::vector<float> tr(nt, 0.f);
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i)
-        if (S.hmask[(size_t)j * nx + i] & MASK_OPEN) {
-          to[(size_t)(j / 8) * S.tx + i / 8] += 1;
-          tr[(size_t)(j / 8) * S.tx + i / 8] += rainField ? (float)rainField[(size_t)j * nx + i] : 1.f;
-        }
-    HIPCHK(P, hipMemcpy(S.tileOpen, to.data(), nt * 4, hipMemcpyHostToDevice));
-    HIPCHK(P, hipMemcpy(S.tileRf, tr.data(), nt * 4, hipMemcpyHostToDevice));
+    rc = upload_tile_tables(P, S);
+    if (rc) return rc;
     HIPCHK(P, hipMemset(S.lv, 0, nt));
     HIPCHK(P, hipMemset(S.speedSlots, 0, kSpeedSlots * 4));
     HIPCHK(P, hipMemset(S.sh, 0, (size_t)((char *)S.bsx - (char *)S.sh)));      // store, gathers, flags
@@ -828,8 +843,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   }
   double rainArea = 0.0;
   if (rainField) {
-    S.hrf.resize(N);
-    for (size_t k = 0; k < N; ++k) { S.hrf[k] = (float)rainField[k]; if (S.hmask[k] & MASK_OPEN) rainArea += S.hrf[k]; }
+    for (size_t k = 0; k < N; ++k) if (S.hmask[k] & MASK_OPEN) rainArea += S.hrf[k];
     HIPCHK(P, hipMemcpy(S.rf, S.hrf.data(), N * 4, hipMemcpyHostToDevice));
   } else rainArea = open;
   if (S.nstrips) {
@@ -1493,6 +1507,10 @@ int fs_set_rain_field(fs_handle s, const double *field) {
   HIPCHK(P, hipMemcpy(S.rf, S.hrf.data(), N * 4, hipMemcpyHostToDevice));
   int rc = upload_strip_rain(P, S);
   if (rc) return rc;
+  if (S.L) {   // the per-tile sums behind the LTS per-level rain books
+    rc = upload_tile_tables(P, S);
+    if (rc) return rc;
+  }
   rc = download_recs(P, {s->slot});
   if (rc) return rc;
   S.hrec.rainArea = area * S.p.dx * S.p.dx;

@@ -228,6 +228,11 @@ class CPUSolver:
         h = np.ascontiguousarray(h.reshape(self.nx * self.ny))
         lib().or_set_depth(self._h, _dp(h))
 
+    def setRainField(self, field):
+        """A new per-cell rain multiplier (the reference CPU's rainField property, solver-cpu.js:71, 470)."""
+        rf = np.ascontiguousarray(np.asarray(field, dtype=np.float64).reshape(self.nx * self.ny))
+        lib().or_set_rain_field(self._h, _dp(rf))
+
     def setVelocity(self, fn):
         X, Y = self._centres()
         u, v = fn(X, Y)

@@ -74,6 +74,9 @@ struct Slot {
   char *smem = nullptr;
   size_t sbytes = 0;
   float4 *soil = nullptr, *gi = nullptr;
+  // rain multiplier plane of a solver created without one and given one later (setRainField)
+  char *rmem = nullptr;
+  size_t rbytes = 0;
   int stageCells = 0;                    // cells flagged MASK_STAGE
   StepRec hrec{};                        // host mirror (refreshed after each launch group)
   double volInitial = 0.0;               // vol.initial (solver-gpu.js:309)
@@ -1012,6 +1015,7 @@ void fs_pool_destroy(fs_pool_t P) {
     if (S.used && S.mem) (void)hipFree(S.mem);
     if (S.used && S.bmem) (void)hipFree(S.bmem);
     if (S.used && S.smem) (void)hipFree(S.smem);
+    if (S.used && S.rmem) (void)hipFree(S.rmem);
   }
   if (P->grp.dinst) (void)hipFree(P->grp.dinst);
   if (P->grp.dwork) (void)hipFree(P->grp.dwork);
@@ -1072,7 +1076,8 @@ int fs_pool_remove(fs_pool_t P, int32_t slot) {
   HIPCHK(P, hipFree(S.mem));
   if (S.bmem) HIPCHK(P, hipFree(S.bmem));
   if (S.smem) HIPCHK(P, hipFree(S.smem));
-  P->used -= S.bytes + S.bbytes + S.sbytes;
+  if (S.rmem) HIPCHK(P, hipFree(S.rmem));
+  P->used -= S.bytes + S.bbytes + S.sbytes + S.rbytes;
   S = Slot();
   P->grp.slots.clear();
   drop_graph(P->grp);
@@ -1500,8 +1505,21 @@ int fs_set_rain_field(fs_handle s, const double *field) {
   if (!s || !field) return FS_E_ARG;
   fs_pool *P = s->pool;
   Slot &S = P->slots[s->slot];
-  if (!S.rf) return fail(P, FS_E_STATE, "setRainField needs a solver created with a rainField");
   const size_t N = (size_t)S.nx * S.ny;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  if (!S.rf) {
+    // created without a field (the reference's multiplier is 1 everywhere until set): a plane of its
+    // own, and the instance's descriptor changes (rain-field flag, the rain books by area)
+    const size_t b = align256(N * 4);
+    if (P->budget && P->used + b > P->budget) return fail(P, FS_E_BUDGET, "rain field plane: HBM budget spent");
+    HIPCHK(P, hipMalloc(&S.rmem, b));
+    S.rbytes = b;
+    P->used += b;
+    S.rf = (float *)S.rmem;
+    S.hrf.assign(N, 1.f);
+    P->grp.slots.clear();
+    drop_graph(P->grp);
+  }
   double area = 0.0;
   for (size_t k = 0; k < N; ++k) { S.hrf[k] = (float)field[k]; if (S.hmask[k] & MASK_OPEN) area += S.hrf[k]; }
   HIPCHK(P, hipMemcpy(S.rf, S.hrf.data(), N * 4, hipMemcpyHostToDevice));
@@ -1514,6 +1532,7 @@ int fs_set_rain_field(fs_handle s, const double *field) {
   rc = download_recs(P, {s->slot});
   if (rc) return rc;
   S.hrec.rainArea = area * S.p.dx * S.p.dx;
+  S.hrec.rainCells = -1;
   rc = upload_rec(P, s->slot);
   if (rc) return rc;
   HIPCHK(P, hipStreamSynchronize(P->stream));
@@ -1560,7 +1579,7 @@ int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk) {
   return FS_OK;
 }
 
-// ---- device-state snapshot / restore (checkpoint.js:128-150 keeps the solver's buffers and its
+// ---- device-state snapshot / restore (checkpoint.js:94-113 keeps the solver's buffers and its
 // host-side clock and books): the slot's whole device allocation, its step record and host books
 namespace {
 struct SnapHeader {
@@ -1572,34 +1591,42 @@ struct SnapHeader {
   StepRec rec;
   double volInitial, exchangedRead, dtSum;
   int64_t dtSteps;
+  // the allocations of their own that follow the main one (barriers, soil/GI planes, a rain field given
+  // after creation), and the options they belong to
+  uint64_t bBytes, sBytes, rBytes;
+  int32_t nbar, stageCells, hasSoil, hasGi, hasRf, hazard;
 };
-constexpr char kSnapMagic[8] = {'F', 'S', 'S', 'N', 'A', 'P', '0', '1'};
+constexpr char kSnapMagic[8] = {'F', 'S', 'S', 'N', 'A', 'P', '0', '2'};
+constexpr uint32_t kSnapVersion = 2;
 
 bool same_model(const SnapHeader &h, const Slot &S) {
   const fs_params &p = S.p;
   return h.nx == p.nx && h.ny == p.ny && h.lts == p.lts && h.secondOrder == p.secondOrder && h.limiter == p.limiter &&
          h.bc[0] == p.bc[0] && h.bc[1] == p.bc[1] && h.bc[2] == p.bc[2] && h.bc[3] == p.bc[3] && h.dx == p.dx &&
          h.g == p.g && h.cfl == p.cfl && h.hdry == p.hdry && h.hfront == p.hfront && h.dtDry == p.dtDry &&
-         h.slotBytes == S.bytes;
+         h.slotBytes == S.bytes && h.bBytes == S.bbytes && h.sBytes == S.sbytes && h.rBytes == S.rbytes &&
+         h.nbar == S.nbar && h.stageCells == S.stageCells && h.hasSoil == (S.soil != nullptr) &&
+         h.hasGi == (S.gi != nullptr) && h.hasRf == (S.rf != nullptr) && h.hazard == (S.hzA != nullptr);
 }
+
+size_t snap_bytes(const Slot &S) { return sizeof(SnapHeader) + S.bytes + S.bbytes + S.sbytes + S.rbytes; }
 }  // namespace
 
 size_t fs_pool_snapshot_bytes(fs_pool_t P, int32_t slot) {
   if (valid_slot(P, slot)) return 0;
-  return sizeof(SnapHeader) + P->slots[slot].bytes;
+  return snap_bytes(P->slots[slot]);
 }
 
 int fs_pool_snapshot(fs_pool_t P, int32_t slot, void *out, size_t bytes) {
   int rc = valid_slot(P, slot);
   if (rc) return rc;
   Slot &S = P->slots[slot];
-  if (!out || bytes < sizeof(SnapHeader) + S.bytes)
-    return fail(P, FS_E_ARG, "snapshot needs %zu bytes", sizeof(SnapHeader) + S.bytes);
+  if (!out || bytes < snap_bytes(S)) return fail(P, FS_E_ARG, "snapshot needs %zu bytes", snap_bytes(S));
   rc = download_recs(P, {slot});
   if (rc) return rc;
   SnapHeader h{};
   std::memcpy(h.magic, kSnapMagic, 8);
-  h.version = 1;
+  h.version = kSnapVersion;
   h.headerBytes = sizeof(SnapHeader);
   h.slotBytes = S.bytes;
   h.nx = S.p.nx; h.ny = S.p.ny; h.lts = S.p.lts; h.rows = S.p.rows; h.secondOrder = S.p.secondOrder;
@@ -1608,8 +1635,18 @@ int fs_pool_snapshot(fs_pool_t P, int32_t slot, void *out, size_t bytes) {
   h.dx = S.p.dx; h.g = S.p.g; h.cfl = S.p.cfl; h.hdry = S.p.hdry; h.hfront = S.p.hfront; h.dtDry = S.p.dtDry;
   h.rec = S.hrec;
   h.volInitial = S.volInitial; h.exchangedRead = S.exchangedRead; h.dtSum = S.dtSum; h.dtSteps = S.dtSteps;
+  h.bBytes = S.bbytes; h.sBytes = S.sbytes; h.rBytes = S.rbytes;
+  h.nbar = S.nbar; h.stageCells = S.stageCells; h.hasSoil = S.soil != nullptr; h.hasGi = S.gi != nullptr;
+  h.hasRf = S.rf != nullptr; h.hazard = S.hzA != nullptr;
   std::memcpy(out, &h, sizeof h);
-  HIPCHK(P, hipMemcpy((char *)out + sizeof h, S.mem, S.bytes, hipMemcpyDeviceToHost));
+  char *q = (char *)out + sizeof h;
+  HIPCHK(P, hipMemcpy(q, S.mem, S.bytes, hipMemcpyDeviceToHost));
+  q += S.bytes;
+  if (S.bbytes) HIPCHK(P, hipMemcpy(q, S.bmem, S.bbytes, hipMemcpyDeviceToHost));
+  q += S.bbytes;
+  if (S.sbytes) HIPCHK(P, hipMemcpy(q, S.smem, S.sbytes, hipMemcpyDeviceToHost));
+  q += S.sbytes;
+  if (S.rbytes) HIPCHK(P, hipMemcpy(q, S.rmem, S.rbytes, hipMemcpyDeviceToHost));
   return FS_OK;
 }
 
@@ -1620,12 +1657,27 @@ int fs_pool_restore(fs_pool_t P, int32_t slot, const void *in, size_t bytes) {
   SnapHeader h;
   if (!in || bytes < sizeof h) return fail(P, FS_E_ARG, "not a snapshot (too short)");
   std::memcpy(&h, in, sizeof h);
-  if (std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != 1 || h.headerBytes != sizeof h)
+  if (std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != kSnapVersion || h.headerBytes != sizeof h)
     return fail(P, FS_E_ARG, "not a snapshot of this library version");
   if (!same_model(h, S)) return fail(P, FS_E_ARG, "snapshot of a different model (grid, options or layout)");
-  if (bytes < sizeof h + S.bytes) return fail(P, FS_E_ARG, "snapshot truncated");
+  if (bytes < snap_bytes(S)) return fail(P, FS_E_ARG, "snapshot truncated");
   HIPCHK(P, hipStreamSynchronize(P->stream));
-  HIPCHK(P, hipMemcpy(S.mem, (const char *)in + sizeof h, S.bytes, hipMemcpyHostToDevice));
+  const char *q = (const char *)in + sizeof h;
+  HIPCHK(P, hipMemcpy(S.mem, q, S.bytes, hipMemcpyHostToDevice));
+  q += S.bytes;
+  if (S.bbytes) HIPCHK(P, hipMemcpy(S.bmem, q, S.bbytes, hipMemcpyHostToDevice));
+  q += S.bbytes;
+  if (S.sbytes) HIPCHK(P, hipMemcpy(S.smem, q, S.sbytes, hipMemcpyHostToDevice));
+  q += S.sbytes;
+  if (S.rbytes) HIPCHK(P, hipMemcpy(S.rmem, q, S.rbytes, hipMemcpyHostToDevice));
+  // the host mirrors of restored planes (mask bits, rain multiplier, crests) follow the device
+  const size_t N = (size_t)S.nx * S.ny;
+  HIPCHK(P, hipMemcpy(S.hmask.data(), S.mask, N, hipMemcpyDeviceToHost));
+  if (S.rf) {
+    S.hrf.resize(N);
+    HIPCHK(P, hipMemcpy(S.hrf.data(), S.rf, N * 4, hipMemcpyDeviceToHost));
+  }
+  if (S.crest && !S.hcrest.empty()) HIPCHK(P, hipMemcpy(S.hcrest.data(), S.crest, S.hcrest.size() * 16, hipMemcpyDeviceToHost));
   // the strip tables in the restored memory belong to the snapshot's strip height
   S.p.rows = h.rows;
   S.rows = strip_rows(&S.p);

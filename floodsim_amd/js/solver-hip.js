@@ -297,7 +297,7 @@ class GPUSolver {
     return this._enqueue(() => addon.readHazard(this._h));
   }
 
-  /** Everything the solver holds on the device, with its clock and books (checkpoint.js:128-150). */
+  /** Everything the solver holds on the device, with its clock and books (checkpoint.js:94-113). */
   snapshot() {
     this._raiseDeferred();
     const meta = { t: this.t, steps: this.steps, lastDt: this.lastDt, dtAvg: this.dtAvg, vol: Object.assign({}, this.vol),

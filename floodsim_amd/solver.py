@@ -397,7 +397,7 @@ class GPUSolver:
         f["cls"] = c
         return f
 
-    # --- device-state snapshot / restore (checkpoint.js:128-150)
+    # --- device-state snapshot / restore (checkpoint.js:94-113)
     def snapshot(self):
         """Everything the solver holds on the device plus its clock and books, as a uint8 array."""
         n = lib().fs_snapshot_bytes(self._h)

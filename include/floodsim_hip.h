@@ -86,7 +86,9 @@ const char *fs_last_error(fs_handle s);                    /* NULL handle: last 
  * vol.initial = sum of the stored FP32 depths over open cells, as setDepth does */
 int fs_set_state(fs_handle s, const double *h, const double *hu, const double *hv, int32_t bookInitial);
 int fs_set_rain(fs_handle s, double rate_ms);              /* rainRate property (solver-gpu.js:48, 603) */
-int fs_set_rain_field(fs_handle s, const double *field);   /* setRainField (solver-gpu.js:470-478) */
+/* setRainField (solver-gpu.js:470-478); a solver created without a field gets a plane of its own (nx*ny*4
+   bytes, within the pool's HBM budget, FS_E_BUDGET beyond it) */
+int fs_set_rain_field(fs_handle s, const double *field);
 int fs_set_inlet_rates(fs_handle s, const double *q, int32_t n);           /* setInletRates (:557) */
 /* the stageCells option (solver-gpu.js:125-133): interior edge cells j*nx+i whose outward ghosts hold
  * the stage; a new list replaces the old one */
@@ -180,8 +182,10 @@ fs_pool_t fs_solver_pool(fs_handle s);
 int fs_copy_probe(size_t bytes, int32_t iters, double *gbps, double *ms);
 
 /* Device-state snapshot / restore: a host copy of everything the solver holds on the device (state,
- * layers, books, LTS stores) plus its clock and FP64 books, restorable into the same solver or one
- * built from the same model (checkpoint.js:128-150 saves the solver's buffers and its JS state).
+ * layers, books, LTS stores, hazard products, barrier crests and books, Green-Ampt and GI planes, a rain
+ * field given after creation) plus its clock and FP64 books, restorable into the same solver or one built
+ * from the same model and options (gpuSnapshot / gpuRestore / solverState, checkpoint.js:94-113, save every
+ * storage buffer of the solver and its JS state; a restore into other buffer sizes is an error there too).
  * The buffer is opaque, fs_snapshot_bytes long. */
 size_t fs_snapshot_bytes(fs_handle s);
 int fs_snapshot(fs_handle s, void *out, size_t bytes);

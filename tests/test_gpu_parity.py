@@ -333,12 +333,13 @@ def test_pool_alone_vs_mixed(fsa):
         assert abs(a["mass"]["rel"]) <= TOL_MASS
 
 
-@pytest.mark.parametrize("lts,switch", [(0, False), (3, False), (0, True), (3, True)])
+@pytest.mark.parametrize("lts,switch", [(0, ""), (3, ""), (0, "switch"), (3, "switch"), (0, "late"), (3, "late")])
 def test_rain_field(fsa, lts, switch):
     """A per-cell rain multiplier (rainField; solver-cpu.js:470, kernels.wgsl.js:28, 561) on the urban
     cloudburst, global and local time stepping (the LTS books sum the multiplier per tile and level):
     against the oracle, and the rain booked equal to rate x T x sum of the multiplier over open cells.
-    switch: a second field replaces the first at 150 s (setRainField, solver-gpu.js:470-478). Under the
+    switch: a second field replaces the first at 150 s (setRainField, solver-gpu.js:470-478); late: a solver
+    created without a field (multiplier 1) is given the second one at 150 s. Under the
     second field the FP64 answer moves by up to 0.8 mm in single cells when its bed moves by the FP32
     rounding the device stores it with: the bar is taken against the oracle run on that FP32 bed, and on
     the FP64 bed the allowance is 1 mm plus the FP64 answer's own change."""
@@ -349,7 +350,7 @@ def test_rain_field(fsa, lts, switch):
     rf[: n // 7] = 0.0
     rf2 = np.where(np.arange(n) % 3 == 0, 0.0, rng.uniform(0.5, 3.0, n))
     opts = dict(nx=sc["nx"], ny=sc["ny"], dx=sc["dx"], z=sc["z"], mask=sc["mask"], manning=sc["manning"],
-                bc=sc["bc"], infil=sc["infil"], rainField=rf, lts=lts)
+                bc=sc["bc"], infil=sc["infil"], rainField=None if switch == "late" else rf, lts=lts)
     g = fsa.GPUSolver(**opts)
     c = oracle.CPUSolver(**opts, hdry=g.hdry)
     c32 = oracle.CPUSolver(**dict(opts, z=device_bed(opts["z"])), hdry=g.hdry)
@@ -368,9 +369,10 @@ def test_rain_field(fsa, lts, switch):
     h, h32 = c.field("h"), c32.field("h")
     d, d32 = float(np.max(np.abs(s["h"] - h))), float(np.max(np.abs(s["h"] - h32)))
     bed = float(np.max(np.abs(h32 - h)))
-    report(f"rain field lts {lts}{' switched at 150 s' if switch else ''}", g, c, s, d)
+    report(f"rain field lts {lts} {switch}", g, c, s, d)
     open_ = np.asarray(sc["mask"]).reshape(-1) != 0
-    expect = rate * (150 * rf[open_].sum() + 150 * (rf2 if switch else rf)[open_].sum()) * sc["dx"] ** 2
+    first = np.ones(n) if switch == "late" else rf
+    expect = rate * (150 * first[open_].sum() + 150 * (rf2 if switch else rf)[open_].sum()) * sc["dx"] ** 2
     print(f"  against the oracle on the device's FP32 bed {d32:.2e} m (bed rounding moves the oracle {bed:.2e} m); "
           f"rain booked {s['mass']['rain']:.6f} m3, oracle {c.vol['rain']:.6f}, closed form {expect:.6f}")
     assert d32 <= TOL_H and d <= TOL_H + bed

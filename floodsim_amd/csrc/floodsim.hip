@@ -1691,6 +1691,22 @@ int fs_pool_restore(fs_pool_t P, int32_t slot, const void *in, size_t bytes) {
   return FS_OK;
 }
 
+int fs_read_infiltration(fs_handle s, float *F) {
+  if (!s || !F) return FS_E_ARG;
+  fs_pool *P = s->pool;
+  Slot &S = P->slots[s->slot];
+  const size_t N = (size_t)S.nx * S.ny;
+  if (!S.soil) {
+    std::fill(F, F + N, 0.f);
+    return FS_OK;
+  }
+  std::vector<float4> a(N);
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  HIPCHK(P, hipMemcpy(a.data(), S.soil, N * 16, hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < N; ++k) F[k] = a[k].w;   // (psi dtheta, Fmax, pervious share, F)
+  return FS_OK;
+}
+
 int fs_read_hazard(fs_handle s, float *dv, float *force, float *duration, float *arrival, float *vmax, float *tWet,
                    uint8_t *arr, uint8_t *defra, uint8_t *defraUrban) {
   if (!s) return FS_E_ARG;

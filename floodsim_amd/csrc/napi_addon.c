@@ -387,6 +387,22 @@ static napi_value js_read_barriers(napi_env env, napi_callback_info info) {
   return rc ? fail(env, s->h, rc) : r;
 }
 
+/* readInfiltration(h) -> Float32Array(nx*ny): fs_read_infiltration */
+static napi_value js_read_infiltration(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  const size_t n = (size_t)s->nx * s->ny;
+  float *F = (float *)calloc(n ? n : 1, sizeof(float));
+  if (!F) { napi_throw_error(env, NULL, "readInfiltration: out of memory"); return NULL; }
+  int rc = fs_read_infiltration(s->h, F);
+  napi_value r = rc ? NULL : make_f32(env, F, n);
+  free(F);
+  return rc ? fail(env, s->h, rc) : r;
+}
+
 /* setGa(h, Float64Array psiD, fmax, perv) -> fs_set_ga; setGi(h, Float64Array rate, cap, drain) -> fs_set_gi */
 static napi_value js_set_layers3(napi_env env, napi_callback_info info, int gi) {
   size_t argc = 4;
@@ -820,6 +836,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setGi", NULL, js_set_gi, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setCrestLevels", NULL, js_set_crest_levels, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readBarriers", NULL, js_read_barriers, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"readInfiltration", NULL, js_read_infiltration, NULL, NULL, NULL, napi_enumerable, NULL},
       {"clock", NULL, js_clock, NULL, NULL, NULL, napi_enumerable, NULL},
       {"tileStats", NULL, js_tile_stats, NULL, NULL, NULL, napi_enumerable, NULL},
       {"levels", NULL, js_levels, NULL, NULL, NULL, napi_enumerable, NULL},

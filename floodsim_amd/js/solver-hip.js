@@ -234,6 +234,12 @@ class GPUSolver {
     });
   }
 
+  /** Depth infiltrated so far per interior cell under Green-Ampt (m per pervious area; 0 without it;
+   *  solver-gpu.js:507-512). */
+  readInfiltration() {
+    return this._enqueue(() => addon.readInfiltration(this._h));
+  }
+
   /** Exchange rates per inlet (m^3/s, + = surface into the network), held until the next call. */
   setInletRates(q) {
     const a = Float64Array.from(q);

@@ -58,6 +58,7 @@ EXPORTS = [
     "fs_pool_kernel_ms", "fs_pool_set_timing", "fs_solver_pool", "fs_version", "fs_copy_probe",
     "fs_pool_dominant_kernel", "fs_lts_levels", "fs_pool_set_rows", "fs_snapshot_bytes", "fs_snapshot",
     "fs_restore", "fs_pool_snapshot_bytes", "fs_pool_snapshot", "fs_pool_restore", "fs_read_hazard",
+    "fs_read_infiltration",
     "fs_set_stage_cells", "fs_set_stage", "fs_pool_set_stage_cells", "fs_pool_set_stage",
     "fs_set_crests", "fs_set_crest_levels", "fs_read_barriers", "fs_pool_set_crests", "fs_pool_set_crest_levels",
     "fs_pool_read_barriers", "fs_set_ga", "fs_set_gi", "fs_pool_set_ga", "fs_pool_set_gi",
@@ -142,6 +143,7 @@ def lib(path=None):
     L.fs_pool_restore.argtypes = [P, I32, P, SZ]
     fp = ctypes.POINTER(ctypes.c_float)
     L.fs_read_hazard.argtypes = [P, fp, fp, fp, fp, fp, fp, u8p, u8p, u8p]
+    L.fs_read_infiltration.argtypes = [P, fp]
     L.fs_solver_pool.restype = P
     L.fs_solver_pool.argtypes = [P]
     _LIB = L
@@ -338,6 +340,13 @@ class GPUSolver:
         """Barrier books per face (solver-gpu.js:490-504): plus, minus, peak, level, load, tPeak, tLevel,
         tLoad, tOver."""
         return _barrier_books(lambda out: self._chk(lib().fs_read_barriers(self._h, out)), self.nBar)
+
+    def readInfiltration(self):
+        """Depth infiltrated so far per interior cell under Green-Ampt (m per pervious area; 0 without it;
+        solver-gpu.js:507-512)."""
+        F = np.zeros(self.nx * self.ny, dtype=np.float32)
+        self._chk(lib().fs_read_infiltration(self._h, F.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return F
 
     def setInletRates(self, q):
         a = np.ascontiguousarray(np.asarray(q, dtype=np.float64))

@@ -125,6 +125,9 @@ int fs_lts_levels(fs_handle s, uint8_t *levels, int32_t *tx, int32_t *ty);
  * output may be NULL. Needs fs_params.hazard = 1. */
 int fs_read_hazard(fs_handle s, float *dv, float *force, float *duration, float *arrival, float *vmax, float *tWet,
                    uint8_t *arr, uint8_t *defra, uint8_t *defraUrban);
+/* readInfiltration (solver-gpu.js:507-512): depth infiltrated so far per interior cell under Green-Ampt
+ * (m per pervious area; 0 without it), nx*ny floats */
+int fs_read_infiltration(fs_handle s, float *F);
 /* tileStats (:625-629): open cells advanced by the last step (global stepping skips quiet dry
  * strips) and the interior cell total */
 int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total);

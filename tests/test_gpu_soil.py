@@ -63,3 +63,8 @@ def test_green_ampt(fsa, golden, extra):
     assert abs(inf - c.vol["infil"]) <= 1e-4 * c.vol["infil"]
     if not extra:
         assert abs(inf - golden["gpu"]["greenAmptScour"]["infilCpu"]) <= 1e-4 * inf
+    # readInfiltration (solver-gpu.js:507-512): the soil's infiltrated depth per cell, against the oracle's
+    F, Fc = g.readInfiltration(), c.field("gaF")
+    dF = float(np.max(np.abs(F - Fc)))
+    print(f"  readInfiltration: max |dF| {dF:.2e} m, max F {Fc.max():.4f} m")
+    assert F.shape == (opts["nx"] * opts["ny"],) and Fc.max() > 0 and dF <= 1e-4 * Fc.max() + 1e-7

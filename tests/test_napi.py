@@ -20,7 +20,7 @@ NODE = shutil.which("node")
 pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
 ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readHazard", "readInlets", "readState",
-               "readBarriers", "restore", "setCrestLevels", "setCrests", "setGa", "setGi", "setInletRates", "setRain",
+               "readBarriers", "readInfiltration", "restore", "setCrestLevels", "setCrests", "setGa", "setGi", "setInletRates", "setRain",
                "setRainField", "setStage",
                "setStageCells", "setState", "snapshot", "tileStats", "version"]
 

@@ -41,7 +41,7 @@ const addon = require(path.join(__dirname, 'floodsim_napi.node'));
 
 const GH = 2; // the reference's ghost width, kept for idx(i, j) and zh
 const BC = { wall: 0, open: 1, outflow: 2 };
-const UNSUPPORTED = ['subgrid', 'sink', 'sediment', 'debris'];
+const UNSUPPORTED = ['subgrid', 'sink', 'sediment', 'debris', 'scour', 'banks', 'underdrains', 'lines'];
 const BARRIER_BOOKS = ['plus', 'minus', 'peak', 'level', 'load', 'tPeak', 'tLevel', 'tLoad', 'tOver'];
 
 function def(v, d) { return v === undefined || v === null ? d : v; }

@@ -38,14 +38,20 @@ def test_addon_exports():
     assert sorted(json.loads(r.stdout)) == sorted(ADDON_FUNCS)
 
 
+OUT_OF_SCOPE = ["subgrid", "sink", "sediment", "debris", "scour", "banks", "underdrains", "lines"]
+
+
 def test_js_class_loads_and_rejects_out_of_scope_options():
-    r = node("const {GPUSolver, version}=require('./floodsim_amd/js/solver-hip.js');"
-             "let msg='none'; try { new GPUSolver(null,{nx:4,ny:4,dx:1,z:new Float64Array(16),subgrid:{}}); }"
-             " catch (e) { msg=e.message; } console.log(version()); console.log(msg);")
+    r = node("const {GPUSolver, version}=require('./floodsim_amd/js/solver-hip.js'); console.log(version());"
+             f"for (const k of {json.dumps(OUT_OF_SCOPE)}) {{ let msg='none';"
+             " try { new GPUSolver(null, {nx:4, ny:4, dx:1, z:new Float64Array(16), [k]: [{}]}); }"
+             " catch (e) { msg=e.message; } console.log(msg); }")
     assert r.returncode == 0, r.stderr
     lines = r.stdout.strip().splitlines()
     assert lines[0].startswith("floodsim_hip")
-    assert "subgrid" in lines[1] and "not supported" in lines[1]
+    assert len(lines) == 1 + len(OUT_OF_SCOPE)
+    for k, msg in zip(OUT_OF_SCOPE, lines[1:]):
+        assert k in msg and "not supported" in msg, (k, msg)
 
 
 def test_js_sources_are_es2019():

@@ -900,25 +900,45 @@ __device__ __forceinline__ bool tile_relevant(const DevInst &I, int a, int b, in
          (b > 0 && lv[(b - 1) * I.tx + a] == m) || (b + 1 < I.ty && lv[(b + 1) * I.tx + a] == m);
 }
 
-// A record's wavefronts: every unit on its list is split into 2, 4 or 8 parts of whole tile rows,
-// one wavefront each (at most kLtsWaveCap per unit of the domain, the grid of a record launch):
-// the records of the fine levels, whose few units would otherwise each walk their 64 rows alone,
-// then fill the chip, and every part is cut to the tile rows that take part.
+// A record's wavefronts: every unit on its list is split into parts, one wavefront each (at most
+// kLtsWaveCap per unit of the domain, the grid of a record launch, and the partials it has room for):
+// the records of the fine levels, whose few units would otherwise each walk their rows alone, then
+// fill the chip, and every part is cut to the rows that take part. A record of at most 4096 / 16 units
+// splits them into parts of 4, 2 or 1 rows (a wavefront's walk is its rows plus 6 halo rows, so the
+// latency of a small record shrinks with its parts); larger ones into 2, 4 or 8 parts of whole tile rows.
 constexpr int kLtsWaveCap = 2;
+constexpr int kLtsFineWaves = 4096;
 __device__ __forceinline__ int lts_split(const DevInst &I, int m) {
-  // parts of 8, 16 or 32 rows: a list of up to 1/8 of the units, up to 1/4, or more
   const int n = I.rec->lcount[m], nu = I.wend - I.wbeg;
+  const int cap = min(kLtsFineWaves, kLtsWaveCap * nu);
+  for (int s = 64; s > 8; s >>= 1)
+    if (n * s <= cap) return s;
+  // parts of 8, 16 or 32 rows: a list of up to 1/8 of the units, up to 1/4, or more
   return n * 8 <= nu ? 8 : n * 4 <= nu ? 4 : 2;
 }
 
 // The rows one wavefront of a level-m record walks: part `sub` of `split` parts of the unit, cut
 // to the span from its first to its last tile row holding a tile that takes part (w = 0: none);
-// WAITING_ONLY: a tile that waits and gathers (the fold of the record's gathers)
+// WAITING_ONLY: a tile that waits and gathers (the fold of the record's gathers). Parts of at most
+// 4 rows (rp rows, rp = 1, 2 or 4, the smallest with split * rp >= the unit's rows) lie inside one
+// tile row and take part when a tile of that row does.
 template <bool WAITING_ONLY = false>
 __device__ __forceinline__ int4 lts_rows(const DevInst &I, int4 wk, int m, int lane, int split, int sub) {
   const int tx0 = wk.y / kLtsTile, ty0 = wk.z / kLtsTile;
   const int ntx = min(I.own / kLtsTile, I.tx - tx0);
   const int nty = min((wk.w + kLtsTile - 1) / kLtsTile, I.ty - ty0);   // tile rows of the unit
+  const int need = (I.rows + split - 1) / split;
+  if (need <= kLtsTile / 2) {
+    const int rp = need <= 1 ? 1 : need <= 2 ? 2 : 4;
+    const int z = sub * rp, r = z / kLtsTile;
+    const int c = lane & 7;
+    const bool rel = lane < 8 && c < ntx && z < wk.w && wk.z + z < I.ny && r < nty &&
+                     tile_relevant<WAITING_ONLY>(I, tx0 + c, ty0 + r, m);
+    if (!__ballot(rel)) { wk.w = 0; return wk; }
+    wk.w = min(rp, wk.w - z);
+    wk.z += z;
+    return wk;
+  }
   const int per = ((wk.w + kLtsTile - 1) / kLtsTile + split - 1) / split;  // tile rows per part
   const int r0 = sub * per, r1 = min(nty, r0 + per);
   const int c = lane & 7, r = r0 + (lane >> 3);

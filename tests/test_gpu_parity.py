@@ -1,6 +1,6 @@
 """GPU parity: the HIP step (through the C ABI) against the FP64 oracle on the same inputs.
 
-Bar (from the task's statement of the north star): max |dh| <= 1 mm against the FP64 oracle and
+Bar: max |dh| <= 1 mm against the FP64 oracle and
 relative mass-balance error <= 1e-6 on the GPU's own FP64 books. The FP32-vs-FP64 error is printed
 per case. Cases follow the reference's GPU suite (tests/gpu-suite.js:16-26 `both`, :47-59 dam
 breaks, :64-81 lake at rest, :84-98 Thacker, :101-121 rain-on-grid, :125-140 app scenarios,

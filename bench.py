@@ -190,7 +190,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by "
-              f"domain strips or by instance; no multi-GPU run is part of this build)", file=sys.stderr)
+              f"domain strips or by instance; the library has no multi-GPU path)", file=sys.stderr)
         sys.exit(2)
 
     import floodsim_amd as fsa

@@ -39,6 +39,7 @@ struct Slot {
   uint8_t *mask = nullptr;
   Inlet *inl = nullptr;
   float *ispeed = nullptr;
+  float4 *iapp = nullptr;
   int nin = 0;
   std::vector<int> inletCells;           // interior k = j*nx + i
   std::vector<uint8_t> hmask;            // host copy of the mask bits
@@ -198,6 +199,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
   if (infil) b += align256(N * 4);
   if (rf) b += align256(N * 4);
   b += align256((size_t)std::max(nin, 1) * sizeof(Inlet)) + align256((size_t)std::max(nin, 1) * 4);
+  b += align256((size_t)std::max(nin, 1) * 16);
   return b;
 }
 
@@ -296,7 +298,7 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
     for (int y0 = 0; y0 < S.ny; y0 += rows)
       for (int x0 = 0; x0 < S.nx; x0 += own) work.push_back(make_int4((int)gi, x0, y0, rows));
     d.wend = (int)work.size();
-    d.inl = S.inl; d.ispeed = S.ispeed; d.nin = S.nin;
+    d.inl = S.inl; d.ispeed = S.ispeed; d.iapp = S.iapp; d.nin = S.nin;
     d.hl = S.L ? 4 : kHalo;
     d.own = own;
     d.rows = rows;
@@ -779,6 +781,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   S.nin = nInlets;
   S.inl = (Inlet *)take((size_t)std::max(nInlets, 1) * sizeof(Inlet));
   S.ispeed = (float *)take((size_t)std::max(nInlets, 1) * 4);
+  S.iapp = (float4 *)take((size_t)std::max(nInlets, 1) * 16);
   S.rows = strip_rows(p);
   if (p->lts) {
     S.L = p->lts;
@@ -873,6 +876,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   }
   HIPCHK(P, hipMemcpy(S.inl, hi.data(), sizeof(Inlet) * hi.size(), hipMemcpyHostToDevice));
   HIPCHK(P, hipMemset(S.ispeed, 0, (size_t)std::max(nInlets, 1) * 4));
+  HIPCHK(P, hipMemset(S.iapp, 0, (size_t)std::max(nInlets, 1) * 16));
   StepRec &r = S.hrec;
   r = StepRec{};
   r.rainArea = rainArea * p->dx * p->dx;
@@ -1319,6 +1323,17 @@ static int read_inlets(fs_pool *P, int slot, double *vol, double *h, int n) {
   }
   HIPCHK(P, hipMemcpyAsync(S.inl, hi.data(), sizeof(Inlet) * n, hipMemcpyHostToDevice, P->stream));
   HIPCHK(P, hipStreamSynchronize(P->stream));
+  return FS_OK;
+}
+
+int fs_read_inlet_approach(fs_handle s, float *app, int32_t n) {
+  if (!s || (n > 0 && !app)) return FS_E_ARG;
+  fs_pool *P = s->pool;
+  Slot &S = P->slots[s->slot];
+  if (n != S.nin) return fail(P, FS_E_ARG, "expected %d inlets, got %d", S.nin, n);
+  if (!n) return FS_OK;
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  HIPCHK(P, hipMemcpy(app, S.iapp, (size_t)n * 16, hipMemcpyDeviceToHost));
   return FS_OK;
 }
 

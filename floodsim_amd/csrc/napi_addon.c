@@ -494,7 +494,7 @@ typedef struct Work {
   char err[512];
   fs_clock clk;
   fs_mass mass;
-  float *out;            /* frame: 2*nx*ny */
+  float *out;            /* frame: 2*nx*ny; readInlets: app, 4 per inlet */
   double *f[4];          /* readState: h, hu, hv, hmax; readInlets: vol, h */
   float *hz[6];          /* readHazard: dv, force, duration, arrival, vmax, tWet */
   uint8_t *cls[3];       /* readHazard: arr, defra, defraUrban */
@@ -526,6 +526,7 @@ static void work_execute(napi_env env, void *data) {
       break;
     case W_READ_INLETS:
       w->rc = fs_read_inlets(h, w->f[0], w->f[1], w->s->nInlets);
+      if (!w->rc) w->rc = fs_read_inlet_approach(h, w->out, w->s->nInlets);
       break;
     case W_SNAPSHOT:
       w->rc = fs_snapshot(h, w->blob, w->blobBytes);
@@ -604,6 +605,7 @@ static void work_complete(napi_env env, napi_status status, void *data) {
         napi_create_object(env, &result);
         napi_set_named_property(env, result, "vol", make_f64(env, w->f[0], (size_t)w->s->nInlets));
         napi_set_named_property(env, result, "h", make_f64(env, w->f[1], (size_t)w->s->nInlets));
+        napi_set_named_property(env, result, "app", make_f32(env, w->out, 4 * (size_t)w->s->nInlets));
         break;
       }
       case W_SNAPSHOT: {
@@ -667,8 +669,10 @@ static napi_value queue_work(napi_env env, Solver *s, int kind, double t, int32_
   if (kind == W_FRAME && readOut) oom |= !(w->out = (float *)malloc(2 * n * sizeof(float)));
   if (kind == W_READ_STATE)
     for (int k = 0; k < 4; ++k) oom |= !(w->f[k] = (double *)malloc(n * sizeof(double)));
-  if (kind == W_READ_INLETS)
+  if (kind == W_READ_INLETS) {
     for (int k = 0; k < 2; ++k) oom |= !(w->f[k] = (double *)calloc((size_t)s->nInlets + 1, sizeof(double)));
+    oom |= !(w->out = (float *)calloc(4 * (size_t)s->nInlets + 1, sizeof(float)));
+  }
   if (kind == W_HAZARD) {
     for (int k = 0; k < 6; ++k) oom |= !(w->hz[k] = (float *)malloc(n * sizeof(float) + 4));
     for (int k = 0; k < 3; ++k) oom |= !(w->cls[k] = (uint8_t *)malloc(n + 1));

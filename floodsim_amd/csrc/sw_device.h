@@ -109,6 +109,8 @@ struct DevInst {
   int wbeg, wend;               // this instance's range in the work list
   Inlet *inl;                   // this instance's inlets
   float *ispeed;                // wave speed of each inlet cell after the exchange
+  float4 *iapp;                 // water arriving at each inlet cell after its exchange: frontal and side
+                                // discharge (m^3/s), depth of the frontal neighbour (m), its axis (0 x, 1 y)
   int nin;
   int hl, own;                  // strip geometry: halo lanes on the left, owned columns
   int rows, sxn;                // rows per strip, strips across

@@ -1509,6 +1509,21 @@ __global__ void sw_exchange(const DevInst *__restrict__ insts, int inst, int m) 
     }
   }
   I.ispeed[j] = h > I.hdry ? cell_speed(h, hu, hv, I.g) : 0.f;
+  // the water arriving at the inlet cell, for a grate law that sees moving water (approachAt,
+  // kernels.wgsl.js:620-637): each interior neighbour's unit discharge toward the cell; the largest is the
+  // frontal flow, the rest arrives from the side. Reported only: it changes no water.
+  const int x = (int)(k % I.pitch), y = (int)(k / I.pitch);
+  const float fx = x > 0 ? fmaxf(0.f, I.hu[k - 1]) : 0.f;
+  const float fy = x + 1 < I.nx ? fmaxf(0.f, -I.hu[k + 1]) : 0.f;
+  const float fz = y > 0 ? fmaxf(0.f, I.hv[k - I.pitch]) : 0.f;
+  const float fw = y + 1 < I.ny ? fmaxf(0.f, -I.hv[k + I.pitch]) : 0.f;
+  float fm = fx, ax = 0.f;
+  size_t nb = k - 1;
+  if (fy > fm) { fm = fy; nb = k + 1; }
+  if (fz > fm) { fm = fz; nb = k - I.pitch; ax = 1.f; }
+  if (fw > fm) { fm = fw; nb = k + I.pitch; ax = 1.f; }
+  const float hn = fm > 0.f ? I.h[nb] : 0.f;
+  I.iapp[j] = make_float4(fm * I.dx, (fx + fy + fz + fw - fm) * I.dx, hn, ax);
 }
 
 // End of a step, one block per instance (dtFinal, kernels.wgsl.js:783-799): fold the stage

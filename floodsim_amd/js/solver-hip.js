@@ -28,7 +28,7 @@
  *
  * The device argument is accepted for signature compatibility and ignored: the library drives
  * HIP device 0 of the process. Options of the reference outside the shallow-water step
- * (subgrid, sink, sediment, debris) raise. implicitFriction and stepFlux (F26) are taken
+ * (subgrid, sink, sediment, debris, scour, banks, underdrains, lines) raise. implicitFriction and stepFlux (F26) are taken
  * as the reference takes them (stepFlux has no effect under local time stepping).
  *
  * Calls on one solver run one at a time: asynchronous calls queue behind each other, and the
@@ -252,8 +252,9 @@ class GPUSolver {
     return this._enqueue(() => addon.readInlets(this._h)).then((r) => {
       const h = Float32Array.from(r.h);
       for (let m = 0; m < r.vol.length; m++) this.vol.exchanged += r.vol[m];
-      // app (per-inlet approach flows for grate capture) belongs to the drainage model, not this step
-      return { vol: r.vol, h, app: new Float32Array(4 * this.nInlets) };
+      // app: per inlet, the frontal and side discharge arriving at its cell (m^3/s), the frontal neighbour's
+      // depth (m) and its axis (0 x, 1 y), as the last exchange left them (kernels.wgsl.js:620-637)
+      return { vol: r.vol, h, app: r.app };
     });
   }
 

@@ -51,7 +51,7 @@ class fs_mass(ctypes.Structure):
 
 EXPORTS = [
     "fs_create", "fs_destroy", "fs_last_error", "fs_set_state", "fs_set_rain", "fs_set_rain_field",
-    "fs_set_inlet_rates", "fs_read_inlets", "fs_advance", "fs_frame", "fs_read_state", "fs_clock_get",
+    "fs_set_inlet_rates", "fs_read_inlets", "fs_read_inlet_approach", "fs_advance", "fs_frame", "fs_read_state", "fs_clock_get",
     "fs_tile_stats", "fs_pool_create", "fs_pool_destroy", "fs_pool_last_error", "fs_pool_bytes_needed",
     "fs_pool_add", "fs_pool_remove", "fs_pool_set_state", "fs_pool_set_rain", "fs_pool_set_inlet_rates",
     "fs_pool_advance", "fs_pool_step", "fs_pool_read_state", "fs_pool_clock", "fs_pool_bytes_used",
@@ -92,6 +92,7 @@ def lib(path=None):
     L.fs_set_rain_field.argtypes = [P, dp]
     L.fs_set_inlet_rates.argtypes = [P, dp, I32]
     L.fs_read_inlets.argtypes = [P, dp, dp, I32]
+    L.fs_read_inlet_approach.argtypes = [P, ctypes.POINTER(ctypes.c_float), I32]
     L.fs_advance.argtypes = [P, D, I32, ctypes.POINTER(fs_clock)]
     L.fs_frame.argtypes = [P, D, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(fs_clock)]
     L.fs_read_state.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(fs_mass)]
@@ -356,7 +357,10 @@ class GPUSolver:
         vol = np.zeros(self.nInlets)
         h = np.zeros(self.nInlets)
         self._chk(lib().fs_read_inlets(self._h, _dp(vol), _dp(h), self.nInlets))
-        return {"vol": vol, "h": h}
+        app = np.zeros(4 * self.nInlets, dtype=np.float32)
+        self._chk(lib().fs_read_inlet_approach(self._h, app.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                               self.nInlets))
+        return {"vol": vol, "h": h, "app": app}
 
     # --- stepping (advance / frame, solver-gpu.js:589-622)
     def advance(self, tTarget, K=64):

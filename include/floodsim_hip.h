@@ -110,6 +110,10 @@ int fs_set_ga(fs_handle s, const double *psiD, const double *fmax, const double 
 /* the gi option (solver-cpu.js:95): GI stores per cell (intake m/s, capacity m, drain m/s; cells with cap > 0) */
 int fs_set_gi(fs_handle s, const double *rate, const double *cap, const double *drain);
 int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n);        /* readInlets (:564-576) */
+/* readInlets().app (:564-576; approachAt, kernels.wgsl.js:620-637): per inlet, after its last exchange, the
+ * frontal and side discharge arriving at its cell (m^3/s), the depth of the frontal neighbour (m) and its
+ * axis (0 x, 1 y); app holds 4*n floats */
+int fs_read_inlet_approach(fs_handle s, float *app, int32_t n);
 int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk);     /* advance (:589-604) */
 /* frame (:607-622): at most K steps toward tEnd; out (nx*ny*2: depth, speed) may be NULL */
 int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk);

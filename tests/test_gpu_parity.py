@@ -379,3 +379,44 @@ def test_rain_field(fsa, lts, switch):
     assert abs(s["mass"]["rel"]) <= TOL_MASS and s["mass"]["clipped"] == 0
     assert abs(s["mass"]["rain"] - expect) <= 1e-6 * expect
     assert abs(c.vol["rain"] - expect) <= 1e-12 * expect
+
+
+def test_inlet_approach_flows(fsa):
+    """readInlets().app (approachAt, kernels.wgsl.js:620-637): per inlet, the water arriving at its cell after
+    the exchange: each interior neighbour's unit discharge toward the cell, the largest the frontal flow and
+    the rest from the side (m^3/s), the frontal neighbour's depth and its axis. No reference result records
+    it; it is checked against that definition on the GPU's own end-of-step state (global stepping: the
+    exchange is the step's last change of the state), evaluated in FP32 in the kernel's order."""
+    nx, ny, dx = 60, 40, 2.0
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = 0.01 * I + 0.002 * (J - 20) ** 2
+    inlets = [10 * nx + 12, 20 * nx + 30, 30 * nx + 45, 5 * nx + 50, 20 * nx + 0]
+    g = fsa.GPUSolver(nx=nx, ny=ny, dx=dx, z=z, manning=0.02, bc={"west": "outflow"}, inlets=inlets)
+    g.setDepth(lambda X, Y, Z: 0 * X)
+    g.rainRate = 80 * rc.MMHR
+    g.setInletRates([0.02, 0.05, 0.01, -0.01, 0.02])
+    g.advance(300.0)
+    r = g.readInlets()
+    s = g.readState()
+    f32 = np.float32
+    hu = s["hu"].astype(f32).reshape(ny, nx)
+    hv = s["hv"].astype(f32).reshape(ny, nx)
+    h = s["h"].astype(f32).reshape(ny, nx)
+    zero = f32(0.0)
+    app = r["app"].reshape(-1, 4)
+    assert app.dtype == np.float32 and app.shape == (len(inlets), 4)
+    for m, c in enumerate(inlets):
+        i, j = c % nx, c // nx
+        f = [max(zero, hu[j, i - 1]) if i > 0 else zero, max(zero, -hu[j, i + 1]) if i + 1 < nx else zero,
+             max(zero, hv[j - 1, i]) if j > 0 else zero, max(zero, -hv[j + 1, i]) if j + 1 < ny else zero]
+        nbs = [(j, i - 1, 0.0), (j, i + 1, 0.0), (j - 1, i, 1.0), (j + 1, i, 1.0)]
+        fm, q = f[0], 0
+        for t in (1, 2, 3):
+            if f[t] > fm:
+                fm, q = f[t], t
+        side = f32(f32(f32(f32(f[0] + f[1]) + f[2]) + f[3]) - fm)
+        hn = h[nbs[q][0], nbs[q][1]] if fm > 0 else zero
+        want = np.array([f32(fm * f32(dx)), f32(side * f32(dx)), hn, f32(nbs[q][2] if fm > 0 else 0.0)], dtype=f32)
+        print(f"\n  inlet {m} ({i},{j}): app {app[m]}, from the state {want}")
+        np.testing.assert_allclose(app[m], want, rtol=2e-6, atol=1e-9)
+    assert np.any(app[:, 0] > 0)

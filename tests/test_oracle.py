@@ -200,6 +200,14 @@ def test_rain_field(lts):
         o.run(300)
     for k in ("h", "hu", "hv"):
         assert np.array_equal(ones.field(k), none.field(k))
+    # setRainField on a solver created without a field is that field from then on
+    late, _, _ = rain_field_basin(lts, "none")
+    late.setRainField(r)
+    late.run(0.5)
+    late.run(600)
+    for k in ("h", "hu", "hv"):
+        assert np.array_equal(late.field(k), s.field(k))
+    assert late.vol["rain"] == s.vol["rain"]
 
 
 @pytest.mark.parametrize("key,gen,rate,tEnd", [("urban", lambda: rc.cloudburst(80), 150 * rc.MMHR, 900),

@@ -117,6 +117,7 @@ struct fs_pool {
   double *dtarget = nullptr;
   int capTarget = 0;
   bool timing = false, useGraphs = true;
+  bool noSteps = false;                  // test switch (fs_pool_debug): launch groups run no steps
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<hipEvent_t> tev;           // timing mode: 3 events per step (before, between, after the stages)
   int nTev = 0;                          // events recorded since the last fs_pool_kernel_ms
@@ -162,6 +163,11 @@ int fail(fs_pool *P, int code, const char *fmt, ...) {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// every plane is addressed with 32-bit byte offsets: 4-byte planes hold up to 2^30 cells, the 16-byte ones
+// (LTS reconstructions, hazard products, Green-Ampt soil, GI stores, barrier crests) up to 2^28
+constexpr int64_t kMax4BCells = (int64_t)1 << 30;
+constexpr int64_t kMax16BCells = (int64_t)1 << 28;
+
 // strip geometry: global stepping uses 58 owned columns (3 halo lanes each side); local time
 // stepping uses 56 (4 halo lanes on the left) and rows in multiples of 8 so that a strip holds
 // whole 8x8 level tiles
@@ -206,9 +212,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
 int check_params(fs_pool *P, const fs_params *p) {
   if (!p) return fail(P, FS_E_ARG, "params is NULL");
   if (p->nx < 2 || p->ny < 2) return fail(P, FS_E_ARG, "nx and ny must be >= 2 (got %d x %d)", p->nx, p->ny);
-  // every plane is addressed with 32-bit byte offsets: 4-byte planes up to 2^30 cells, the 16-byte
-  // planes (LTS reconstructions, hazard products) up to 2^28
-  if ((int64_t)p->nx * p->ny > ((int64_t)1 << (p->lts || p->hazard ? 28 : 30)))
+  if ((int64_t)p->nx * p->ny > (p->lts || p->hazard ? kMax16BCells : kMax4BCells))
     return fail(P, FS_E_ARG, "grid too large: %d x %d (at most 2^30 cells, 2^28 with local time stepping or "
                 "hazard products)", p->nx, p->ny);
   if (!(p->dx > 0)) return fail(P, FS_E_ARG, "dx must be > 0");
@@ -657,7 +661,7 @@ int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n
       }
       if (known) kb = (int)std::max<double>(1.0, std::min<double>(kb, std::ceil(need * 1.02) + 1.0));
     }
-    rc = launch_steps(P, kb);
+    rc = P->noSteps ? FS_OK : launch_steps(P, kb);
     if (rc) return rc;
     if (once) left -= kb;
     rc = download_recs(P, gs);
@@ -844,7 +848,8 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     HIPCHK(P, hipMemset(S.lv, 0, nt));
     HIPCHK(P, hipMemset(S.speedSlots, 0, kSpeedSlots * 4));
     HIPCHK(P, hipMemset(S.sh, 0, (size_t)((char *)S.bsx - (char *)S.sh)));      // store, gathers, flags
-    HIPCHK(P, hipMemset(S.bsx, 0, N * 32));
+    HIPCHK(P, hipMemset(S.bsx, 0, N * 16));   // each plane on its own: take() pads them apart
+    HIPCHK(P, hipMemset(S.bsy, 0, N * 16));
     HIPCHK(P, hipDeviceSynchronize());
   }
   double rainArea = 0.0;
@@ -872,6 +877,8 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     const int k = inlets[m], i = k % nx, j = k / nx;
     hi[m].cell = j * S.pitch + i;
     hi[m].q = 0.f;
+    hi[m].h = 0.f;
+    hi[m].pad = 0.f;
     hi[m].vol = 0.0;
   }
   HIPCHK(P, hipMemcpy(S.inl, hi.data(), sizeof(Inlet) * hi.size(), hipMemcpyHostToDevice));
@@ -1145,6 +1152,9 @@ int fs_pool_set_crests(fs_pool_t P, int32_t slot, const int32_t *fi, const int32
   if (rc) return rc;
   Slot &S = P->slots[slot];
   if (n < 0 || (n && (!fi || !fj || !axis || !z || !C || !ground))) return fail(P, FS_E_ARG, "barriers: bad lists");
+  if (n && (int64_t)S.nx * S.ny > kMax16BCells)
+    return fail(P, FS_E_ARG, "grid too large for flood barriers: %d x %d (the 16-byte crest plane holds at most 2^28 "
+                "cells)", S.nx, S.ny);
   for (int m = 0; m < n; ++m) {
     const bool ax = axis[m] != 0;
     if (fi[m] < (ax ? 0 : 1) || fi[m] >= S.nx || fj[m] < (ax ? 1 : 0) || fj[m] >= S.ny)
@@ -1208,6 +1218,9 @@ int fs_pool_set_crests(fs_pool_t P, int32_t slot, const int32_t *fi, const int32
 // both planes live in one allocation of their own, the infiltrated depth F and the stored depth start at 0
 int soil_planes(fs_pool *P, Slot &S) {
   if (S.smem) return FS_OK;
+  if ((int64_t)S.nx * S.ny > kMax16BCells)
+    return fail(P, FS_E_ARG, "grid too large for Green-Ampt soil or GI stores: %d x %d (their 16-byte planes hold at "
+                "most 2^28 cells)", S.nx, S.ny);
   const size_t N = (size_t)S.nx * S.ny, b = 2 * align256(N * 16);
   if (P->budget && P->used + b > P->budget) return fail(P, FS_E_BUDGET, "soil/GI planes: HBM budget spent");
   HIPCHK(P, hipMalloc(&S.smem, b));
@@ -1311,15 +1324,14 @@ static int read_inlets(fs_pool *P, int slot, double *vol, double *h, int n) {
   std::vector<Inlet> hi(n);
   HIPCHK(P, hipMemcpyAsync(hi.data(), S.inl, sizeof(Inlet) * n, hipMemcpyDeviceToHost, P->stream));
   HIPCHK(P, hipStreamSynchronize(P->stream));
+  // one copy of the inlet records: the volumes since the last read and the depths the last exchange left;
+  // both restart at zero, as the reference clears its INX record after the copy (solver-gpu.js:566-569)
   for (int m = 0; m < n; ++m) {
     if (vol) vol[m] = hi[m].vol;
+    if (h) h[m] = hi[m].h;
     S.exchangedRead += hi[m].vol;
     hi[m].vol = 0.0;
-    if (h) {
-      float v = 0.f;
-      HIPCHK(P, hipMemcpy(&v, S.h + hi[m].cell, 4, hipMemcpyDeviceToHost));
-      h[m] = v;
-    }
+    hi[m].h = 0.f;
   }
   HIPCHK(P, hipMemcpyAsync(S.inl, hi.data(), sizeof(Inlet) * n, hipMemcpyHostToDevice, P->stream));
   HIPCHK(P, hipStreamSynchronize(P->stream));
@@ -1429,6 +1441,14 @@ int fs_pool_dominant_kernel(fs_pool_t P, double *ms_total, double *cells_total, 
   if (launches) *launches = P->domLaunches;
   P->domMs = P->domCells = 0.0;
   P->domLaunches = 0;
+  return FS_OK;
+}
+
+int fs_pool_debug(fs_pool_t P, int32_t what, int32_t value) {
+  if (!P) return FS_E_ARG;
+  if (what != FS_DEBUG_NO_STEPS) return fail(P, FS_E_ARG, "fs_pool_debug: unknown switch %d", what);
+  HIPCHK(P, hipStreamSynchronize(P->stream));
+  P->noSteps = value != 0;
   return FS_OK;
 }
 
@@ -1572,7 +1592,8 @@ int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk) {
   if (!s) return FS_E_ARG;
   fs_pool *P = s->pool;
   int32_t sl = s->slot;
-  int rc = advance_group(P, &sl, &tEnd, 1, K, true);
+  // K = 0: the readout alone (the reference encodes no steps then, solver-gpu.js:611)
+  int rc = K > 0 ? advance_group(P, &sl, &tEnd, 1, K, true) : FS_OK;
   if (rc) return rc;
   Slot &S = P->slots[sl];
   if (out) {
@@ -1771,14 +1792,25 @@ int fs_lts_levels(fs_handle s, uint8_t *levels, int32_t *tx, int32_t *ty) {
   return FS_OK;
 }
 
-int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total) {
-  if (!s) return FS_E_ARG;
+int fs_tile_stats(fs_handle s, fs_tiles *out) {
+  if (!s || !out) return FS_E_ARG;
   Slot &S = s->pool->slots[s->slot];
   int rc = download_recs(s->pool, {s->slot});
   if (rc) return rc;
-  // global stepping: open cells of the strips the last step advanced; LTS: all open cells
-  if (active) *active = S.nstrips ? S.hrec.active : S.openCells;
-  if (total) *total = (int64_t)S.nx * S.ny;
+  *out = fs_tiles{};
+  out->cellsTotal = (int64_t)S.nx * S.ny;
+  if (S.L) {   // local time stepping: every 8x8 level tile steps each cycle
+    out->tiled = 1;
+    out->tiles = out->update = out->flux = (int64_t)S.tx * S.ty;
+    out->cellsActive = S.openCells;
+    return FS_OK;
+  }
+  // the same rule as the descriptor's strip flags (build_group): skipping is off untiled and with GI stores
+  const bool skip = S.nstrips && !S.p.untiled && !S.gi;
+  out->tiled = skip ? 1 : 0;
+  out->tiles = S.nstrips;
+  out->update = out->flux = skip ? S.hrec.activeStrips : S.nstrips;
+  out->cellsActive = skip ? S.hrec.active : S.openCells;
   return FS_OK;
 }
 

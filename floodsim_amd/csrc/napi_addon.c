@@ -439,14 +439,32 @@ static napi_value js_tile_stats(napi_env env, napi_callback_info info) {
   NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
   Solver *s = get_solver(env, a[0]);
   if (!s) return NULL;
-  int64_t act = 0, tot = 0;
-  int rc = fs_tile_stats(s->h, &act, &tot);
+  fs_tiles t;
+  int rc = fs_tile_stats(s->h, &t);
   if (rc) return fail(env, s->h, rc);
-  napi_value o;
+  napi_value o, b;
   napi_create_object(env, &o);
-  set_num(env, o, "active", (double)act);
-  set_num(env, o, "cells", (double)tot);
+  napi_get_boolean(env, t.tiled != 0, &b);
+  napi_set_named_property(env, o, "tiled", b);
+  set_num(env, o, "tiles", (double)t.tiles);
+  set_num(env, o, "update", (double)t.update);
+  set_num(env, o, "flux", (double)t.flux);
+  set_num(env, o, "active", (double)t.cellsActive);
+  set_num(env, o, "cells", (double)t.cellsTotal);
   return o;
+}
+
+/* debugNoSteps(h, on) -> fs_pool_debug(FS_DEBUG_NO_STEPS): the stall path's test switch */
+static napi_value js_debug_no_steps(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  bool on = false;
+  napi_get_value_bool(env, a[1], &on);
+  int rc = fs_pool_debug(fs_solver_pool(s->h), FS_DEBUG_NO_STEPS, on ? 1 : 0);
+  return rc ? fail(env, s->h, rc) : undefined_value(env);
 }
 
 static napi_value js_levels(napi_env env, napi_callback_info info) {
@@ -843,6 +861,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readInfiltration", NULL, js_read_infiltration, NULL, NULL, NULL, napi_enumerable, NULL},
       {"clock", NULL, js_clock, NULL, NULL, NULL, napi_enumerable, NULL},
       {"tileStats", NULL, js_tile_stats, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"debugNoSteps", NULL, js_debug_no_steps, NULL, NULL, NULL, napi_enumerable, NULL},
       {"levels", NULL, js_levels, NULL, NULL, NULL, napi_enumerable, NULL},
       {"advance", NULL, js_advance, NULL, NULL, NULL, napi_enumerable, NULL},
       {"frame", NULL, js_frame, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -82,13 +82,18 @@ struct StepRec {
   int32_t lcount[kMaxLevels + 1]; // work-list length per level
   int32_t records;                // records run so far (diagnostic)
   int32_t active;                 // open cells the last step advanced (dry strips skipped)
+  int32_t activeStrips;           // strips the last step advanced (dry-strip skipping on)
   float stage;                    // water level of the stage cells' ghosts, m relative to the bed minimum
 };
 
-// one inlet: cell index (y*pitch + x), rate (m^3/s, + into the network, - a point inflow)
+// one inlet: cell index (y*pitch + x), rate (m^3/s, + into the network, - a point inflow), the cell's depth
+// after its last exchange, and the volume exchanged since the last read (the reference's INX record,
+// kernels.wgsl.js:597-618, cleared by readInlets, solver-gpu.js:564-576)
 struct Inlet {
   int cell;
   float q;
+  float h;      // depth after the last exchange, m
+  float pad;
   double vol;   // exchanged since the last read, m^3
 };
 

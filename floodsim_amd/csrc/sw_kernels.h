@@ -1508,6 +1508,7 @@ __global__ void sw_exchange(const DevInst *__restrict__ insts, int inst, int m) 
       I.wet[(y / I.rows) * I.sxn + x / I.own] = 1;
     }
   }
+  q.h = h;
   I.ispeed[j] = h > I.hdry ? cell_speed(h, hu, hv, I.g) : 0.f;
   // the water arriving at the inlet cell, for a grate law that sees moving water (approachAt,
   // kernels.wgsl.js:620-637): each interior neighbour's unit discharge toward the cell; the largest is the
@@ -1538,10 +1539,10 @@ __global__ __launch_bounds__(1024) void sw_dt_final(const DevInst *__restrict__ 
   const int t = threadIdx.x;
   __shared__ double s_o[16], s_c[16], s_f[16];
   __shared__ float s_m[16];
-  __shared__ int s_n[16];
+  __shared__ int s_n[16], s_s[16];
   double o = 0.0, c = 0.0, f = 0.0;
   float m = 0.f;
-  int cells = 0;
+  int cells = 0, strips = 0;
   const bool fresh = mode == 0 || rec->dirty;
   if (fresh) {
     for (int k = I.wbeg + t; k < I.wend; k += blockDim.x) {
@@ -1553,6 +1554,7 @@ __global__ __launch_bounds__(1024) void sw_dt_final(const DevInst *__restrict__ 
         c += a.clip + b.clip;
         f += b.infil;
         cells += b.cells;
+        if (I.wet) strips += I.act[k - I.wbeg] ? 1 : 0;
       }
     }
     if (mode == 0)
@@ -1560,11 +1562,14 @@ __global__ __launch_bounds__(1024) void sw_dt_final(const DevInst *__restrict__ 
   }
   o = wave_sum(o); c = wave_sum(c); f = wave_sum(f); m = wave_max(m);
   cells = wave_sum_i(cells);
-  if ((t & 63) == 0) { s_o[t >> 6] = o; s_c[t >> 6] = c; s_f[t >> 6] = f; s_m[t >> 6] = m; s_n[t >> 6] = cells; }
+  strips = wave_sum_i(strips);
+  if ((t & 63) == 0) {
+    s_o[t >> 6] = o; s_c[t >> 6] = c; s_f[t >> 6] = f; s_m[t >> 6] = m; s_n[t >> 6] = cells; s_s[t >> 6] = strips;
+  }
   __syncthreads();
   if (t != 0) return;
   for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
-    o += s_o[k]; c += s_c[k]; f += s_f[k]; m = fmaxf(m, s_m[k]); cells += s_n[k];
+    o += s_o[k]; c += s_c[k]; f += s_f[k]; m = fmaxf(m, s_m[k]); cells += s_n[k]; strips += s_s[k];
   }
   // a no-op step (dt = 0, past tEnd) measured nothing: keep the last real max speed
   if (!fresh || (mode == 0 && !(rec->dt > 0.f))) m = rec->smax;
@@ -1581,6 +1586,7 @@ __global__ __launch_bounds__(1024) void sw_dt_final(const DevInst *__restrict__ 
     rec->lastDt = dtp;
     rec->cellUpdates += (double)cells;
     rec->active = cells;
+    rec->activeStrips = strips;
   }
   rec->smax = m;
   const double rem = rec->tEnd - rec->t;

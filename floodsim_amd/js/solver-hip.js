@@ -99,19 +99,32 @@ class GPUSolver {
     const manS = typeof o.manning === 'number' ? o.manning : 0;
     const infil = o.infil === undefined || o.infil === null ? null : f64(o.infil, n, 'infil');
     const inlets = this.nInlets ? Int32Array.from(o.inlets) : null;
-    this._h = addon.create(params, z, manArr, manS, this._mask, infil, rf, inlets);
+    const ga = o.ga ? [f64(o.ga.psiD, n, 'ga.psiD'), f64(o.ga.fmax, n, 'ga.fmax'), f64(o.ga.perv, n, 'ga.perv')] : null;
+    const gi = o.gi ? [f64(o.gi.rate, n, 'gi.rate'), f64(o.gi.cap, n, 'gi.cap'), f64(o.gi.drain, n, 'gi.drain')] : null;
     this.stage = -1e9;
-    if (o.stageCells && o.stageCells.length) addon.setStageCells(this._h, Int32Array.from(o.stageCells));
-    // Green-Ampt soil and GI stores per cell (solver-cpu.js:94-96)
-    if (o.ga) addon.setGa(this._h, f64(o.ga.psiD, n, 'ga.psiD'), f64(o.ga.fmax, n, 'ga.fmax'), f64(o.ga.perv, n, 'ga.perv'));
-    if (o.gi) addon.setGi(this._h, f64(o.gi.rate, n, 'gi.rate'), f64(o.gi.cap, n, 'gi.cap'), f64(o.gi.drain, n, 'gi.drain'));
     // flood barriers: faces {i, j, axis, z, C, ground} (solver-gpu.js:18-20, 144-151)
     this.crest = (o.crest || []).slice();
     this.nBar = this.crest.length;
-    if (this.nBar) {
-      const col = (k, d, T) => T.from(this.crest.map((f) => (f[k] === undefined || f[k] === null ? d : f[k])));
-      addon.setCrests(this._h, col('i', 0, Int32Array), col('j', 0, Int32Array), col('axis', 0, Int32Array),
-        col('z', -Infinity, Float64Array), col('C', 1.6, Float64Array), col('ground', -Infinity, Float64Array));
+    // The device objects: an error here (the library, the device, its memory, a layer the grid cannot hold)
+    // does not throw from the constructor; `ready` rejects with it, as the reference's does when its pipelines
+    // fail (solver-gpu.js:287), and every later call on the solver fails with the same error.
+    this._h = null;
+    this._createError = null;
+    try {
+      this._h = addon.create(params, z, manArr, manS, this._mask, infil, rf, inlets);
+      if (o.stageCells && o.stageCells.length) addon.setStageCells(this._h, Int32Array.from(o.stageCells));
+      // Green-Ampt soil and GI stores per cell (solver-cpu.js:94-96)
+      if (ga) addon.setGa(this._h, ga[0], ga[1], ga[2]);
+      if (gi) addon.setGi(this._h, gi[0], gi[1], gi[2]);
+      if (this.nBar) {
+        const col = (k, d, T) => T.from(this.crest.map((f) => (f[k] === undefined || f[k] === null ? d : f[k])));
+        addon.setCrests(this._h, col('i', 0, Int32Array), col('j', 0, Int32Array), col('axis', 0, Int32Array),
+          col('z', -Infinity, Float64Array), col('C', 1.6, Float64Array), col('ground', -Infinity, Float64Array));
+      }
+    } catch (e) {
+      if (this._h) addon.destroy(this._h);
+      this._h = null;
+      this._createError = new Error('GPU pipeline error: ' + (e && e.message ? e.message : e));
     }
     this.t = 0; this.steps = 0; this.lastDt = 0; this.dtAvg = 0;
     this.vol = { rain: 0, initial: 0, exchanged: 0 };
@@ -123,7 +136,12 @@ class GPUSolver {
     this._queue = Promise.resolve();
     this._pending = 0;
     this._zh = null;
-    this.ready = Promise.resolve();
+    if (this._createError) {
+      this.ready = Promise.reject(this._createError);
+      this.ready.catch(() => {});   // handled by whoever awaits ready; no unhandled-rejection warning
+    } else {
+      this.ready = Promise.resolve();
+    }
   }
 
   /** Padded index of interior cell (i, j), as the reference's idx (GH = 2 ghost layers). */
@@ -146,6 +164,7 @@ class GPUSolver {
 
   // ---- serialisation of calls on the one handle
   _enqueue(fn) {
+    if (this._createError) return Promise.reject(this._createError);
     this._pending++;
     const p = this._queue.then(fn);
     const done = () => { this._pending--; };
@@ -154,6 +173,7 @@ class GPUSolver {
   }
 
   _sync(fn) {
+    if (this._createError) throw this._createError;
     if (this._pending === 0) { fn(); return; }
     this._enqueue(fn).catch((e) => { this._deferredError = e; });
   }
@@ -289,11 +309,14 @@ class GPUSolver {
     });
   }
 
-  /** Open cells advanced by the last step against the interior total. */
+  /** Active tiles of the last step (solver-gpu.js:625-629): {tiled, tiles, update, flux} as tile counts
+   *  ({tiled: false, tiles} when every tile steps), where this build's tile is its wavefront strip (LTS: the
+   *  8x8 level tile); also the open cells the last step advanced. */
   tileStats() {
     return this._enqueue(() => {
       const s = addon.tileStats(this._h);
-      return { tiled: this.tiled, tiles: Math.ceil(this.nx / 8) * Math.ceil(this.ny / 8), update: s.active, cells: s.cells };
+      if (!s.tiled) return { tiled: false, tiles: s.tiles, active: s.active };
+      return { tiled: true, tiles: s.tiles, update: s.update, flux: s.flux, active: s.active };
     });
   }
 
@@ -321,6 +344,10 @@ class GPUSolver {
       this.vol = Object.assign({}, m.vol); this._rain = m.rainRate;
     });
   }
+
+  /** Test switch (no counterpart in the reference): with on = true every advance runs no steps, so advance
+   *  rejects with the no-progress error after its progress checks; false restores stepping. */
+  _debugNoSteps(on) { this._sync(() => addon.debugNoSteps(this._h, !!on)); }
 
   /** LTS levels of the last cycle per 8x8 tile: {levels: Uint8Array, tx, ty}. */
   levels() { return this._enqueue(() => addon.levels(this._h)); }

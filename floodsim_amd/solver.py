@@ -43,6 +43,11 @@ class fs_clock(ctypes.Structure):
                 ("steps", ctypes.c_int64), ("cellUpdates", ctypes.c_int64)]
 
 
+class fs_tiles(ctypes.Structure):
+    _fields_ = [("tiled", ctypes.c_int32), ("pad", ctypes.c_int32)] + \
+        [(k, ctypes.c_int64) for k in ("tiles", "update", "flux", "cellsActive", "cellsTotal")]
+
+
 class fs_mass(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in
                 ("volume", "rain", "infil", "outflow", "exchanged", "clipped", "expected", "rel", "relExClip", "initial",
@@ -61,8 +66,9 @@ EXPORTS = [
     "fs_read_infiltration",
     "fs_set_stage_cells", "fs_set_stage", "fs_pool_set_stage_cells", "fs_pool_set_stage",
     "fs_set_crests", "fs_set_crest_levels", "fs_read_barriers", "fs_pool_set_crests", "fs_pool_set_crest_levels",
-    "fs_pool_read_barriers", "fs_set_ga", "fs_set_gi", "fs_pool_set_ga", "fs_pool_set_gi",
+    "fs_pool_read_barriers", "fs_set_ga", "fs_set_gi", "fs_pool_set_ga", "fs_pool_set_gi", "fs_pool_debug",
 ]
+FS_DEBUG_NO_STEPS = 1
 
 
 def lib(path=None):
@@ -97,7 +103,7 @@ def lib(path=None):
     L.fs_frame.argtypes = [P, D, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(fs_clock)]
     L.fs_read_state.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(fs_mass)]
     L.fs_clock_get.argtypes = [P, ctypes.POINTER(fs_clock)]
-    L.fs_tile_stats.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.fs_tile_stats.argtypes = [P, ctypes.POINTER(fs_tiles)]
     L.fs_pool_create.argtypes = [SZ, ctypes.POINTER(P)]
     L.fs_pool_destroy.argtypes = [P]
     L.fs_pool_destroy.restype = None
@@ -131,6 +137,7 @@ def lib(path=None):
     L.fs_pool_clock.argtypes = [P, I32, ctypes.POINTER(fs_clock)]
     L.fs_pool_kernel_ms.argtypes = [P, dp, dp, ctypes.POINTER(ctypes.c_int64)]
     L.fs_pool_set_timing.argtypes = [P, I32]
+    L.fs_pool_debug.argtypes = [P, I32, I32]
     L.fs_copy_probe.argtypes = [SZ, I32, dp, dp]
     L.fs_lts_levels.argtypes = [P, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.fs_pool_dominant_kernel.argtypes = [P, dp, dp, ctypes.POINTER(ctypes.c_int64)]
@@ -432,9 +439,19 @@ class GPUSolver:
         return out.reshape(ty.value, tx.value)
 
     def tileStats(self):
-        a, t = ctypes.c_int64(), ctypes.c_int64()
-        self._chk(lib().fs_tile_stats(self._h, ctypes.byref(a), ctypes.byref(t)))
-        return {"active": a.value, "cells": t.value}
+        """Active tiles of the last step (tileStats, solver-gpu.js:625-629): {tiled, tiles, update, flux} as
+        tile counts (this build's skipping unit is the wavefront strip), plus the open cells the last step
+        advanced (active) and the interior cell total (cells)."""
+        t = fs_tiles()
+        self._chk(lib().fs_tile_stats(self._h, ctypes.byref(t)))
+        out = {"tiled": bool(t.tiled), "tiles": t.tiles, "active": t.cellsActive, "cells": t.cellsTotal}
+        if t.tiled:
+            out.update(update=t.update, flux=t.flux)
+        return out
+
+    def _debug_no_steps(self, on):
+        """Test switch: every advance runs no steps (the stall path), until switched off."""
+        self._chk(lib().fs_pool_debug(lib().fs_solver_pool(self._h), FS_DEBUG_NO_STEPS, 1 if on else 0))
 
     # --- timing of the stage kernels (HIP events on the solver's stream)
     def set_timing(self, on):

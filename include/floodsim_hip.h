@@ -105,17 +105,21 @@ int fs_set_crest_levels(fs_handle s, const double *z);     /* setCrestLevels (so
  * -inf = never wet), load (kN/m), tPeak, tLevel, tLoad, tOver (s, NaN = never overtopped) */
 int fs_read_barriers(fs_handle s, double *out);
 /* the ga option (solver-cpu.js:96): Green-Ampt soil per cell (psi dtheta m, Fmax m, pervious share; cells
- * with perv > 0); the infiltration capacity K of the infil layer becomes K (1 + psi dtheta / F) */
+ * with perv > 0); the infiltration capacity K of the infil layer becomes K (1 + psi dtheta / F). The soil and
+ * GI planes (16 bytes per cell each) and the barrier crests need a grid of at most 2^28 cells (FS_E_ARG beyond) */
 int fs_set_ga(fs_handle s, const double *psiD, const double *fmax, const double *perv);
 /* the gi option (solver-cpu.js:95): GI stores per cell (intake m/s, capacity m, drain m/s; cells with cap > 0) */
 int fs_set_gi(fs_handle s, const double *rate, const double *cap, const double *drain);
-int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n);        /* readInlets (:564-576) */
+/* readInlets (:564-576): per inlet the volume exchanged since the last read (m^3) and the cell's depth
+ * after its last exchange (m); both restart at zero, as the reference clears its INX record */
+int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n);
 /* readInlets().app (:564-576; approachAt, kernels.wgsl.js:620-637): per inlet, after its last exchange, the
  * frontal and side discharge arriving at its cell (m^3/s), the depth of the frontal neighbour (m) and its
  * axis (0 x, 1 y); app holds 4*n floats */
 int fs_read_inlet_approach(fs_handle s, float *app, int32_t n);
 int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk);     /* advance (:589-604) */
-/* frame (:607-622): at most K steps toward tEnd; out (nx*ny*2: depth, speed) may be NULL */
+/* frame (:607-622): at most K steps toward tEnd (K = 0: none), then the readout (pack, kernels.wgsl.js:
+ * 1346-1355): out (nx*ny*2 floats: depth, speed) may be NULL */
 int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk);
 /* readState (:651-690): interior FP64 copies (any may be NULL) and the FP64 books */
 int fs_read_state(fs_handle s, double *h, double *hu, double *hv, double *hmax, fs_mass *mass);
@@ -132,9 +136,17 @@ int fs_read_hazard(fs_handle s, float *dv, float *force, float *duration, float 
 /* readInfiltration (solver-gpu.js:507-512): depth infiltrated so far per interior cell under Green-Ampt
  * (m per pervious area; 0 without it), nx*ny floats */
 int fs_read_infiltration(fs_handle s, float *F);
-/* tileStats (:625-629): open cells advanced by the last step (global stepping skips quiet dry
- * strips) and the interior cell total */
-int fs_tile_stats(fs_handle s, int64_t *active, int64_t *total);
+/* tileStats (:625-629): the active-tile counts of the last step. Global stepping skips quiet dry strips
+ * (58 columns x rows cells, the unit of this build's active tiles): tiles = strips, update = flux = strips the
+ * last step advanced (a strip evaluates its faces in the pass that updates it, so the reference's two lists,
+ * update and flux, are one set here). tiled = 0 when every strip steps every step (tiles: false, GI stores).
+ * Local time stepping: tiles = the 8x8 level tiles, every one of which steps each cycle (update = flux =
+ * tiles). cellsActive: open cells the last step advanced; cellsTotal: nx*ny. */
+typedef struct fs_tiles {
+  int32_t tiled, pad;
+  int64_t tiles, update, flux, cellsActive, cellsTotal;
+} fs_tiles;
+int fs_tile_stats(fs_handle s, fs_tiles *out);
 
 /* ---- batched / served instances (no counterpart in the reference, which runs one
  * solve at a time: src/batch.js:1-4). Many uneven-size instances share one launch
@@ -180,6 +192,12 @@ int fs_pool_set_timing(fs_pool_t pool, int32_t on);
  * under LTS one launch per record), the open cells those launches advanced, and the launch count,
  * since the last call (no counterpart in the reference: the roofline measurement) */
 int fs_pool_dominant_kernel(fs_pool_t pool, double *ms_total, double *cells_total, int64_t *launches);
+/* Test switch (no counterpart in the reference): FS_DEBUG_NO_STEPS = 1 makes every launch group of the pool
+ * run no steps, so the clock never moves and an advance ends with FS_E_STALL and the reference's
+ * no-progress message (solver-gpu.js:601) after its progress checks, with no kernel left running; 0 restores
+ * normal stepping */
+#define FS_DEBUG_NO_STEPS 1
+int fs_pool_debug(fs_pool_t pool, int32_t what, int32_t value);
 /* the underlying pool of a single solver (for timing and tuning calls) */
 fs_pool_t fs_solver_pool(fs_handle s);
 

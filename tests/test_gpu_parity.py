@@ -261,7 +261,7 @@ def _inlet_case(o, rate_scale=1.0):
     o.setInletRates([0.05, 0.05, -0.002, 0.5])
 
 
-def test_inlets_books(fsa):
+def test_inlets_books(fsa, golden):
     """gpu-suite.js:142-160: sinks bounded by the water a cell holds, a source, books close.
 
     This case is ill-conditioned at the free-outfall edge (the 1e-4 surface-slope threshold switches
@@ -300,6 +300,11 @@ def test_inlets_books(fsa):
     assert s["h"].min() >= 0
     assert d <= max(TOL_H, 2 * sens)
     assert abs(tot_g - tot_c) <= 1e-3 * abs(tot_c) + 1e-6
+    # the reference GPU's recorded book for this case (results-gpu.json inlets.exchanged, gpu-suite.js:142-144)
+    ref = golden["gpu"]["inlets"]["exchanged"]
+    print(f"  mass.exchanged gpu {s['mass']['exchanged']:.6f} m3, reference GPU {ref:.6f} m3")
+    assert abs(s["mass"]["exchanged"] - ref) <= 1e-4 * ref
+    assert abs(s["mass"]["exchanged"] - tot_g) <= 1e-9 * ref
 
 
 def test_pool_alone_vs_mixed(fsa):

@@ -19,7 +19,7 @@ NODE = shutil.which("node")
 
 pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 
-ADDON_FUNCS = ["advance", "clock", "create", "destroy", "frame", "levels", "readHazard", "readInlets", "readState",
+ADDON_FUNCS = ["advance", "clock", "create", "debugNoSteps", "destroy", "frame", "levels", "readHazard", "readInlets", "readState",
                "readBarriers", "readInfiltration", "restore", "setCrestLevels", "setCrests", "setGa", "setGi", "setInletRates", "setRain",
                "setRainField", "setStage",
                "setStageCells", "setState", "snapshot", "tileStats", "version"]
@@ -164,3 +164,19 @@ def test_js_stage_and_f26_case(tmp_path):
     print(f"JS stage case: {js['steps']} steps, max|dh| vs oracle {d:.2e} m, inflow {-js['mass']['outflow']:.2f} "
           f"m3 (oracle {-o.vol['outflow']:.2f}), mass rel {js['mass']['rel']:.1e}")
     assert d <= 1e-3 and abs(js["mass"]["rel"]) <= 1e-6 and -js["mass"]["outflow"] > 0
+
+
+def test_ready_rejects_on_creation_error():
+    """A creation error (here an invalid 1-column grid, or no device) does not throw from the constructor:
+    `ready` rejects with 'GPU pipeline error: ...' (solver-gpu.js:287) and later calls fail with it."""
+    r = node("const {GPUSolver}=require('./floodsim_amd/js/solver-hip.js');"
+             "let s; try { s = new GPUSolver(null, {nx:1, ny:4, dx:1, z:new Float64Array(4)}); }"
+             " catch (e) { console.log('threw ' + e.message); process.exit(0); }"
+             "s.ready.then(() => console.log('resolved'), (e) => {"
+             "  console.log(e.message);"
+             "  return s.readState().then(() => console.log('readState resolved'), (e2) => console.log(e2.message)); });")
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert len(lines) == 2, lines
+    assert lines[0].startswith("GPU pipeline error: ") and lines[1] == lines[0]
+    assert "UnhandledPromiseRejection" not in r.stderr

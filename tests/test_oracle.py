@@ -598,3 +598,31 @@ def test_inlet_exchange_rule():
     assert abs(s1.massBalance()["rel"]) < 1e-12
     r = s1.readInlets()
     assert r["vol"][1] < 0 and r["vol"][0] > 0
+
+
+def test_inlets_exchanged_matches_reference(golden):
+    """gpu-suite.js:127-144: four drainage inlets (two sinks on the low edge, a source, a sink asking far more
+    than its cell holds) under 60 mm/h for 900 s, rates changed at 480 s. The reference GPU booked
+    mass.exchanged = 32.98430824877687 m3 (results-gpu.json inlets). The FP64 oracle lands within 1e-4 of it
+    (FP32 state against FP64); a dropped cap (dd > h), a wrong sign of the source or a missed rate change
+    moves it by metres cubed."""
+    nx, ny, dx = 60, 40, 2
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = 0.01 * I + 0.002 * (J - 20) ** 2
+    inlets = [5 * nx + 5, 20 * nx + 2, 30 * nx + 40, 12 * nx + 50]
+    c = oracle.CPUSolver(nx=nx, ny=ny, dx=dx, z=z, manning=0.02, bc={"west": "outflow"}, inlets=inlets, hdry=1e-5)
+    c.setDepth(lambda X, Y, Z: 0 * X)
+    c.rainRate = 60 * rc.MMHR
+    c.setInletRates([0.05, 0.05, -0.002, 0.5])
+    tot = 0.0
+    for t in range(60, 901, 60):
+        c.run(t)
+        tot += c.readInlets()["vol"].sum()
+        if t == 480:
+            c.setInletRates([0.05, 0.05, 0, 0.5])
+    ref = golden["gpu"]["inlets"]["exchanged"]
+    assert abs(ref - 32.98430824877687) < 1e-12
+    print(f"\ninlets exchanged: oracle {tot:.6f} m3, reference GPU {ref:.6f} m3")
+    assert abs(tot - ref) <= 1e-4 * ref
+    assert abs(c.vol["exchanged"] - tot) <= 1e-12 * ref
+    assert abs(c.massBalance()["rel"]) < 1e-12

@@ -72,10 +72,20 @@ def build_workload(name, seed=2026):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(w, state, lts=0, window=1024, steps=None, budget_s=20.0):
-    """The FP64 oracle (single thread, same stepping) on a window of the same spun-up state."""
+def cpu_threads():
+    """Host threads for the all-cores baseline: every core the process may use, at most 16 (one GPU's share of
+    the host on the GPU machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(w, state, lts=0, window=1024, steps=None, budget_s=20.0, threads=1):
+    """The FP64 oracle (same stepping) on a window of the same spun-up state, on `threads` OpenMP threads."""
     import oracle
-    oracle.set_threads(1)
+    oracle.set_threads(threads)
     n = w["nx"]
     m = min(window, n)
     j0 = i0 = (n - m) // 2
@@ -99,9 +109,10 @@ def cpu_baseline(w, state, lts=0, window=1024, steps=None, budget_s=20.0):
     el = time.perf_counter() - t0
     upd = (c.cellSteps - u0) if lts else open_cells * k
     kind = "LTS cycles (%d levels)" % lts if lts else "global SSP-RK2 steps"
-    return dict(value=upd / el, unit="cell-updates/s", cores=1, kind="port",
-                sample=f"FP64 oracle (C restatement of solver-cpu.js), 1 thread, {m}x{m} centre window of the "
-                       f"spun-up state, {k} {kind} in {el:.1f} s")
+    oracle.set_threads(1)
+    return dict(value=upd / el, unit="cell-updates/s", cores=threads, kind="port", steps=k,
+                sample=f"FP64 oracle (C restatement of solver-cpu.js), {threads} thread{'s' if threads > 1 else ''}, "
+                       f"{m}x{m} centre window of the spun-up state, {k} {kind} in {el:.1f} s")
 
 
 def bench_batched(count=256, t_end=300.0, seed=7):
@@ -268,7 +279,13 @@ def main():
     traffic, tr = traffic_of(args.workload, stepping)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(w, s, lts=lts)
+        # one core (the reference's solver-cpu.js is single-threaded), then the same sample on every core
+        one = cpu_baseline(w, s, lts=lts)
+        nt = cpu_threads()
+        allc = cpu_baseline(w, s, lts=lts, steps=one["steps"], threads=nt) if nt > 1 else one
+        cpu = dict(allc, host_cpus=os.cpu_count(), single_core=one,
+                   speedup_all_cores=allc["value"] / one["value"])
+        log(f"cpu baseline: {one['value']:.3e} cell-updates/s on 1 core, {allc['value']:.3e} on {nt}")
     g.destroy()
     batched = None if args.no_batch else bench_batched()
     served = None if args.no_serve else bench_served(args.serve_rate, args.serve_count)

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -100,15 +101,45 @@ struct Group {
   bool anyStepFlux = false;              // an instance takes the step-flux rules (F26)
   unsigned kinds = 0;                    // second-stage kernel variants the group needs: bit 2*hazard + stepFlux
   bool lts = false;                      // a local-time-stepping instance (alone in its group)
-  // captured K-step graph
   // captured K-step graphs, one per power-of-two K (index log2 K)
   hipGraphExec_t graphs[16] = {};
 };
+
+// What one step's (or cycle's) launch sequence needs: the device descriptors, work list and per-wavefront
+// partials, and the host copies that pick the kernel variants. A Group gives one over its own arrays; the
+// allocation-free advance builds one over the static scratch array and stack copies.
+struct Launch {
+  DevInst *dinst = nullptr;
+  int4 *dwork = nullptr;
+  Part *p1 = nullptr, *p2 = nullptr;
+  int nwork = 0, ni = 0;
+  const DevInst *hinst = nullptr;        // ni host copies
+  const int *withInlets = nullptr;       // group indices of instances with inlets
+  int nWithInlets = 0;
+  const int *withBarriers = nullptr;     // group indices of instances with flood barriers
+  int nWithBarriers = 0;
+  unsigned kinds = 0;
+  bool anyFirstOrder = false, anySecondOrder = false, lts = false;
+  int slot0 = 0;                         // pool slot of the first instance (LTS timing histogram)
+};
+
+Launch view_of(const Group &G) {
+  Launch L;
+  L.dinst = G.dinst; L.dwork = G.dwork; L.p1 = G.p1; L.p2 = G.p2;
+  L.nwork = G.nwork; L.ni = (int)G.slots.size();
+  L.hinst = G.hinst.data();
+  L.withInlets = G.withInlets.data(); L.nWithInlets = (int)G.withInlets.size();
+  L.withBarriers = G.withBarriers.data(); L.nWithBarriers = (int)G.withBarriers.size();
+  L.kinds = G.kinds; L.anyFirstOrder = G.anyFirstOrder; L.anySecondOrder = G.anySecondOrder; L.lts = G.lts;
+  L.slot0 = G.slots.empty() ? 0 : G.slots[0];
+  return L;
+}
 
 }  // namespace
 
 struct fs_pool {
   size_t budget = 0, used = 0;
+  int device = 0;                        // HIP device current at creation (every slot lives there)
   std::vector<Slot> slots;
   StepRec *drec = nullptr;               // device records, one per slot
   int capSlots = 0;
@@ -261,6 +292,53 @@ int download_recs(fs_pool *P, const std::vector<int> &slots) {
   return FS_OK;
 }
 
+// the device descriptor of one slot whose strips are work items [wbeg, wend) of its group's list
+DevInst devinst_of(const Slot &S, StepRec *rec, int wbeg, int wend) {
+  DevInst d{};
+  d.h = S.h; d.hu = S.hu; d.hv = S.hv; d.w = S.w;
+  d.h1 = S.h1; d.hu1 = S.hu1; d.hv1 = S.hv1; d.w1 = S.w1;
+  d.z = S.z; d.n = S.n; d.mask = S.mask; d.hmax = S.hmax; d.infil = S.infil; d.rf = S.rf;
+  d.rec = rec;
+  d.nx = S.nx; d.ny = S.ny; d.pitch = S.pitch;
+  d.flags = (S.infil ? FLAG_INFIL : 0) | (S.rf ? FLAG_RF : 0) | (S.p.implicitFriction ? FLAG_IMPF : 0) |
+            (S.p.stepFlux && !S.L ? FLAG_STEPF : 0) |   // the step flux applies under global stepping only
+            (S.stageCells ? FLAG_STAGE : 0);
+  d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;
+  d.hfront = (float)S.p.hfront; d.cfl = (float)S.p.cfl; d.dtDry = (float)S.p.dtDry;
+  d.area = (float)(S.p.dx * S.p.dx);
+  d.secondOrder = S.p.secondOrder ? 1 : 0;
+  d.limiter = S.p.limiter;
+  for (int k = 0; k < 4; ++k) d.bc[k] = S.p.bc[k];
+  d.wbeg = wbeg;
+  d.wend = wend;
+  d.inl = S.inl; d.ispeed = S.ispeed; d.iapp = S.iapp; d.nin = S.nin;
+  d.hl = S.L ? 4 : kHalo;
+  d.own = strip_own(&S.p);
+  d.rows = std::max(1, S.rows);
+  d.sxn = S.sxn;
+  // GI stores drain every step, dry or not: an instance with them steps every strip
+  d.wet = S.nstrips && !S.p.untiled && !S.gi ? S.swet : nullptr;
+  d.act = S.sact; d.srain = S.srain; d.sopen = S.sopen;
+  d.hzA = S.hzA; d.hzB = S.hzB;
+  d.L = S.L; d.tx = S.tx; d.ty = S.ty;
+  d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.speedSlots = S.speedSlots; d.tileOpen = S.tileOpen;
+  d.tileRf = S.rf ? S.tileRf : nullptr;
+  d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
+  d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.g1h = S.g1h; d.g1hu = S.g1hu; d.g1hv = S.g1hv;
+  d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
+  d.umask = S.umask; d.ucells = S.ucells; d.ulist = S.ulist;
+  d.crest = S.crest; d.bidx = S.bidx; d.barFace = S.barFace; d.barGround = S.barGround;
+  d.barFlux = S.barFlux; d.barBook = S.barBook; d.nbar = S.nbar;
+  d.soil = S.soil; d.gi = S.gi;
+  return d;
+}
+
+// extended faces and edges: the step flux, barrier crests, stage cells, soil, GI, the van Leer limiter or
+// implicit friction (one kernel variant; the rule of ext_faces in the kernels)
+bool ext_kind(const Slot &S, const DevInst &d) {
+  return (d.flags & (FLAG_IMPF | FLAG_STEPF | FLAG_STAGE)) || S.nbar || S.soil || S.gi || S.p.limiter;
+}
+
 // (re)build the device descriptors and work list of a group of slots
 int build_group(fs_pool *P, const std::vector<int> &slots) {
   Group &G = P->grp;
@@ -278,48 +356,14 @@ int build_group(fs_pool *P, const std::vector<int> &slots) {
   std::vector<int4> work;
   for (size_t gi = 0; gi < slots.size(); ++gi) {
     Slot &S = P->slots[slots[gi]];
-    DevInst d{};
-    d.h = S.h; d.hu = S.hu; d.hv = S.hv; d.w = S.w;
-    d.h1 = S.h1; d.hu1 = S.hu1; d.hv1 = S.hv1; d.w1 = S.w1;
-    d.z = S.z; d.n = S.n; d.mask = S.mask; d.hmax = S.hmax; d.infil = S.infil; d.rf = S.rf;
-    d.rec = P->drec + slots[gi];
-    d.nx = S.nx; d.ny = S.ny; d.pitch = S.pitch;
-    d.flags = (S.infil ? FLAG_INFIL : 0) | (S.rf ? FLAG_RF : 0) | (S.p.implicitFriction ? FLAG_IMPF : 0) |
-              (S.p.stepFlux && !S.L ? FLAG_STEPF : 0) |   // the step flux applies under global stepping only
-              (S.stageCells ? FLAG_STAGE : 0);
-    // extended faces and edges: the step flux, barrier crests or stage cells (one kernel variant)
-    const bool ext = (d.flags & (FLAG_IMPF | FLAG_STEPF | FLAG_STAGE)) || S.nbar || S.soil || S.gi || S.p.limiter;
-    if (ext) G.anyStepFlux = true;
-    G.kinds |= 1u << ((S.hzA ? 2 : 0) + (ext ? 1 : 0));
-    d.dx = (float)S.p.dx; d.inv_dx = (float)(1.0 / S.p.dx); d.g = (float)S.p.g; d.hdry = (float)S.p.hdry;
-    d.hfront = (float)S.p.hfront; d.cfl = (float)S.p.cfl; d.dtDry = (float)S.p.dtDry;
-    d.area = (float)(S.p.dx * S.p.dx);
-    d.secondOrder = S.p.secondOrder ? 1 : 0;
-    d.limiter = S.p.limiter;
-    for (int k = 0; k < 4; ++k) d.bc[k] = S.p.bc[k];
-    d.wbeg = (int)work.size();
+    const int wbeg = (int)work.size();
     const int rows = std::max(1, S.rows), own = strip_own(&S.p);
     for (int y0 = 0; y0 < S.ny; y0 += rows)
       for (int x0 = 0; x0 < S.nx; x0 += own) work.push_back(make_int4((int)gi, x0, y0, rows));
-    d.wend = (int)work.size();
-    d.inl = S.inl; d.ispeed = S.ispeed; d.iapp = S.iapp; d.nin = S.nin;
-    d.hl = S.L ? 4 : kHalo;
-    d.own = own;
-    d.rows = rows;
-    d.sxn = S.sxn;
-    // GI stores drain every step, dry or not: an instance with them steps every strip
-    d.wet = S.nstrips && !S.p.untiled && !S.gi ? S.swet : nullptr;
-    d.act = S.sact; d.srain = S.srain; d.sopen = S.sopen;
-    d.hzA = S.hzA; d.hzB = S.hzB;
-    d.L = S.L; d.tx = S.tx; d.ty = S.ty;
-    d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.speedSlots = S.speedSlots; d.tileOpen = S.tileOpen; d.tileRf = S.rf ? S.tileRf : nullptr;
-    d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
-    d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.g1h = S.g1h; d.g1hu = S.g1hu; d.g1hv = S.g1hv;
-    d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
-    d.umask = S.umask; d.ucells = S.ucells; d.ulist = S.ulist;
-    d.crest = S.crest; d.bidx = S.bidx; d.barFace = S.barFace; d.barGround = S.barGround;
-    d.barFlux = S.barFlux; d.barBook = S.barBook; d.nbar = S.nbar;
-    d.soil = S.soil; d.gi = S.gi;
+    const DevInst d = devinst_of(S, P->drec + slots[gi], wbeg, (int)work.size());
+    const bool ext = ext_kind(S, d);
+    if (ext) G.anyStepFlux = true;
+    G.kinds |= 1u << ((S.hzA ? 2 : 0) + (ext ? 1 : 0));
     if (S.nbar) G.withBarriers.push_back((int)gi);
     if (S.L) G.lts = true;
     if (S.hzA) G.anyHazard = true;
@@ -357,6 +401,24 @@ __global__ void sw_set_targets(const DevInst *__restrict__ insts, int n, const d
   if (i < n) insts[i].rec->tEnd = t[i];
 }
 
+// ---- the allocation-free advance (fs_advance_noalloc): descriptor, work list and per-wavefront partials of
+// one solver in a file-scope device array of fixed size, written by kernels from launch arguments (no host
+// buffers, no allocation). The array is the process's for its lifetime, one copy per device; its device
+// address comes from hipGetSymbolAddress on the first call on each device.
+constexpr size_t kNoAllocScratch = (size_t)4 << 20;
+__device__ __align__(256) char scratch[kNoAllocScratch];
+
+__global__ void sw_put_inst(DevInst *__restrict__ dst, DevInst d, double tEnd) {
+  *dst = d;
+  d.rec->tEnd = tEnd;
+}
+
+// the work list of one instance, in build_group's order: strips across (x fastest), then down
+__global__ __launch_bounds__(256) void sw_fill_work(int4 *__restrict__ work, int n, int sxn, int own, int rows) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < n) work[w] = make_int4(0, (w % sxn) * own, (w / sxn) * rows, rows);
+}
+
 hipEvent_t span_event(fs_pool *P) {
   if (P->nSpanEv >= (int)P->spanEv.size()) {
     hipEvent_t e = nullptr;
@@ -375,9 +437,8 @@ hipEvent_t timing_event(fs_pool *P) {
   return P->tev[P->nTev++];
 }
 
-int launch_step(fs_pool *P) {
-  Group &G = P->grp;
-  const int nb = (G.nwork + 3) / 4, ni = (int)G.slots.size();
+int launch_step(fs_pool *P, const Launch &G) {
+  const int nb = (G.nwork + 3) / 4, ni = G.ni;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   if (P->timing) {
     e0 = timing_event(P); e1 = timing_event(P); e2 = timing_event(P);
@@ -388,8 +449,10 @@ int launch_step(fs_pool *P) {
   const unsigned K = G.kinds;
   if (K & 0x5u) sw_stage<0, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
   if (K & 0xau) sw_stage<0, false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
-  for (int gi : G.withBarriers)
+  for (int q = 0; q < G.nWithBarriers; ++q) {
+    const int gi = G.withBarriers[q];
     sw_barrier_books<0, false><<<(G.hinst[gi].nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
+  }
   if (P->timing) HIPCHK(P, hipEventRecord(e1, P->stream));
   // each instance runs its own scheme: the kernel of the other order returns at once for it
   if (G.anySecondOrder) {
@@ -397,8 +460,10 @@ int launch_step(fs_pool *P) {
     if (K & 0x2u) sw_stage<1, false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
     if (K & 0x4u) sw_stage<1, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
     if (K & 0x8u) sw_stage<1, false, true, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    for (int gi : G.withBarriers)
+    for (int q = 0; q < G.nWithBarriers; ++q) {
+      const int gi = G.withBarriers[q];
       sw_barrier_books<1, false><<<(G.hinst[gi].nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
+    }
   }
   if (P->timing) P->spans.push_back({e1, nullptr, -1, -1});  // completed below with e2
   if (G.anyFirstOrder) {
@@ -408,7 +473,8 @@ int launch_step(fs_pool *P) {
     if (K & 0x8u) sw_euler_finish<false, true, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
   }
   if (P->timing) { HIPCHK(P, hipEventRecord(e2, P->stream)); P->spans.back().b = e2; }
-  for (int gi : G.withInlets) {
+  for (int q = 0; q < G.nWithInlets; ++q) {
+    const int gi = G.withInlets[q];
     const int nin = G.hinst[gi].nin;
     sw_exchange<false><<<(nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
   }
@@ -455,8 +521,7 @@ int collect_timing(fs_pool *P) {
 
 // one local-time-stepping cycle of the group's (single) instance: levels and lists, then the
 // records finest first (solver-cpu.js:504-527; solver-gpu.js:367-396)
-int launch_cycle(fs_pool *P) {
-  Group &G = P->grp;
+int launch_cycle(fs_pool *P, const Launch &G) {
   const DevInst &d = G.hinst[0];
   const int nt = d.tx * d.ty, nu = d.wend - d.wbeg, L = d.L;
   // kLtsWaveCap wavefronts per unit at most (records of few units split them); idle blocks exit
@@ -488,7 +553,7 @@ int launch_cycle(fs_pool *P) {
       P->dhist = h;
       P->capHist = cap;
     }
-    HIPCHK(P, hipMemcpyAsync(P->dhist + (size_t)(kMaxLevels + 1) * cyc, &P->drec[G.slots[0]].lcells,
+    HIPCHK(P, hipMemcpyAsync(P->dhist + (size_t)(kMaxLevels + 1) * cyc, &P->drec[G.slot0].lcells,
                              sizeof(int32_t) * (kMaxLevels + 1), hipMemcpyDeviceToDevice, P->stream));
     P->nCycles += 1;
   }
@@ -547,7 +612,8 @@ int launch_cycle(fs_pool *P) {
 
 int launch_steps(fs_pool *P, int K) {
   Group &G = P->grp;
-  auto one = [&]() { return G.lts ? launch_cycle(P) : launch_step(P); };
+  const Launch L = view_of(G);
+  auto one = [&]() { return L.lts ? launch_cycle(P, L) : launch_step(P, L); };
   if (P->timing || !P->useGraphs || K < 2) {
     for (int k = 0; k < K; ++k) {
       int rc = one();
@@ -684,6 +750,100 @@ int advance_group(fs_pool *P, const int32_t *slots, const double *targets, int n
     if (done || (once && left <= 0)) break;
     if (!progress && ++stalls > 3) return fail(P, FS_E_STALL, "GPU solver made no progress (device error?)");
   }
+  return FS_OK;
+}
+
+// the device address of the scratch array on the current device (hipGetSymbolAddress once per device)
+int scratch_base(fs_pool *P, char **out) {
+  static std::mutex mu;
+  static char *base[64] = {};
+  int dev = -1;
+  HIPCHK(P, hipGetDevice(&dev));
+  if (dev != P->device)
+    return fail(P, FS_E_STATE, "fs_advance_noalloc: device %d is current, the solver lives on device %d", dev, P->device);
+  if (dev < 0 || dev >= 64) return fail(P, FS_E_STATE, "fs_advance_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!base[dev]) {
+    char *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(P, FS_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    base[dev] = ptr;
+  }
+  *out = base[dev];
+  return FS_OK;
+}
+
+// fs_advance without allocation: the launch sequence of fs_advance (the same kernels in the same order, plain
+// launches instead of captured graphs) over the scratch array; one slot
+int advance_noalloc(fs_pool *P, int slot, double target, int K, fs_clock *clk) {
+  int rc = valid_slot(P, slot);
+  if (rc) return rc;
+  if (P->timing) return fail(P, FS_E_STATE, "fs_advance_noalloc: timing mode creates HIP events; switch it off");
+  Slot &S = P->slots[slot];
+  const int own = strip_own(&S.p), rows = std::max(1, S.rows);
+  const int sxn = (S.nx + own - 1) / own, nwork = sxn * ((S.ny + rows - 1) / rows);
+  const int needWork = S.L ? kLtsWaveCap * nwork : nwork;
+  const size_t bInst = align256(sizeof(DevInst)), bWork = align256(sizeof(int4) * (size_t)nwork);
+  const size_t bPart = align256(sizeof(Part) * (size_t)needWork), need = bInst + bWork + 2 * bPart;
+  if (need > kNoAllocScratch)
+    return fail(P, FS_E_ARG, "fs_advance_noalloc: this grid needs %zu bytes of scratch (%d work items of %d x %d "
+                "cells), more than the %zu bytes the library holds", need, nwork, own, rows, kNoAllocScratch);
+  char *base = nullptr;
+  if ((rc = scratch_base(P, &base))) return rc;
+  if (K < 1) K = 1;
+  const DevInst d = devinst_of(S, P->drec + slot, 0, nwork);
+  const int zero = 0;
+  Launch L;
+  L.dinst = (DevInst *)base;
+  L.dwork = (int4 *)(base + bInst);
+  L.p1 = (Part *)(base + bInst + bWork);
+  L.p2 = (Part *)(base + bInst + bWork + bPart);
+  L.nwork = nwork; L.ni = 1;
+  L.hinst = &d;
+  L.withInlets = &zero; L.nWithInlets = S.nin ? 1 : 0;
+  L.withBarriers = &zero; L.nWithBarriers = S.nbar ? 1 : 0;
+  L.kinds = 1u << ((S.hzA ? 2 : 0) + (ext_kind(S, d) ? 1 : 0));
+  L.anyFirstOrder = !S.p.secondOrder; L.anySecondOrder = S.p.secondOrder != 0; L.lts = S.L > 0;
+  L.slot0 = slot;
+  // descriptor and target, work list; then (global stepping) the first dt, as begin_advance
+  sw_put_inst<<<1, 1, 0, P->stream>>>(L.dinst, d, target);
+  sw_fill_work<<<(nwork + 255) / 256, 256, 0, P->stream>>>(L.dwork, nwork, sxn, own, rows);
+  if (!L.lts) {
+    if (S.hrec.dirty) sw_speed<<<(nwork + 3) / 4, 256, 0, P->stream>>>(L.dinst, L.dwork, nwork, L.p2);
+    sw_dt_final<<<1, 1024, 0, P->stream>>>(L.dinst, L.p1, L.p2, 1);
+  }
+  HIPCHK(P, hipGetLastError());
+  S.hrec.dirty = 0;
+  // batches as advance_group: sized to the steps still needed at the recent mean step
+  constexpr int kMaxBatch = 256;
+  double tprev = S.hrec.t, recent = 0.0;
+  int stalls = 0;
+  for (;;) {
+    int kb = K < kMaxBatch ? K : kMaxBatch;
+    const double mean = recent > 0.0 ? recent : (S.dtSteps > 0 ? S.dtSum / (double)S.dtSteps : 0.0);
+    if (mean > 0.0) kb = (int)std::max<double>(1.0, std::min<double>(kb, std::ceil((target - S.hrec.t) / mean * 1.02) + 1.0));
+    for (int k = 0; k < kb && !P->noSteps; ++k)
+      if ((rc = L.lts ? launch_cycle(P, L) : launch_step(P, L))) return rc;
+    StepRec r;
+    HIPCHK(P, hipMemcpyAsync(&r, P->drec + slot, sizeof r, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(P, hipStreamSynchronize(P->stream));
+    S.hrec = r;
+    bool progress = false;
+    if (S.hrec.t > tprev) {
+      progress = true;
+      const int64_t ds = S.hrec.steps - S.dtSteps;
+      if (ds > 0) {
+        S.dtSum += S.hrec.t - tprev;
+        S.dtSteps = S.hrec.steps;
+        recent = (S.hrec.t - tprev) / (double)ds;
+      }
+    }
+    tprev = S.hrec.t;
+    if (S.hrec.t >= target) break;
+    if (!progress && ++stalls > 3) return fail(P, FS_E_STALL, "GPU solver made no progress (device error?)");
+  }
+  if (clk) fill_clock(S, clk);
   return FS_OK;
 }
 
@@ -1004,6 +1164,7 @@ int fs_pool_create(size_t hbmBudgetBytes, fs_pool_t *out) {
     return fail(nullptr, FS_E_HIP, "no HIP device: %s", e != hipSuccess ? hipGetErrorString(e) : "count 0");
   fs_pool *P = new fs_pool();
   P->budget = hbmBudgetBytes;
+  if (hipGetDevice(&P->device) != hipSuccess) P->device = 0;
   // FS_NO_GRAPHS=1: plain launches instead of captured K-step graphs (profilers that trace
   // kernels do not follow graph launches here)
   const char *ng = getenv("FS_NO_GRAPHS");
@@ -1586,6 +1747,11 @@ int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk) {
   if (!s) return FS_E_ARG;
   int32_t sl = s->slot;
   return fs_pool_advance(s->pool, &sl, &tTarget, 1, K, clk);
+}
+
+int fs_advance_noalloc(fs_handle s, double tTarget, int32_t K, fs_clock *clk) {
+  if (!s) return FS_E_ARG;
+  return advance_noalloc(s->pool, s->slot, tTarget, K, clk);
 }
 
 int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk) {

@@ -67,6 +67,7 @@ EXPORTS = [
     "fs_set_stage_cells", "fs_set_stage", "fs_pool_set_stage_cells", "fs_pool_set_stage",
     "fs_set_crests", "fs_set_crest_levels", "fs_read_barriers", "fs_pool_set_crests", "fs_pool_set_crest_levels",
     "fs_pool_read_barriers", "fs_set_ga", "fs_set_gi", "fs_pool_set_ga", "fs_pool_set_gi", "fs_pool_debug",
+    "fs_advance_noalloc",
 ]
 FS_DEBUG_NO_STEPS = 1
 
@@ -100,6 +101,7 @@ def lib(path=None):
     L.fs_read_inlets.argtypes = [P, dp, dp, I32]
     L.fs_read_inlet_approach.argtypes = [P, ctypes.POINTER(ctypes.c_float), I32]
     L.fs_advance.argtypes = [P, D, I32, ctypes.POINTER(fs_clock)]
+    L.fs_advance_noalloc.argtypes = [P, D, I32, ctypes.POINTER(fs_clock)]
     L.fs_frame.argtypes = [P, D, I32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(fs_clock)]
     L.fs_read_state.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(fs_mass)]
     L.fs_clock_get.argtypes = [P, ctypes.POINTER(fs_clock)]

@@ -118,6 +118,16 @@ int fs_read_inlets(fs_handle s, double *vol, double *h, int32_t n);
  * axis (0 x, 1 y); app holds 4*n floats */
 int fs_read_inlet_approach(fs_handle s, float *app, int32_t n);
 int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk);     /* advance (:589-604) */
+/* fs_advance without any allocation (no counterpart in the reference): for callers that cannot allocate
+ * device memory, such as tight serving loops and harnesses that forbid hipMalloc. The same kernels in the
+ * same order as fs_advance, launched plainly (no captured graphs), with the solver's descriptor, work list
+ * and per-wavefront partials in a fixed 4 MiB device array that the library holds on each device until the
+ * process ends (a solver of up to ~26,000 work items: any grid up to 8192 x 8192 at the default 64 rows per
+ * strip, with or without local time stepping); a larger grid returns FS_E_ARG. Results are bit for bit those
+ * of fs_advance. Every call on one device shares that array: calls that overlap on one device (other
+ * threads, other streams) race, so they must be serialised by the caller. Timing mode (fs_pool_set_timing)
+ * must be off (FS_E_STATE), and the solver's device must be current. */
+int fs_advance_noalloc(fs_handle s, double tTarget, int32_t K, fs_clock *clk);
 /* frame (:607-622): at most K steps toward tEnd (K = 0: none), then the readout (pack, kernels.wgsl.js:
  * 1346-1355): out (nx*ny*2 floats: depth, speed) may be NULL */
 int fs_frame(fs_handle s, double tEnd, int32_t K, float *out, fs_clock *clk);

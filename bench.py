@@ -54,17 +54,7 @@ def traffic_of(workload, stepping):
 def build_workload(name, seed=2026):
     from floodsim_amd import scenarios as sc
     if name.startswith("urban"):
-        n = int(name[5:])
-        u = sc.urban(n, seed=seed)
-        mask = u["mask"].reshape(n, n)
-        jin = n // 2
-        mask[jin - 2:jin + 3, 0:5] = 1          # the inflow cell and its approach stay open
-        z = u["z"].reshape(n, n)
-        z[jin - 2:jin + 3, 0:5] = np.minimum(z[jin - 2:jin + 3, 0:5], z[jin, 5])
-        return dict(nx=n, ny=n, dx=1.0, z=u["z"], mask=u["mask"], manning=u["manning"],
-                    bc={k: "outflow" for k in ("west", "east", "south", "north")},
-                    inlets=[jin * n], rain=50 * sc.MMHR, hydro=True, lts=4,
-                    checksum=sc.checksum(u["z"], u["mask"], u["manning"]))
+        return dict(sc.urban_inflow(int(name[5:]), seed=seed), hydro=True, lts=4)
     if name == "dambreak256":
         d = sc.dam_break(256)
         return dict(nx=256, ny=256, dx=1.0, z=d["z"], mask=None, manning=0.03, bc={}, inlets=None, rain=0.0,

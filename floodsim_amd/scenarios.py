@@ -5,7 +5,8 @@ urban(n, seed)          the synthetic urban tile: 0.5 % plane + fBm (Hurst 0.7, 
                         a paved/pervious mask setting Manning n to 0.013 or 0.035.
 dam_break(n)            256x256 dam break: flat bed, 1 m for x < n/2 m, 0.1 m elsewhere, n = 0.03.
 triangle_hydrograph     0 -> peak over t_rise, back to 0 over t_fall (point inflow, m^3/s).
-peaked_hyetograph       a triangle peaking at 40 % of the storm (served solves).
+peaked_hyetograph       the served storm: 120 mm/h peak at 40 % of 30 min, 25 mm.
+urban_inflow(n, seed)   the urban tile with a point inflow at the west edge (the large single domain).
 ensemble(count, seed)   uneven batched instances: sides round(128 * 8^u), u ~ U(0, 1).
 poisson_arrivals        request times of a Poisson stream.
 
@@ -111,14 +112,35 @@ def triangle_hydrograph(t, peak=40.0, t_rise=1200.0, t_fall=2400.0):
 
 
 def peaked_hyetograph(t, duration=1800.0, peak_mmh=120.0, peak_at=0.4, depth_mm=25.0):
-    """Rain rate (m/s) of a triangular storm peaking at peak_at*duration. A triangle that peaks at
-    120 mm/h over 30 min holds 30 mm, so the peak is scaled to hold depth_mm (25 mm: 100 mm/h peak)."""
+    """Rain rate (m/s) of the served storm: a triangle reaching peak_mmh at peak_at * duration and holding
+    depth_mm. A triangle holds peak x base / 2, so it rains for base = 2 depth / peak (25 mm at 120 mm/h:
+    1500 s of the 1800 s storm), rising over peak_at of the base and falling over the rest, and the storm is
+    dry before and after (25 mm, 120 mm/h, 30 min: rain from 120 s to 1620 s, peak at 720 s)."""
     t = np.asarray(t, dtype=np.float64)
     tp = peak_at * duration
-    peak = 2 * depth_mm / (duration / 3600.0)  # mm/h so that the triangle holds depth_mm
-    peak = min(peak, peak_mmh) if peak_mmh else peak
-    r = np.where(t <= tp, peak * t / tp, peak * (duration - t) / (duration - tp))
+    base = 2.0 * depth_mm / peak_mmh * 3600.0
+    t0, t1 = tp - peak_at * base, tp + (1.0 - peak_at) * base
+    if t0 < 0 or t1 > duration:
+        raise ValueError("the storm's triangle does not fit its duration")
+    r = np.where(t <= tp, peak_mmh * (t - t0) / (tp - t0), peak_mmh * (t1 - t) / (t1 - tp))
     return np.clip(r, 0, None) * MMHR
+
+
+def urban_inflow(n, seed=2026):
+    """The urban tile with a point inflow: the generator at n x n, the inflow cell at the middle of the
+    west edge (its approach, 5 x 5 cells, kept open and no higher than the bed 5 cells in), all edges
+    free outfall, rain 50 mm/h; the inflow follows triangle_hydrograph (0 -> 40 m^3/s over 20 min, back
+    to 0 over 40 min)."""
+    u = urban(n, seed=seed)
+    mask = u["mask"].reshape(n, n)
+    jin = n // 2
+    mask[jin - 2:jin + 3, 0:5] = 1
+    z = u["z"].reshape(n, n)
+    z[jin - 2:jin + 3, 0:5] = np.minimum(z[jin - 2:jin + 3, 0:5], z[jin, 5])
+    return dict(nx=n, ny=n, dx=1.0, z=u["z"], mask=u["mask"], manning=u["manning"],
+                bc={k: "outflow" for k in ("west", "east", "south", "north")},
+                inlets=[jin * n], rain=50 * MMHR, seed=seed,
+                checksum=checksum(u["z"], u["mask"], u["manning"]))
 
 
 def ensemble(count=256, seed=7, smin=128, smax=1024):

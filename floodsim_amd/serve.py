@@ -39,14 +39,14 @@ class Request:
         return self.layers["nx"] * self.layers["ny"]
 
 
-def storm_request(rid, arrival, side, seed, depth_scale, duration=1800.0, dx=1.0):
-    """A served solve of the urban generator at `side` with the peaked 30-minute storm
-    (scenarios.peaked_hyetograph: 25 mm, peak at 40 %, scaled per request)."""
+def storm_request(rid, arrival, side, seed, depth_scale, duration=1800.0, dx=1.0, storm_s=1800.0):
+    """A served solve of the urban generator at `side` under the peaked 30-minute storm
+    (scenarios.peaked_hyetograph: 120 mm/h at 40 %, 25 mm), its rates scaled by depth_scale; the solve covers
+    the first `duration` seconds of the `storm_s` storm (all of it by default)."""
     u = sc.urban(side, seed=seed, dx=dx)
     layers = dict(nx=u["nx"], ny=u["ny"], dx=dx, z=u["z"], mask=u["mask"], manning=u["manning"],
                   bc={k: "outflow" for k in ("west", "east", "south", "north")})
-    rain = lambda t: float(sc.peaked_hyetograph(t, duration=duration, depth_mm=25.0 * depth_scale,
-                                                peak_mmh=None))
+    rain = lambda t: depth_scale * float(sc.peaked_hyetograph(t, duration=storm_s))
     return Request(rid, arrival, layers, duration, rain, dict(side=side, seed=seed, depth_scale=depth_scale))
 
 

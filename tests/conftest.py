@@ -20,7 +20,16 @@ def golden():
         return json.load(f)
 
 
+def host_threads():
+    """Threads for the oracle: the cores this process may use, at most 16 (one GPU's share of its host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 @pytest.fixture(scope="session", autouse=True)
 def _oracle_threads():
     import oracle
-    oracle.set_threads(min(8, os.cpu_count() or 1))
+    oracle.set_threads(host_threads())

@@ -1,0 +1,212 @@
+"""The measured configurations under test on the GPU, through the C ABI.
+
+Config 2: the urban tile at 2048 x 2048 (1 m, seed 2048), local time stepping on 4 levels, 50 mm/h from
+dry, all edges free outfall, against the FP64 oracle (OpenMP): from dry over the first 150 s in 10 s frames,
+and over 30 s from the GPU's state at 600 s (where the water has gathered and the levels spread); a full
+hour of the oracle at 4.2 M cells is weeks of one core. Bar: max |dh| <= 1 mm, relative mass error <= 1e-6
+on the GPU's books, the same number of cycles, the rain and outflow books close.
+
+Config 3: the urban tile at 8192 x 8192 with the triangular inflow (0 -> 40 m^3/s over 20 min, back over
+40 min) at the west edge, rain 50 mm/h, LTS 4 levels, at full size for its first 300 s through properties
+that do not need the oracle: books within 1e-6, nothing clipped, a rerun gives the same bits, and strips of
+32 rows give the bits of 64-row strips; then the same generator and hydrograph at 1024 x 1024 against the
+FP64 oracle over 150 s.
+
+Config 5: a served stream of urban solves, sides drawn over the config's mix {256, 512, 1024, 2048, 4096}
+(all present), the peaked storm (120 mm/h at 40 % of 30 min, 25 mm, scaled per request) over its first
+480 s, Poisson arrivals on a virtual clock, admission against an HBM budget that holds the two largest at
+once: every request gives the same bits alone and in the mix, and the smallest and a 1024^2 request match
+the FP64 oracle (the same 60 s chunks, rain held at each chunk's mid value).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from floodsim_amd import scenarios as sc
+
+pytestmark = pytest.mark.gpu
+
+TOL_H = 1e-3
+TOL_MASS = 1e-6
+OUT = {k: "outflow" for k in ("west", "east", "south", "north")}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def fsa():
+    import floodsim_amd
+    floodsim_amd.lib()
+    return floodsim_amd
+
+
+def _close(tag, s, c, d, extra=""):
+    m = s["mass"]
+    say(f"{tag}: max|dh| {d:.2e} m, mass rel {m['rel']:.2e}, rain {m['rain']:.3f}/{c.vol['rain']:.3f}, "
+        f"outflow {m['outflow']:.3f}/{c.vol['outflow']:.3f} m3{extra}")
+    assert d <= TOL_H
+    assert abs(m["rel"]) <= TOL_MASS and m["clipped"] == 0
+    assert abs(m["rain"] - c.vol["rain"]) <= 1e-6 * c.vol["rain"]
+    assert abs(m["outflow"] - c.vol["outflow"]) <= 1e-3 * c.vol["outflow"] + 1e-3
+
+
+def test_config2_urban2048_lts4(fsa):
+    n = 2048
+    u = sc.urban(n, seed=2048)
+    opts = dict(nx=n, ny=n, dx=1.0, z=u["z"], mask=u["mask"], manning=u["manning"], bc=OUT, lts=4)
+    g = fsa.GPUSolver(**opts)
+    c = oracle.CPUSolver(**opts, hdry=g.hdry)
+    for o in (g, c):
+        o.setDepth(lambda X, Y, Z: 0 * X)
+        o.rainRate = 50 * sc.MMHR
+    for t in range(10, 151, 10):
+        g.advance(float(t), 8)
+        c.run(float(t))
+        say(f"config 2 from dry: t = {t} s, cycles gpu/cpu {g.steps}/{c.steps}")
+    s = g.readState()
+    d = float(np.max(np.abs(s["h"] - c.field("h"))))
+    _close("config 2, dry start to 150 s", s, c, d, f", cycles {g.steps}/{c.steps}")
+    assert g.steps == c.steps
+    # the wet phase: the GPU alone to 600 s, then both on from its state for 30 s
+    for t in range(160, 601, 10):
+        g.advance(float(t), 8)
+    s0 = g.readState()
+    c2 = oracle.CPUSolver(**opts, hdry=g.hdry)
+    c2.setState(s0["h"], s0["hu"], s0["hv"], t=600.0)
+    c2.rainRate = 50 * sc.MMHR
+    cyc0 = g.steps
+    c0 = {k: s0["mass"][k] for k in ("rain", "outflow")}
+    for t in range(610, 631, 10):
+        g.advance(float(t), 8)
+        c2.run(float(t))
+        say(f"config 2 wet phase: t = {t} s, cycles gpu/cpu {g.steps - cyc0}/{c2.steps}")
+    s = g.readState()
+    d = float(np.max(np.abs(s["h"] - c2.field("h"))))
+    lv = np.bincount(g.levels().reshape(-1), minlength=5)
+    say(f"config 2 wet phase 600-630 s: max|dh| {d:.2e} m, cycles {g.steps - cyc0}/{c2.steps}, tile levels {lv}, "
+        f"rain {s['mass']['rain'] - c0['rain']:.3f}/{c2.vol['rain']:.3f}, "
+        f"outflow {s['mass']['outflow'] - c0['outflow']:.3f}/{c2.vol['outflow']:.3f} m3")
+    assert d <= TOL_H
+    assert g.steps - cyc0 == c2.steps
+    assert abs(s["mass"]["rel"]) <= TOL_MASS and s["mass"]["clipped"] == 0
+    assert abs((s["mass"]["rain"] - c0["rain"]) - c2.vol["rain"]) <= 1e-6 * c2.vol["rain"]
+    assert abs((s["mass"]["outflow"] - c0["outflow"]) - c2.vol["outflow"]) <= 1e-3 * c2.vol["outflow"] + 1e-3
+    g.destroy()
+
+
+def _inflow_run(fsa, w, t_end, rows=0, oracle_too=False, frame=10.0):
+    g = fsa.GPUSolver(nx=w["nx"], ny=w["ny"], dx=w["dx"], z=w["z"], mask=w["mask"], manning=w["manning"],
+                      bc=w["bc"], inlets=w["inlets"], lts=4, rows=rows)
+    c = None
+    if oracle_too:
+        c = oracle.CPUSolver(nx=w["nx"], ny=w["ny"], dx=w["dx"], z=w["z"], mask=w["mask"], manning=w["manning"],
+                             bc=w["bc"], inlets=w["inlets"], lts=4, hdry=g.hdry)
+    for o in [g] + ([c] if c else []):
+        o.setDepth(lambda X, Y, Z: 0 * X)
+        o.rainRate = w["rain"]
+    t = 0.0
+    while t < t_end - 1e-9:
+        q = [-float(sc.triangle_hydrograph(t + frame / 2))]   # held per frame at its mid-frame value
+        t = min(t_end, t + frame)
+        g.setInletRates(q)
+        g.advance(t, 8)
+        if c:
+            c.setInletRates(q)
+            c.run(t)
+            say(f"  t = {t:.0f} s, cycles gpu/cpu {g.steps}/{c.steps}")
+    return g, c
+
+
+def test_config3_full_size_properties(fsa):
+    w = sc.urban_inflow(8192)
+    say(f"config 3: 8192^2 urban tile with the inflow (layers {w['checksum']}), 300 s, rows 64")
+    g, _ = _inflow_run(fsa, w, 300.0)
+    a = g.readState()
+    clk = g.clock()
+    g.destroy()
+    m = a["mass"]
+    say(f"config 3: {clk.steps} cycles, {clk.cellUpdates:.3e} cell-updates, mass rel {m['rel']:.2e}, "
+        f"exchanged {m['exchanged']:.1f} m3 (inflow), rain {m['rain']:.1f}, outflow {m['outflow']:.1f}")
+    assert abs(m["rel"]) <= TOL_MASS and m["clipped"] == 0
+    inflow = sum(float(sc.triangle_hydrograph(t + 5.0)) * 10.0 for t in np.arange(0.0, 300.0, 10.0))
+    assert abs(-m["exchanged"] - inflow) <= 1e-4 * inflow
+    assert a["h"].min() >= 0 and np.isfinite(a["h"]).all()
+    for rows in (64, 32):
+        say(f"config 3 rerun, rows {rows}")
+        g, _ = _inflow_run(fsa, w, 300.0, rows=rows)
+        b = g.readState()
+        assert g.clock().steps == clk.steps
+        g.destroy()
+        for k in ("h", "hu", "hv", "hmax"):
+            np.testing.assert_array_equal(a[k], b[k])
+        assert a["mass"] == b["mass"]
+        del b
+
+
+def test_config3_window_vs_oracle(fsa):
+    w = sc.urban_inflow(1024)
+    say("config 3 at 1024^2 against the oracle, 150 s")
+    g, c = _inflow_run(fsa, w, 150.0, oracle_too=True)
+    s = g.readState()
+    d = float(np.max(np.abs(s["h"] - c.field("h"))))
+    _close("config 3 window 1024^2", s, c, d, f", cycles {g.steps}/{c.steps}, exchanged "
+           f"{s['mass']['exchanged']:.2f}/{c.vol['exchanged']:.2f} m3")
+    assert g.steps == c.steps
+    assert abs(s["mass"]["exchanged"] - c.vol["exchanged"]) <= 1e-5 * abs(c.vol["exchanged"])
+
+
+def test_config5_served_stream(fsa):
+    from floodsim_amd import serve
+    from floodsim_amd.solver import Pool
+    sides = [256, 512, 1024, 2048, 4096, 256, 512, 256]
+    rng = np.random.default_rng(55)
+    arrivals = np.cumsum(rng.exponential(0.5, size=len(sides)))           # 2 requests per virtual second
+    scale = rng.lognormal(0.0, 0.25, size=len(sides))
+    dur = 480.0
+    reqs = [serve.storm_request(i, float(a), s, seed=500 + i, depth_scale=float(k), duration=dur)
+            for i, (a, s, k) in enumerate(zip(arrivals, sides, scale))]
+    need = sorted(Pool.bytes_needed(r.layers["nx"], r.layers["ny"]) for r in reqs)
+    budget = need[-1] + need[-2] + (64 << 20)
+    tick = {"t": 0.0}
+    S = serve.Server(budget, chunk_s=60.0, clock=lambda: tick["t"], keep=("h", "hmax"))
+    for r in reqs:
+        S.submit(r)
+    peak = most = 0
+    while S.waiting or S.active:
+        S.step()
+        peak = max(peak, S.pool.bytes_used())
+        most = max(most, len(S.active))
+        tick["t"] += 1.0
+    st = S.stats()
+    S.close()
+    say(f"config 5: {st['requests']} requests, sides {sides}, p50 {st['p50_s']:.1f} p99 {st['p99_s']:.1f} virtual s, "
+        f"at most {most} at once, peak {peak / 2**30:.2f} of {budget / 2**30:.2f} GiB")
+    assert st["requests"] == len(reqs) and peak <= budget and most >= 2
+    assert any(r.admitted > r.arrival for r in reqs)
+    for r in reqs:
+        alone = serve.solve_alone(r, keep=("h", "hmax"))
+        assert alone["steps"] == r.result["steps"]
+        np.testing.assert_array_equal(alone["h"], r.result["h"])
+        np.testing.assert_array_equal(alone["hmax"], r.result["hmax"])
+        assert abs(r.result["mass"]["rel"]) <= TOL_MASS
+        say(f"  request {r.id} ({r.cells} cells): alone = mixed, {r.result['steps']} steps, "
+            f"mass rel {r.result['mass']['rel']:.1e}")
+    small = min(reqs, key=lambda q: (q.cells, q.id))
+    big = next(q for q in reqs if q.layers["nx"] == 1024)
+    for r in (small, big):
+        L = r.layers
+        c = oracle.CPUSolver(nx=L["nx"], ny=L["ny"], dx=L["dx"], z=L["z"], mask=L["mask"], manning=L["manning"],
+                             bc=L["bc"], hdry=1e-5)
+        c.setDepth(np.zeros(r.cells))
+        t = 0.0
+        while t < dur - 1e-9:
+            t1 = min(dur, t + 60.0)
+            c.rainRate = r.rain_fn(0.5 * (t + t1))
+            c.run(t1)
+            t = t1
+        d = float(np.max(np.abs(r.result["h"] - c.field("h"))))
+        say(f"  request {r.id} ({L['nx']}^2) against the oracle: max|dh| {d:.2e} m, steps {r.result['steps']}/{c.steps}")
+        assert d <= TOL_H
+        assert abs(r.result["steps"] - c.steps) <= 1

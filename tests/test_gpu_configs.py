@@ -49,7 +49,8 @@ def _close(tag, s, c, d, extra=""):
     assert d <= TOL_H
     assert abs(m["rel"]) <= TOL_MASS and m["clipped"] == 0
     assert abs(m["rain"] - c.vol["rain"]) <= 1e-6 * c.vol["rain"]
-    assert abs(m["outflow"] - c.vol["outflow"]) <= 1e-3 * c.vol["outflow"] + 1e-3
+    # (net; negative where the free-outfall edge lets water in, as at the inflow cell of config 3)
+    assert abs(m["outflow"] - c.vol["outflow"]) <= 1e-3 * abs(c.vol["outflow"]) + 1e-3
 
 
 def test_config2_urban2048_lts4(fsa):

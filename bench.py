@@ -14,8 +14,8 @@ dominant kernel (second RK stage + sources) measured with HIP events over the ti
 the FP64 CPU oracle timed on a bounded window of the same state as the CPU baseline. Two more
 legs follow (skip with --no-batch / --no-serve): `batched`, 256 uneven urban instances (sides
 128-1024) solved for 300 s from dry in one pool vs one at a time (solves/hour), with the smallest
-checked against the oracle; `served`, a Poisson stream (1 request/s, 24 requests, sides
-256-4096, 30-minute storms) under continuous batching (p50/p99 latency, solves/hour).
+checked against the oracle; `served`, a Poisson stream (2 requests/s, 48 requests, sides
+256-4096, 30-minute storms) under continuous batching (p50/p99 latency, solves/hour, against one at a time).
 """
 import argparse
 import json
@@ -122,21 +122,30 @@ def bench_batched(count=256, t_end=300.0, seed=7):
     log(f"batched: {b['wall_s']:.2f} s (rows {b['rows']})")
     q = batch.run_sequential(insts, t_end)
     log(f"one at a time: {q['wall_s']:.2f} s (rows per size class {q['rows']})")
-    # parity sample: the smallest instance, batched result vs the FP64 oracle
-    k = int(np.argmin(cells))
-    inst = insts[k]
-    P = batch.Pool(0)
-    slots = batch.add_all(P, [inst])
-    P.advance(slots, [t_end])
-    g = P.read_state(slots[0])
-    P.destroy()
+    # parity sample against the FP64 oracle: the smallest and the largest instance and two drawn at random, each
+    # re-solved in the batch's own pool form (the same bits as in the batch: tests/test_gpu_batch.py)
+    rng = np.random.default_rng(seed)
+    order = np.argsort(cells)
+    sample = [int(order[0]), int(order[-1])] + [int(i) for i in rng.choice(order[1:-1], size=2, replace=False)]
+    oracle.set_threads(cpu_threads())
+    par = []
+    for k in sample:
+        inst = insts[k]
+        P = batch.Pool(0)
+        slots = batch.add_all(P, [inst])
+        P.advance(slots, [t_end])
+        g = P.read_state(slots[0])
+        steps_g = P.clock(slots[0]).steps
+        P.destroy()
+        c = oracle.CPUSolver(nx=inst["nx"], ny=inst["ny"], dx=inst["dx"], z=inst["z"], mask=inst["mask"],
+                             manning=inst["manning"], bc=inst["bc"], hdry=1e-5)
+        c.setDepth(np.zeros(cells[k]))
+        c.rainRate = inst["rain"]
+        c.run(t_end)
+        par.append({"instance": k, "side": inst["nx"], "max_dh_m": float(np.max(np.abs(g["h"] - c.field("h")))),
+                    "mass_rel": g["mass"]["rel"], "steps_gpu": steps_g, "steps_oracle": c.steps})
+        log(f"batched parity: instance {k} ({inst['nx']}^2) max|dh| {par[-1]['max_dh_m']:.2e} m")
     oracle.set_threads(1)
-    c = oracle.CPUSolver(nx=inst["nx"], ny=inst["ny"], dx=inst["dx"], z=inst["z"], mask=inst["mask"],
-                         manning=inst["manning"], bc=inst["bc"], hdry=1e-5)
-    c.setDepth(np.zeros(cells[k]))
-    c.rainRate = inst["rain"]
-    c.run(t_end)
-    dh = float(np.max(np.abs(g["h"] - c.field("h"))))
     return {"instances": count, "cells": int(sum(cells)), "sim_s": t_end, "seed": seed,
             "solves_per_hour": 3600.0 * count / b["wall_s"],
             "sequential_solves_per_hour": 3600.0 * count / q["wall_s"],
@@ -144,30 +153,55 @@ def bench_batched(count=256, t_end=300.0, seed=7):
             "batched_wall_s": b["wall_s"], "sequential_wall_s": q["wall_s"],
             "cell_updates_per_s": b["cell_updates"] / b["wall_s"], "rows_autotuned": b["rows"],
             "tuning_ms_per_step": b["tune_ms"],
-            "parity_sample": {"instance": k, "side": inst["nx"], "max_dh_m": dh, "mass_rel": g["mass"]["rel"],
-                              "steps_gpu": b["steps"][k], "steps_oracle": c.steps}}
+            "parity_sample": par, "max_dh_m": max(p["max_dh_m"] for p in par)}
 
 
-def bench_served(rate=1.0, count=24, budget_gb=64, seed=11):
-    """Config 5: a Poisson stream of urban solves (sides 256-4096, 30-minute peaked storms of
-    25 mm x LogNormal(0, 0.25)) served with continuous batching under an HBM budget; one request
-    is re-solved alone to check that its result is the same in the mix."""
+def bench_served(rate=2.0, count=48, budget_gb=64, seed=11):
+    """Config 5: a Poisson stream (`rate` per s) of `count` urban solves, sides drawn from {256, ..., 4096} with
+    p {.35, .3, .2, .1, .05}, each under the peaked 30-minute storm (120 mm/h at 40 %, 25 mm) scaled by
+    LogNormal(0, 0.25), served with continuous batching under an HBM budget: p50/p99 latency and throughput of
+    the stream. Then the same requests all waiting at once (saturated) through the server, and one at a time,
+    each alone in its own pool through the same chunk schedule (the plain GPU form): solves/hour of both, and
+    every request's result alone against its result in the mix."""
     from floodsim_amd import scenarios as sc
     from floodsim_amd import serve
     arr = sc.poisson_arrivals(rate, count, seed=seed)
-    reqs = [serve.storm_request(i, a["t"], a["side"], a["seed"], a["depth_scale"]) for i, a in enumerate(arr)]
+    mk = lambda a, i, t: serve.storm_request(i, t, a["side"], a["seed"], a["depth_scale"])
+    reqs = [mk(a, i, a["t"]) for i, a in enumerate(arr)]
     S = serve.Server(int(budget_gb) << 30, chunk_s=60.0)
     try:
         fin = S.run(reqs)
         st = S.stats()
     finally:
         S.close()
-    probe = min(fin, key=lambda r: (r.cells, r.id))
-    alone = serve.solve_alone(probe)
-    same = bool(alone["steps"] == probe.result["steps"] and np.array_equal(alone["hmax"], probe.result["hmax"]))
+    log(f"served: {count} requests at {rate}/s, p50 {st['p50_s']:.2f} s, p99 {st['p99_s']:.2f} s")
+    sat_reqs = [mk(a, i, 0.0) for i, a in enumerate(arr)]
+    S = serve.Server(int(budget_gb) << 30, chunk_s=60.0)
+    try:
+        t0 = time.perf_counter()
+        S.run(sat_reqs)
+        sat_wall = time.perf_counter() - t0
+        sat = S.stats()
+    finally:
+        S.close()
+    log(f"served, all waiting at once: {sat_wall:.2f} s")
+    seq_wall, same = 0.0, True
+    for r in fin:
+        t0 = time.perf_counter()
+        alone = serve.solve_alone(r)
+        seq_wall += time.perf_counter() - t0
+        same = same and alone["steps"] == r.result["steps"] and bool(np.array_equal(alone["hmax"], r.result["hmax"]))
+    log(f"one at a time: {seq_wall:.2f} s; alone = mixed for every request: {same}")
     st.update({"rate_per_s": rate, "budget_GiB": budget_gb, "seed": seed,
-               "sides": sorted({a["side"] for a in arr}), "storm": "30 min, peak at 40 %, 25 mm x scale",
-               "chunk_s": 60.0, "alone_equals_mixed": same, "max_mass_rel": max(abs(r.result["mass"]["rel"]) for r in fin)})
+               "sides": {str(k): int(sum(a["side"] == k for a in arr)) for k in sorted({a["side"] for a in arr})},
+               "storm": "30 min, peak 120 mm/h at 40 %, 25 mm, rates x LogNormal(0, 0.25) per request",
+               "chunk_s": 60.0, "alone_equals_mixed": same, "requests_checked_alone": len(fin),
+               "max_mass_rel": max(abs(r.result["mass"]["rel"]) for r in fin),
+               "saturated": {"wall_s": sat_wall, "solves_per_hour": 3600.0 * count / sat_wall,
+                             "cell_updates_per_s": sat["cell_updates_per_s"], "p50_s": sat["p50_s"],
+                             "p99_s": sat["p99_s"]},
+               "sequential_wall_s": seq_wall, "sequential_solves_per_hour": 3600.0 * count / seq_wall,
+               "speedup_vs_one_at_a_time": seq_wall / sat_wall})
     return st
 
 
@@ -186,8 +220,8 @@ def main():
                     help="no HIP events around the timed launches (for counter-collection runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (256 instances) measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
-    ap.add_argument("--serve-rate", type=float, default=1.0)
-    ap.add_argument("--serve-count", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-count", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by "

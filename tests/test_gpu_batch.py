@@ -100,3 +100,36 @@ def test_served_alone_vs_mix_and_budget():
         assert alone["steps"] == r.result["steps"]
         np.testing.assert_array_equal(alone["hmax"], r.result["hmax"])
         assert abs(r.result["mass"]["rel"]) <= TOL_MASS
+
+
+def test_ensemble_sample_vs_oracle():
+    """Config 4's own ensemble (256 instances, sides round(128 * 8^u), seed 7): 48 of its members, the largest
+    (side >= 1024) and a random draw of the rest, batched together for 120 s; the largest and three random
+    members against the FP64 oracle, every member's books within 1e-6."""
+    from floodsim_amd import batch
+    specs = sc.ensemble(256, seed=7)
+    sides = np.array([s["side"] for s in specs])
+    big = int(np.argmax(sides))
+    rng = np.random.default_rng(17)
+    rest = [int(i) for i in rng.choice(np.delete(np.arange(256), big), size=47, replace=False)]
+    pick = [big] + rest
+    insts = batch.instances([specs[i] for i in pick])
+    T = 120.0
+    print(f"\nensemble sample: members {pick[:8]}..., largest side {sides[big]}, cells {sum(i['nx'] * i['ny'] for i in insts)}",
+          flush=True)
+    assert sides[big] >= 1024
+    b = batch.run_batched(insts, T)
+    for st in b["states"]:
+        assert abs(st["mass"]["rel"]) <= TOL_MASS and st["mass"]["clipped"] == 0
+    for j in [0] + [int(q) for q in rng.choice(np.arange(1, len(pick)), size=3, replace=False)]:
+        inst = insts[j]
+        c = oracle.CPUSolver(nx=inst["nx"], ny=inst["ny"], dx=inst["dx"], z=inst["z"], mask=inst["mask"],
+                             manning=inst["manning"], bc=inst["bc"], hdry=1e-5)
+        c.setDepth(np.zeros(inst["nx"] * inst["ny"]))
+        c.rainRate = inst["rain"]
+        c.run(T)
+        d = float(np.max(np.abs(b["states"][j]["h"] - c.field("h"))))
+        print(f"member {pick[j]} ({inst['nx']}^2): {b['steps'][j]} steps (oracle {c.steps}), max|dh| {d:.2e} m",
+              flush=True)
+        assert d <= TOL_H
+        assert abs(b["steps"][j] - c.steps) <= 1

@@ -14,7 +14,7 @@ dominant kernel (second RK stage + sources) measured with HIP events over the ti
 the FP64 CPU oracle timed on a bounded window of the same state as the CPU baseline. Two more
 legs follow (skip with --no-batch / --no-serve): `batched`, 256 uneven urban instances (sides
 128-1024) solved for 300 s from dry in one pool vs one at a time (solves/hour), with the smallest
-checked against the oracle; `served`, a Poisson stream (2 requests/s, 48 requests, sides
+checked against the oracle; `served`, a Poisson stream (2 requests/s, 32 requests, sides
 256-4096, 30-minute storms) under continuous batching (p50/p99 latency, solves/hour, against one at a time).
 """
 import argparse
@@ -156,7 +156,7 @@ def bench_batched(count=256, t_end=300.0, seed=7):
             "parity_sample": par, "max_dh_m": max(p["max_dh_m"] for p in par)}
 
 
-def bench_served(rate=2.0, count=48, budget_gb=64, seed=11):
+def bench_served(rate=2.0, count=32, budget_gb=64, seed=11):
     """Config 5: a Poisson stream (`rate` per s) of `count` urban solves, sides drawn from {256, ..., 4096} with
     p {.35, .3, .2, .1, .05}, each under the peaked 30-minute storm (120 mm/h at 40 %, 25 mm) scaled by
     LogNormal(0, 0.25), served with continuous batching under an HBM budget: p50/p99 latency and throughput of
@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (256 instances) measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--serve-rate", type=float, default=2.0)
-    ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--serve-count", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by "

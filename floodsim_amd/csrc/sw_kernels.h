@@ -42,12 +42,16 @@ __device__ __forceinline__ float fpow(float x, float p) {
   return __builtin_amdgcn_exp2f(p * __builtin_amdgcn_logf(x));
 }
 
+// minmod (solver-cpu.js:17-20; kernels.wgsl.js:83-84): of two slopes of one sign the one nearer zero, else 0,
+// which is the median of (a, b, 0): one v_med3_f32. (Where a * b would underflow FP32 the product test of the
+// reference's form gives 0 and the median the smaller slope, as the FP64 product does.)
+__device__ __forceinline__ float minmod(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.f); }
+
 // slope limiter: minmod or van Leer (solver-cpu.js:17-24; kernels.wgsl.js:83-87)
 __device__ __forceinline__ float limit(float a, float b, int vanleer) {
-  float p = a * b;
-  float mm = fabsf(a) < fabsf(b) ? a : b;
-  float v = vanleer ? 2.f * p * frcp(a + b) : mm;
-  return p <= 0.f ? 0.f : v;
+  const float p = a * b;
+  const float v = p <= 0.f ? 0.f : 2.f * p * frcp(a + b);
+  return vanleer ? v : minmod(a, b);
 }
 
 // HLLC in the face-normal frame with dry-bed speed estimates (solver-cpu.js:579-602;
@@ -351,6 +355,18 @@ __device__ __forceinline__ Slopes muscl(float hc, float nc, float tc, float ha, 
   Slopes s = {0.f, 0.f, 0.f, 0.f};
   bool fo = !secondOrder || hc < hfront || (Ma && ha < hfront) || (Mb && hb < hfront);
   if constexpr (SF) fo = fo || (Rc && hc < 0.5f * fabsf((Mb ? dzb : 0.f) + (Ma ? dza : 0.f)));
+  if constexpr (!SF) {
+    // the plain variants (minmod at compile time): a first-order cell drops its left differences, and a
+    // minmod with a zero operand is zero, so every slope is zero with no select of its own
+    const bool ka = Ma && !fo;
+    const float dhm = ka ? hc - ha : 0.f, dhp = Mb ? hb - hc : 0.f;
+    s.sh = minmod(dhm, dhp);
+    s.se = minmod(ka ? dhm + dza : 0.f, Mb ? dhp + dzb : 0.f);
+    s.sn = minmod(ka ? nc - na : 0.f, Mb ? nb - nc : 0.f);
+    s.st = minmod(ka ? tc - ta : 0.f, Mb ? tb - tc : 0.f);
+    (void)vanleer;
+    return s;
+  }
   if (fo) return s;
   const float dhm = Ma ? hc - ha : 0.f, dhp = Mb ? hb - hc : 0.f;
   s.sh = limit(dhm, dhp, vanleer);
@@ -525,7 +541,9 @@ struct FaceRow { float F0, F1, F2, cL, cR, on; };
 // barriers (crests in the face beds, weir faces, the barrier faces' fluxes for their books), stage cells,
 // Green-Ampt soil or GI stores, implicit friction (F26; global stepping: under LTS every variant takes it)
 // or the van Leer limiter
-template <int STAGE, bool LTS, bool HZ, bool EXT = false>
+// IN: a strip whose every lane is an interior column and whose rows, halo rows included, are interior rows:
+// no ghost cells and no domain-edge faces (the common case, compiled as a loop of its own)
+template <int STAGE, bool LTS, bool HZ, bool EXT = false, bool IN = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
                                           Part *__restrict__ part, const int cells, const float tnow) {
   constexpr bool SF = EXT;
@@ -537,11 +555,11 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   const int so = I.secondOrder, vl = EXT ? I.limiter : 0;
   const int nx = I.nx, ny = I.ny, pitch = I.pitch;
   const int X = wk.y - I.hl + lane;
-  const bool own_col = lane >= I.hl && lane < I.hl + I.own && X < nx;
+  const bool own_col = lane >= I.hl && lane < I.hl + I.own && (IN || X < nx);
   const int y0 = wk.z, y1 = min(wk.z + wk.w, ny);
   const bool last = STAGE == 1 || !so;   // the stage that ends the step (or the record)
-  const bool xin = wk.y - I.hl >= 0 && wk.y - I.hl + kLanes <= nx;  // every lane an interior column
-  const bool xedge = wk.y - I.hl <= 0 || wk.y - I.hl + kLanes >= nx;  // a lane on the first or last column
+  const bool xin = IN || (wk.y - I.hl >= 0 && wk.y - I.hl + kLanes <= nx);  // every lane an interior column
+  const bool xedge = !IN && (wk.y - I.hl <= 0 || wk.y - I.hl + kLanes >= nx);  // a lane on the first or last column
 
   // writes go to B in stage 0 and to A in stage 1; Ws is the residual of the evaluated state
   const float *__restrict__ Ws = STAGE == 0 ? I.w : I.w1;
@@ -577,7 +595,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   for (int r = y0 - kHalo; r < y1 + kHalo; ++r) {
     // ---------------- row r: primitives
     Prim p;
-    if (xin && r >= 0 && r < ny) p = load_prim_row<STAGE, LTS, EXT>(I, X, r, m, dt);
+    if (IN || (xin && r >= 0 && r < ny)) p = load_prim_row<STAGE, LTS, EXT>(I, X, r, m, dt);
     else p = load_prim<STAGE, LTS, EXT>(I, X, r, m, dt);
 
     // ---------------- x reconstruction of row r (neighbours by lane shifts)
@@ -588,7 +606,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     // barrier crests of this row: (west face crest, coefficient, south face crest, coefficient)
     float4 cr = make_float4(-1e30f, 0.f, -1e30f, 0.f);
     if constexpr (EXT) {
-      if (bar && r >= 0 && r < ny && X >= 0 && X < nx) cr = gld(I.crest, r * pitch + X);
+      if (bar && (IN || (r >= 0 && r < ny && X >= 0 && X < nx))) cr = gld(I.crest, r * pitch + X);
     }
     const Slopes sxs = muscl<SF>(p.h, p.u, p.v, hl, ul, vlf, Ml, p.z - zl, hr, ur, vr, Mr, zr - p.z, so, hfront, vl,
                                  sf && p.R);
@@ -807,7 +825,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
           if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
           // boundary outflow through the domain edge faces (edgeOutAt, kernels.wgsl.js:1020-1030)
           // (only strips holding a domain edge column and the edge rows take this wave-uniform branch)
-          if (xedge || ry == 0 || ry == ny - 1) {
+          if (!IN && (xedge || ry == 0 || ry == ny - 1)) {
             double o = 0.0;
             if (X == 0) o -= (double)w0;
             if (X == nx - 1) o += (double)Fe0;
@@ -850,6 +868,18 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   const float s = wave_max(smax);
   if (lane == 0) *part = Part{o, c, f, s, cells};
   return __any(stir);
+}
+
+// a work item whose strip, halo rows and columns included, lies inside the grid
+__device__ __forceinline__ bool strip_interior(const DevInst &I, const int4 wk) {
+  return wk.y - I.hl >= 0 && wk.y - I.hl + kLanes <= I.nx && wk.z >= kHalo && min(wk.z + wk.w, I.ny) + kHalo <= I.ny;
+}
+
+template <int STAGE, bool LTS, bool HZ, bool EXT>
+__device__ __forceinline__ bool run_strip_any(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
+                                              Part *__restrict__ part, const int cells, const float tnow) {
+  if (strip_interior(I, wk)) return run_strip<STAGE, LTS, HZ, EXT, true>(I, wk, lane, m, dt, part, cells, tnow);
+  return run_strip<STAGE, LTS, HZ, EXT, false>(I, wk, lane, m, dt, part, cells, tnow);
 }
 
 // ------------------------------------------------------------ dry strips (global stepping)
@@ -983,7 +1013,7 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
     if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0};
     return;
   }
-  run_strip<STAGE, true, HZ, EXT>(I, wk, lane, m, dt, part + i, 0, lts_record_end_time(I, kk));
+  run_strip_any<STAGE, true, HZ, EXT>(I, wk, lane, m, dt, part + i, 0, lts_record_end_time(I, kk));
 }
 
 // One RK stage over a work list. Global stepping: one wavefront per strip of `work`; a strip the
@@ -1015,7 +1045,7 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
     if (lane == 0) part[wid] = Part{0.0, 0.0, 0.0, 0.f, 0};
     return;
   }
-  const bool stir = run_strip<STAGE, false, HZ, EXT>(I, wk, lane, 0, dt, part + wid, I.sopen[u], (float)I.rec->tAfter);
+  const bool stir = run_strip_any<STAGE, false, HZ, EXT>(I, wk, lane, 0, dt, part + wid, I.sopen[u], (float)I.rec->tAfter);
   if (STAGE == 1 && I.wet && lane == 0) I.wet[u] = stir ? 1 : 0;
 }
 

@@ -104,7 +104,7 @@ def test_served_alone_vs_mix_and_budget():
 
 def test_ensemble_sample_vs_oracle():
     """Config 4's own ensemble (256 instances, sides round(128 * 8^u), seed 7): 48 of its members, the largest
-    (side >= 1024) and a random draw of the rest, batched together for 120 s; the largest and three random
+    (side 1014, the top of the 128-1024 range this draw reaches) and a random draw of the rest, batched together for 120 s; the largest and three random
     members against the FP64 oracle, every member's books within 1e-6."""
     from floodsim_amd import batch
     specs = sc.ensemble(256, seed=7)
@@ -117,7 +117,7 @@ def test_ensemble_sample_vs_oracle():
     T = 120.0
     print(f"\nensemble sample: members {pick[:8]}..., largest side {sides[big]}, cells {sum(i['nx'] * i['ny'] for i in insts)}",
           flush=True)
-    assert sides[big] >= 1024
+    assert sides[big] >= 1000
     b = batch.run_batched(insts, T)
     for st in b["states"]:
         assert abs(st["mass"]["rel"]) <= TOL_MASS and st["mass"]["clipped"] == 0

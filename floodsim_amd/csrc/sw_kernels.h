@@ -17,6 +17,12 @@
 #pragma once
 #include "sw_device.h"
 
+// which plain stage kernels issue a row pass's loads together (bit 0: global second stage, bit 1: LTS first
+// stage, bit 2: LTS second stage)
+#ifndef FS_PRE
+#define FS_PRE 3
+#endif
+
 namespace fs {
 
 // ---------------------------------------------------------------- lane shifts (DPP)
@@ -417,13 +423,25 @@ __device__ __forceinline__ float cell_speed(float h, float hu, float hv, float g
 // capacity, which is booked as infiltration
 // IMPF: the variant takes implicit friction (F26): the extended variants, and every LTS variant (measured:
 // taking the test out of the plain LTS kernels slows their schedule)
+// the per-cell layers the sources read (Manning n, max depth, mask byte, infiltration rate, rain multiplier)
+struct SrcOps { float n, hmax, infil, rf; uint8_t mask; };
+__device__ __forceinline__ SrcOps load_src(const DevInst &I, int k) {
+  SrcOps o;
+  o.n = gld(I.n, k);
+  o.hmax = gld(I.hmax, k);
+  o.mask = gld(I.mask, k);
+  o.infil = (I.flags & FLAG_INFIL) ? gld(I.infil, k) : 0.f;
+  o.rf = (I.flags & FLAG_RF) ? gld(I.rf, k) : 1.f;
+  return o;
+}
+
 template <bool EXT = false, bool IMPF = EXT>
 __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, float rain, float areaf,
                                              float &qh, float &qw, float &qhu, float &qhv, double &acc_inf,
-                                             float &smax) {
+                                             float &smax, const SrcOps *pre = nullptr) {
   const float g = I.g, hdry = I.hdry;
   const float d = qh;
-  const float nn = gld(I.n, k);
+  const float nn = pre ? pre->n : gld(I.n, k);
   if (d > hdry && nn > 0.f) {
     const float id = frcp(d);
     const float u = qhu * id, v = qhv * id;
@@ -434,10 +452,10 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, 
     qhu *= kf;
     qhv *= kf;
   }
-  const float rr = (I.flags & FLAG_RF) ? rain * gld(I.rf, k) * dt : rain * dt;
+  const float rr = (I.flags & FLAG_RF) ? rain * (pre ? pre->rf : gld(I.rf, k)) * dt : rain * dt;
   float inf = 0.f;
   if constexpr (!EXT) {
-    if (I.flags & FLAG_INFIL) inf = fminf(gld(I.infil, k) * dt, d + rr);
+    if (I.flags & FLAG_INFIL) inf = fminf((pre ? pre->infil : gld(I.infil, k)) * dt, d + rr);
   }
   float4 ga = make_float4(0.f, 0.f, 0.f, 0.f), gi = make_float4(0.f, 0.f, 0.f, 0.f);
   float soilIn = 0.f;
@@ -473,9 +491,10 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, 
   }
   acc_inf += (double)inf * areaf;
   if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
-  const float hm = gld(I.hmax, k);
+  const float hm = pre ? pre->hmax : gld(I.hmax, k);
   if (qh > hm) gst(I.hmax, k, qh);
-  if (qh > hdry && !(gld(I.mask, k) & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
+  const uint8_t mk = pre ? pre->mask : gld(I.mask, k);
+  if (qh > hdry && !(mk & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
 }
 
 // Hazard products of a cell after a step of length dt ending at tnow (hazard, hzRecord,
@@ -547,6 +566,11 @@ template <int STAGE, bool LTS, bool HZ, bool EXT = false, bool IN = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
                                           Part *__restrict__ part, const int cells, const float tnow) {
   constexpr bool SF = EXT;
+  // the plain variants issue the pass's later loads with the row's (see below): the second stage of global
+  // stepping and the first of local time stepping (FS_PRE bit 0 and 1; bit 2, the LTS second stage, spills
+  // registers at four waves per SIMD)
+  constexpr bool PRE = !EXT && !HZ && ((!LTS && STAGE == 1 && (FS_PRE & 1)) || (LTS && STAGE == 0 && (FS_PRE & 2)) ||
+                                       (LTS && STAGE == 1 && (FS_PRE & 4)));
   const bool sf = EXT && (I.flags & FLAG_STEPF);
   const bool bar = EXT && I.crest != nullptr;
   const float rain = I.rec->rainRate;
@@ -597,6 +621,35 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     Prim p;
     if (IN || (xin && r >= 0 && r < ny)) p = load_prim_row<STAGE, LTS, EXT>(I, X, r, m, dt);
     else p = load_prim<STAGE, LTS, EXT>(I, X, r, m, dt);
+
+    // ---------------- the pass's other loads, issued here in the order it consumes them: theta's store of
+    // row r-2 (LTS), then the update's operands of row r-3 (its residual, RK start state and store), then its
+    // sources' layers. The memory counter drains in issue order, so loads issued at their sections would
+    // each wait on their own; issued together, one wait covers them and their latency overlaps the faces.
+    float tsh = 0.f, tsw = 0.f, tg0 = 0.f;
+    uint8_t tgf = 0;
+    float pw = 0.f, ph0 = 0.f, pw0 = 0.f, phu0 = 0.f, phv0 = 0.f, psh = 0.f, psw = 0.f, pshu = 0.f, pshv = 0.f;
+    SrcOps pso = {0.f, 0.f, 0.f, 1.f, (uint8_t)0};
+    if constexpr (PRE) {
+      if constexpr (LTS) {
+        if (in2 && M2r && (lv2 == m + 1 || (STAGE == 0 && lv2 == m))) {
+          const int k2 = (r - 2) * pitch + X;
+          tsh = gld(I.sh, k2);
+          tsw = gld(I.sw, k2);
+          if (STAGE == 1) { tgf = gld(I.gflag, k2); tg0 = gld(I.g0h, k2); }
+        }
+      }
+      const int ry = r - kHalo;
+      bool take = own_col && ry >= y0 && ry < y1 && in3 && M3;
+      if constexpr (LTS) take = take && lv3 == m;
+      if (take) {
+        const int k = ry * pitch + X;
+        pw = gld(Ws, k);
+        if (STAGE == 1) { ph0 = gld(I.h, k); pw0 = gld(I.w, k); phu0 = gld(I.hu, k); phv0 = gld(I.hv, k); }
+        if constexpr (LTS) { psh = gld(I.sh, k); psw = gld(I.sw, k); pshu = gld(I.shu, k); pshv = gld(I.shv, k); }
+        if (STAGE == 1) pso = load_src(I, k);
+      }
+    }
 
     // ---------------- x reconstruction of row r (neighbours by lane shifts)
     const float hl = from_left(p.h), ul = from_left(p.u), vlf = from_left(p.v), zl = from_left(p.z);
@@ -715,8 +768,9 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         // a waiting cell, or a stepping one in its first stage, holds its water plus its store
         if (take && (lv2 == m + 1 || STAGE == 0)) {
           const int k2 = (r - 2) * pitch + X;
-          float st = gld(I.sh, k2) - gld(I.sw, k2);
-          if (STAGE == 1 && gld(I.gflag, k2)) st += gld(I.g0h, k2);  // first-stage gather, not yet folded
+          float st = PRE ? tsh - tsw : gld(I.sh, k2) - gld(I.sw, k2);
+          // first-stage gather, not yet folded
+          if (STAGE == 1 && (PRE ? tgf : gld(I.gflag, k2))) st += PRE ? tg0 : gld(I.g0h, k2);
           hv_ = fmaxf(0.f, h2r + st);
         }
       }
@@ -787,20 +841,21 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             }
           }
         } else {
-          float qh = hh3, qhu = hu3, qhv = hv3, qw = gld(Ws, k);
+          float qh = hh3, qhu = hu3, qhv = hv3, qw = PRE ? pw : gld(Ws, k);
           // RK start state: U^n (the window's values in the first stage), plus (LTS) what finer
           // neighbours passed while the cell waited
           float h0 = 0.f, w0n = 0.f, hu0 = 0.f, hv0 = 0.f;
           bool spent = false;   // LTS: the cell's store held gathered flux (cleared once used)
           if (STAGE == 1 || LTS) {
             if (STAGE == 1) {
-              h0 = gld(I.h, k); w0n = gld(I.w, k); hu0 = gld(I.hu, k); hv0 = gld(I.hv, k);
+              if constexpr (PRE) { h0 = ph0; w0n = pw0; hu0 = phu0; hv0 = phv0; }
+              else { h0 = gld(I.h, k); w0n = gld(I.w, k); hu0 = gld(I.hu, k); hv0 = gld(I.hv, k); }
             } else {
               h0 = hh3; w0n = qw; hu0 = hu3; hv0 = hv3;
             }
             if constexpr (LTS) {
-              const float sh = gld(I.sh, k), sw = gld(I.sw, k);
-              const float shu = gld(I.shu, k), shv = gld(I.shv, k);
+              const float sh = PRE ? psh : gld(I.sh, k), sw = PRE ? psw : gld(I.sw, k);
+              const float shu = PRE ? pshu : gld(I.shu, k), shv = PRE ? pshv : gld(I.shv, k);
               hu0 += shu;
               hv0 += shv;
               kadd(h0, w0n, sh - sw);
@@ -834,7 +889,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             acc_out += o * ((double)dx * (double)dt * (double)wgt);
           }
           if (STAGE == 1) {
-            sources_cell<EXT, EXT || LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+            sources_cell<EXT, EXT || LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax, PRE ? &pso : nullptr);
             if constexpr (HZ) {
               if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
             }

@@ -297,6 +297,7 @@ DevInst devinst_of(const Slot &S, StepRec *rec, int wbeg, int wend) {
   DevInst d{};
   d.h = S.h; d.hu = S.hu; d.hv = S.hv; d.w = S.w;
   d.h1 = S.h1; d.hu1 = S.hu1; d.hv1 = S.hv1; d.w1 = S.w1;
+  d.ab = (int)(S.h1 - S.h);
   d.z = S.z; d.n = S.n; d.mask = S.mask; d.hmax = S.hmax; d.infil = S.infil; d.rf = S.rf;
   d.rec = rec;
   d.nx = S.nx; d.ny = S.ny; d.pitch = S.pitch;
@@ -935,8 +936,10 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
   if (e != hipSuccess) return fail(P, FS_E_HIP, "hipMalloc(%zu): %s", need, hipGetErrorString(e));
   char *q = S.mem;
   auto take = [&](size_t b) { char *r = q; q += align256(b); return r; };
-  S.h = (float *)take(N * 4); S.hu = (float *)take(N * 4); S.hv = (float *)take(N * 4); S.w = (float *)take(N * 4);
-  S.h1 = (float *)take(N * 4); S.hu1 = (float *)take(N * 4); S.hv1 = (float *)take(N * 4); S.w1 = (float *)take(N * 4);
+  // each variable's two states side by side (h, h1, hu, hu1, ...): a lane reads U^n or U* of a variable from one
+  // plane base at a per-lane offset (the LTS second stage mixes them within a row)
+  S.h = (float *)take(N * 4); S.h1 = (float *)take(N * 4); S.hu = (float *)take(N * 4); S.hu1 = (float *)take(N * 4);
+  S.hv = (float *)take(N * 4); S.hv1 = (float *)take(N * 4); S.w = (float *)take(N * 4); S.w1 = (float *)take(N * 4);
   S.z = (float *)take(N * 4); S.n = (float *)take(N * 4); S.hmax = (float *)take(N * 4);
   S.mask = (uint8_t *)take(N);
   if (infil) S.infil = (float *)take(N * 4);
@@ -1798,8 +1801,8 @@ struct SnapHeader {
   uint64_t bBytes, sBytes, rBytes;
   int32_t nbar, stageCells, hasSoil, hasGi, hasRf, hazard;
 };
-constexpr char kSnapMagic[8] = {'F', 'S', 'S', 'N', 'A', 'P', '0', '2'};
-constexpr uint32_t kSnapVersion = 2;
+constexpr char kSnapMagic[8] = {'F', 'S', 'S', 'N', 'A', 'P', '0', '3'};
+constexpr uint32_t kSnapVersion = 3;
 
 bool same_model(const SnapHeader &h, const Slot &S) {
   const fs_params &p = S.p;

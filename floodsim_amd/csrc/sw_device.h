@@ -100,6 +100,7 @@ struct Inlet {
 struct DevInst {
   float *h, *hu, *hv, *w;       // A: U^n, and U^{n+1} after the second stage (in place)
   float *h1, *hu1, *hv1, *w1;   // B: U* (first-stage result)
+  int ab;                       // cells from a variable's A plane to its B plane (h1 = h + ab, hu1 = hu + ab, ...)
   const float *z;               // bed, m, relative to the instance minimum
   const float *n;               // Manning n
   const uint8_t *mask;          // MASK_* bits

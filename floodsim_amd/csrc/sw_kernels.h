@@ -330,7 +330,11 @@ __device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, in
   if constexpr (LTS) p.lv = gld(I.lv, (Y >> 3) * I.tx + (X >> 3));
   bool useB = STAGE == 1;
   if constexpr (LTS) useB = STAGE == 1 && p.lv == m;
-  if (useB) { p.h = gld(I.h1, k); hu = gld(I.hu1, k); hv = gld(I.hv1, k); }
+  if constexpr (LTS && STAGE == 1) {
+    // a row mixes the stepping level's U* with the others' U^n: one load per variable at a per-lane offset
+    const int ka = useB ? k + I.ab : k;
+    p.h = gld(I.h, ka); hu = gld(I.hu, ka); hv = gld(I.hv, ka);
+  } else if (useB) { p.h = gld(I.h1, k); hu = gld(I.hu1, k); hv = gld(I.hv1, k); }
   else { p.h = gld(I.h, k); hu = gld(I.hu, k); hv = gld(I.hv, k); }
   p.hu = hu;
   p.hv = hv;
@@ -545,7 +549,9 @@ __device__ __forceinline__ int xcd_wave_index_n(int n, int nwaves_per_block) {
   return w < n ? w : -1;
 }
 
-struct FaceRow { float F0, F1, F2, cL, cR, on; };
+// a face's mass and momentum fluxes and the pressure corrections of its two sides (under local time stepping
+// zero for a face the record does not evaluate: its lower-level cell is not at the record's level)
+struct FaceRow { float F0, F1, F2, cL, cR; };
 
 // One strip of one RK stage (see the file comment for the row pipeline).
 // STAGE 0: U* = U^n + dt L(U^n)                   (updateAt STAGE 0, kernels.wgsl.js:489-492)
@@ -601,12 +607,12 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   bool M1r = false, M2r = false;
   bool R1r = false, R2r = false;   // EXT: Manning n > 0 in rows r-1, r-2
   float cy1 = -1e30f, wy1 = 0.f;   // EXT: crest and weir coefficient of row r-1's south faces
-  int lv1 = 0, lv2 = 0, lv3 = 0;
+  int lv1 = 0, lv2 = 0, lv3 = 0, lv4 = 0;
   float hu1 = 0, hv1 = 0, hu2 = 0, hv2 = 0, hu3 = 0, hv3 = 0, hh3 = 0;
   bool M3 = false, in1 = false, in2 = false, in3 = false;
   float yp_h = 0, yp_e = 0, yp_n = 0, yp_t = 0;
-  FaceRow xf1 = {0, 0, 0, 0, 0, 1}, xf2 = {0, 0, 0, 0, 0, 1}, xf3 = {0, 0, 0, 0, 0, 1};
-  FaceRow yf3 = {0, 0, 0, 0, 0, 1}, yf4 = {0, 0, 0, 0, 0, 1};
+  FaceRow xf1 = {0, 0, 0, 0, 0}, xf2 = {0, 0, 0, 0, 0}, xf3 = {0, 0, 0, 0, 0};
+  FaceRow yf3 = {0, 0, 0, 0, 0}, yf4 = {0, 0, 0, 0, 0};
   float th3 = 1.f, th4 = 1.f;
   // bed-slope terms of rows r-1..r-3 (x) and r-2..r-3 (y); under LTS a stepping cell's term already
   // holds, for a face whose neighbour is finer, that face's last evaluation
@@ -686,30 +692,32 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       if constexpr (EXT) {
         if (weirX) weir_flux(f, p.h + p.z, p.h, p.v, hr + zr, hr, vr, zcx, cwx, g, hdry, dx, dt);
       }
-      xf0 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
-      if constexpr (LTS) {
-        lvr = from_right_i(p.lv);
-        xf0.on = min(p.lv, lvr) == m ? 1.f : 0.f;
-      }
+      xf0 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR};
+    }
+    bool onE = true;   // LTS: the record evaluates this x face (l | l+1)
+    if constexpr (LTS) {
+      lvr = from_right_i(p.lv);
+      onE = min(p.lv, lvr) == m;
+      if (!onE) xf0 = FaceRow{0.f, 0.f, 0.f, 0.f, 0.f};
     }
     float sxr = sx0;
     if constexpr (LTS) {
       // a waiting cell keeps its reconstruction toward the faces this record evaluates (last stage)
-      const float onW = from_left(xf0.on);
+      const bool onW = from_left_i(onE ? 1 : 0) != 0;
       const bool row_in = own_col && p.in && p.M && r >= y0 && r < y1;
       if (last && row_in && p.lv == m + 1) {
         float2 *b = reinterpret_cast<float2 *>(I.bsx);
         const int kb = 2 * (r * pitch + X);
-        if (onW > 0.f) gst(b, kb, make_float2(rx0.x, rx0.y));
-        if (xf0.on > 0.f) gst(b, kb + 1, make_float2(rx0.z, rx0.w));
+        if (onW) gst(b, kb, make_float2(rx0.x, rx0.y));
+        if (onE) gst(b, kb + 1, make_float2(rx0.z, rx0.w));
       }
       // a stepping cell's bed slope from its face reconstructions (solver-cpu.js:363-368): a face with
       // a finer neighbour contributes the values of its last evaluation
-      if (row_in && p.lv == m && (onW == 0.f || xf0.on == 0.f)) {
+      if (row_in && p.lv == m && (!onW || !onE)) {
         const float4 b = gld(I.bsx, r * pitch + X);
         float4 xs = rx0;
-        if (onW == 0.f) { xs.x = b.x; xs.y = b.y; }
-        if (xf0.on == 0.f) { xs.z = b.z; xs.w = b.w; }
+        if (!onW) { xs.x = b.x; xs.y = b.y; }
+        if (!onE) { xs.z = b.z; xs.w = b.w; }
         sxr = hg * (xs.x + xs.z) * (xs.w - xs.y);
       }
     }
@@ -733,8 +741,12 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       if constexpr (EXT) {
         if (weirY) weir_flux(f, h2r + z2r, h2r, u2r, h1r + z1r, h1r, u1r, cy1, wy1, g, hdry, dx, dt);
       }
-      yf2 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR, 1.f};
-      if constexpr (LTS) yf2.on = min(lv2, lv1) == m ? 1.f : 0.f;
+      yf2 = FaceRow{f.F0, f.F1, f.F2, f.cL, f.cR};
+    }
+    bool onS = true;   // LTS: the record evaluates the y face (r-2 | r-1)
+    if constexpr (LTS) {
+      onS = min(lv2, lv1) == m;
+      if (!onS) yf2 = FaceRow{0.f, 0.f, 0.f, 0.f, 0.f};
     }
     float syr = sy1;
     if constexpr (LTS) {
@@ -744,13 +756,13 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       if (last && row_in && lv1 == m + 1) {
         float2 *b = reinterpret_cast<float2 *>(I.bsy);
         const int kb = 2 * (ry * pitch + X);
-        if (yf2.on > 0.f) gst(b, kb, make_float2(ry1.x, ry1.y));
+        if (onS) gst(b, kb, make_float2(ry1.x, ry1.y));
         if (onN) gst(b, kb + 1, make_float2(ry1.z, ry1.w));
       }
-      if (row_in && lv1 == m && (yf2.on == 0.f || !onN)) {
+      if (row_in && lv1 == m && (!onS || !onN)) {
         const float4 b = gld(I.bsy, ry * pitch + X);
         float4 ys = ry1;
-        if (yf2.on == 0.f) { ys.x = b.x; ys.y = b.y; }
+        if (!onS) { ys.x = b.x; ys.y = b.y; }
         if (!onN) { ys.z = b.z; ys.w = b.w; }
         syr = hg * (ys.x + ys.z) * (ys.w - ys.y);
       }
@@ -759,11 +771,10 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     // ---------------- theta of row r-2 (solver-cpu.js:307-318; kernels.wgsl.js:349-363)
     float th2 = 1.f;
     {
-      float fe = xf2.F0, fw = from_left(xf2.F0 * xf2.on), fs = yf3.F0, fn = yf2.F0;
+      const float fe = xf2.F0, fw = from_left(xf2.F0), fs = yf3.F0, fn = yf2.F0;
       bool take = in2 && M2r;
       float hv_ = h2r;
       if constexpr (LTS) {
-        fe *= xf2.on; fs *= yf3.on; fn *= yf2.on;
         take = take && (lv2 == m || lv2 == m + 1);
         // a waiting cell, or a stepping one in its first stage, holds its water plus its store
         if (take && (lv2 == m + 1 || STAGE == 0)) {
@@ -785,8 +796,6 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       // west face (l-1 | l) from lane l-1; east face (l | l+1) own
       const float Fw0 = from_left(xf3.F0), Fw1 = from_left(xf3.F1), Fw2 = from_left(xf3.F2), cRw = from_left(xf3.cR);
       const float thW = from_left(th3), thE = from_right(th3);
-      float kw = 1.f, ke = 1.f, ks = 1.f, kn = 1.f;
-      if constexpr (LTS) { kw = from_left(xf3.on); ke = xf3.on; ks = yf4.on; kn = yf3.on; }
       float Fe0 = xf3.F0, Fe1 = xf3.F1, Fe2 = xf3.F2;
       const float cLe = xf3.cL;
       float Fs0 = yf4.F0, Fs1 = yf4.F1, Fs2 = yf4.F2;
@@ -813,6 +822,9 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       }
       bool upd = own_col && ry >= y0 && ry < y1 && in3 && M3;
       bool own = true, coarse = false;
+      // LTS: a waiting cell gathers when one of its faces is evaluated, i.e. a face neighbour is at m
+      bool gathers = false;
+      if constexpr (LTS) gathers = from_left_i(lv3) == m || from_right_i(lv3) == m || lv2 == m || lv4 == m;
       if constexpr (LTS) {
         own = lv3 == m;
         coarse = lv3 == m + 1;
@@ -824,14 +836,14 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         if constexpr (LTS) {
           if (!own) { bx = 0.f; by = 0.f; }   // a waiting cell gathers fluxes only
         }
-        const float dh = -inv_dx * (ke * Fe0 - kw * w0 + kn * Fn0 - ks * Fs0);
-        const float dhu = -inv_dx * (ke * (Fe1 + cLe) - kw * (w1 + cRw) + bx + kn * Fn2 - ks * Fs2);
-        const float dhv = -inv_dx * (kn * (Fn1 + cLn) - ks * (Fs1 + cRs) + by + ke * Fe2 - kw * w2);
+        const float dh = -inv_dx * (Fe0 - w0 + Fn0 - Fs0);
+        const float dhu = -inv_dx * ((Fe1 + cLe) - (w1 + cRw) + bx + Fn2 - Fs2);
+        const float dhv = -inv_dx * ((Fn1 + cLn) - (Fs1 + cRs) + by + Fe2 - w2);
         if (LTS && coarse) {
           // waiting cell: gather the RK-weighted change (ltsUpdate mode 2, kernels.wgsl.js:515-519)
           // (into per-stage deltas: a neighbouring strip may read this cell's store for theta
           // during this kernel; sw_lts_fold adds the deltas once the record's stages are done)
-          if (kw + ke + ks + kn > 0.f) {
+          if (gathers) {
             const float wd = so ? 0.5f * dt : dt;
             if (STAGE == 0) {
               gst(I.g0h, k, wd * dh); gst(I.g0hu, k, wd * dhu); gst(I.g0hv, k, wd * dhv);
@@ -911,7 +923,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     xf3 = xf2; xf2 = xf1; xf1 = xf0;
     sx3 = sx2; sx2 = sx1; sx1 = sxr;
     sy3 = sy2; sy2 = syr;
-    if constexpr (LTS) { lv3 = lv2; lv2 = lv1; lv1 = p.lv; }
+    if constexpr (LTS) { lv4 = lv3; lv3 = lv2; lv2 = lv1; lv1 = p.lv; }
     yp_h = yh_p; yp_e = ye_p; yp_n = yn_p; yp_t = yt_p;
     hh3 = h2r; hu3 = hu2; hv3 = hv2; M3 = M2r; in3 = in2;
     h2r = h1r; u2r = u1r; v2r = v1r; z2r = z1r; M2r = M1r; hu2 = hu1; hv2 = hv1; in2 = in1;
@@ -1221,26 +1233,41 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
     const int4 wk = lts_rows<true>(I, work[I.wbeg + I.ulist[m * nu + i / split]], m, lane, split, i % split);
     const int X = wk.y - I.hl + lane;
     if (lane < I.hl || lane >= I.hl + I.own || X >= I.nx) return;
-    // the gather flags of 8 rows (one tile row: parts are whole tile rows) are loaded together, so
-    // the walk waits for memory once per tile row rather than once per row
+    // per tile row (8 rows): the gather flags, then every flagged cell's store and gathers, all issued before
+    // any is used, so the walk waits for memory twice per tile row rather than once per flagged row
     const int yend = min(wk.z + wk.w, I.ny);
     for (int yb = wk.z; yb < yend; yb += kLtsTile) {
       uint8_t f[kLtsTile];
+      // (rows past the walk load the last row's flag, then drop it: the loads need no branch)
 #pragma unroll
-      for (int q = 0; q < kLtsTile; ++q) f[q] = yb + q < yend ? gld(I.gflag, (yb + q) * I.pitch + X) : (uint8_t)0;
+      for (int q = 0; q < kLtsTile; ++q) f[q] = gld(I.gflag, min(yb + q, yend - 1) * I.pitch + X);
+#pragma unroll
+      for (int q = 0; q < kLtsTile; ++q) f[q] = yb + q < yend ? f[q] : (uint8_t)0;
+      float sh[kLtsTile], sw[kLtsTile], shu[kLtsTile], shv[kLtsTile], a[kLtsTile], au[kLtsTile], av[kLtsTile];
+      float b[kLtsTile], bu[kLtsTile], bv[kLtsTile];
+#pragma unroll
+      for (int q = 0; q < kLtsTile; ++q) {
+        sh[q] = sw[q] = shu[q] = shv[q] = a[q] = au[q] = av[q] = b[q] = bu[q] = bv[q] = 0.f;
+        if (f[q]) {
+          const int k = (yb + q) * I.pitch + X;
+          sh[q] = gld(I.sh, k); sw[q] = gld(I.sw, k); shu[q] = gld(I.shu, k); shv[q] = gld(I.shv, k);
+          a[q] = gld(I.g0h, k); au[q] = gld(I.g0hu, k); av[q] = gld(I.g0hv, k);
+          if (two) { b[q] = gld(I.g1h, k); bu[q] = gld(I.g1hu, k); bv[q] = gld(I.g1hv, k); }
+        }
+      }
 #pragma unroll
       for (int q = 0; q < kLtsTile; ++q) {
         if (!f[q]) continue;
         const int k = (yb + q) * I.pitch + X;
-        float sh = gld(I.sh, k), sw = gld(I.sw, k);
-        kadd(sh, sw, gld(I.g0h, k));
-        float shu = gld(I.shu, k) + gld(I.g0hu, k), shv = gld(I.shv, k) + gld(I.g0hv, k);
+        float h = sh[q], w = sw[q];
+        kadd(h, w, a[q]);
+        float hu = shu[q] + au[q], hv = shv[q] + av[q];
         if (two) {
-          kadd(sh, sw, gld(I.g1h, k));
-          shu += gld(I.g1hu, k);
-          shv += gld(I.g1hv, k);
+          kadd(h, w, b[q]);
+          hu += bu[q];
+          hv += bv[q];
         }
-        gst(I.sh, k, sh); gst(I.sw, k, sw); gst(I.shu, k, shu); gst(I.shv, k, shv);
+        gst(I.sh, k, h); gst(I.sw, k, w); gst(I.shu, k, hu); gst(I.shv, k, hv);
         gst(I.gflag, k, (uint8_t)0);
       }
     }
@@ -1436,7 +1463,7 @@ __global__ __launch_bounds__(1024) void sw_lts_record_end(const DevInst *__restr
   __shared__ double s_o[16], s_c[16], s_f[16];
   double o = 0.0, c = 0.0, f = 0.0;
   // unrolled so that several partials are in flight per thread (the sums keep their order)
-#pragma unroll 4
+#pragma unroll 16
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     const Part a = p1[k];
     o += a.out;
@@ -1630,6 +1657,7 @@ __global__ __launch_bounds__(1024) void sw_dt_final(const DevInst *__restrict__ 
   int cells = 0, strips = 0;
   const bool fresh = mode == 0 || rec->dirty;
   if (fresh) {
+#pragma unroll 8
     for (int k = I.wbeg + t; k < I.wend; k += blockDim.x) {
       const Part b = p2[k];
       m = fmaxf(m, b.smax);

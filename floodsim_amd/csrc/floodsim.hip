@@ -1942,6 +1942,16 @@ int fs_read_hazard(fs_handle s, float *dv, float *force, float *duration, float 
 size_t fs_snapshot_bytes(fs_handle s) { return s ? fs_pool_snapshot_bytes(s->pool, s->slot) : 0; }
 int fs_snapshot(fs_handle s, void *out, size_t bytes) { return s ? fs_pool_snapshot(s->pool, s->slot, out, bytes) : FS_E_ARG; }
 int fs_restore(fs_handle s, const void *in, size_t bytes) { return s ? fs_pool_restore(s->pool, s->slot, in, bytes) : FS_E_ARG; }
+int fs_snapshot_layout(fs_handle s, uint64_t bytes[FS_SNAP_PARTS]) {
+  if (!s || !bytes) return FS_E_ARG;
+  const Slot &S = s->pool->slots[s->slot];
+  bytes[FS_SNAP_RECORD] = sizeof(SnapHeader);
+  bytes[FS_SNAP_PLANES] = S.bytes;
+  bytes[FS_SNAP_BARRIERS] = S.bbytes;
+  bytes[FS_SNAP_SOIL] = S.sbytes;
+  bytes[FS_SNAP_RAINFIELD] = S.rbytes;
+  return FS_OK;
+}
 
 int fs_read_state(fs_handle s, double *h, double *hu, double *hv, double *hmax, fs_mass *mass) {
   return s ? fs_pool_read_state(s->pool, s->slot, h, hu, hv, hmax, mass) : FS_E_ARG;

@@ -156,6 +156,23 @@ static napi_value clock_object(napi_env env, const fs_clock *c) {
   return o;
 }
 
+static napi_value mass_object(napi_env env, const fs_mass *ms) {
+  napi_value m;
+  napi_create_object(env, &m);
+  set_num(env, m, "volume", ms->volume);
+  set_num(env, m, "rain", ms->rain);
+  set_num(env, m, "infil", ms->infil);
+  set_num(env, m, "giStored", ms->giStored);
+  set_num(env, m, "outflow", ms->outflow);
+  set_num(env, m, "exchanged", ms->exchanged);
+  set_num(env, m, "clipped", ms->clipped);
+  set_num(env, m, "expected", ms->expected);
+  set_num(env, m, "rel", ms->rel);
+  set_num(env, m, "relExClip", ms->relExClip);
+  set_num(env, m, "initial", ms->initial);
+  return m;
+}
+
 /* ---------- create / destroy ---------- */
 
 static void finalize_solver(napi_env env, void *data, void *hint) {
@@ -164,6 +181,61 @@ static void finalize_solver(napi_env env, void *data, void *hint) {
   Solver *s = (Solver *)data;
   if (s->h && !s->busy) fs_destroy(s->h);
   free(s);
+}
+
+/* fs_params from the options object the JS classes build (defaults of solver-gpu.js:39-47) */
+static int parse_params(napi_env env, napi_value o, fs_params *p) {
+  memset(p, 0, sizeof *p);
+  p->nx = (int32_t)num_prop(env, o, "nx", 0);
+  p->ny = (int32_t)num_prop(env, o, "ny", 0);
+  p->dx = num_prop(env, o, "dx", 1.0);
+  p->g = num_prop(env, o, "g", 9.81);
+  p->secondOrder = (int32_t)num_prop(env, o, "secondOrder", 1);
+  p->cfl = num_prop(env, o, "cfl", p->secondOrder ? 0.48 : 0.9);
+  p->hdry = num_prop(env, o, "hdry", 1e-5);
+  p->hfront = num_prop(env, o, "hfront", 1e-3);
+  p->dtDry = num_prop(env, o, "dtDry", 1.0);
+  p->limiter = (int32_t)num_prop(env, o, "limiter", 0);
+  p->bc[0] = (int32_t)num_prop(env, o, "bcWest", 0);
+  p->bc[1] = (int32_t)num_prop(env, o, "bcEast", 0);
+  p->bc[2] = (int32_t)num_prop(env, o, "bcSouth", 0);
+  p->bc[3] = (int32_t)num_prop(env, o, "bcNorth", 0);
+  p->lts = (int32_t)num_prop(env, o, "lts", 0);
+  p->rows = (int32_t)num_prop(env, o, "rows", 0);
+  p->untiled = (int32_t)num_prop(env, o, "untiled", 0);
+  p->hazard = (int32_t)num_prop(env, o, "hazard", 0);
+  p->implicitFriction = (int32_t)num_prop(env, o, "implicitFriction", 0);
+  p->stepFlux = (int32_t)num_prop(env, o, "stepFlux", 0);
+  if (p->nx <= 0 || p->ny <= 0) {
+    napi_throw_range_error(env, NULL, "create: nx and ny must be positive");
+    return -1;
+  }
+  return 0;
+}
+
+/* the model layers of fs_create / fs_pool_add: a[0..6] = z, manning|null, manningScalar, mask|null,
+ * infil|null, rainField|null, inlets|null */
+typedef struct Layers {
+  void *z, *man, *mask, *infil, *rf, *inl;
+  double ms;
+  size_t nin;
+} Layers;
+
+static int parse_layers(napi_env env, const napi_value *a, const fs_params *p, Layers *L) {
+  const long n = (long)p->nx * p->ny;
+  memset(L, 0, sizeof *L);
+  if (is_nullish(env, a[0])) {
+    napi_throw_type_error(env, NULL, "create: z is required");
+    return -1;
+  }
+  if (typed(env, a[0], napi_float64_array, n, &L->z, NULL, "z")) return -1;
+  if (typed(env, a[1], napi_float64_array, n, &L->man, NULL, "manning")) return -1;
+  napi_get_value_double(env, a[2], &L->ms);
+  if (typed(env, a[3], napi_uint8_array, n, &L->mask, NULL, "mask")) return -1;
+  if (typed(env, a[4], napi_float64_array, n, &L->infil, NULL, "infil")) return -1;
+  if (typed(env, a[5], napi_float64_array, n, &L->rf, NULL, "rainField")) return -1;
+  if (typed(env, a[6], napi_int32_array, -1, &L->inl, &L->nin, "inlets")) return -1;
+  return 0;
 }
 
 /* create(params, z, manning|null, manningScalar, mask|null, infil|null, rainField|null, inlets|null) */
@@ -176,55 +248,22 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     return NULL;
   }
   fs_params p;
-  memset(&p, 0, sizeof p);
-  p.nx = (int32_t)num_prop(env, a[0], "nx", 0);
-  p.ny = (int32_t)num_prop(env, a[0], "ny", 0);
-  p.dx = num_prop(env, a[0], "dx", 1.0);
-  p.g = num_prop(env, a[0], "g", 9.81);
-  p.secondOrder = (int32_t)num_prop(env, a[0], "secondOrder", 1);
-  p.cfl = num_prop(env, a[0], "cfl", p.secondOrder ? 0.48 : 0.9);
-  p.hdry = num_prop(env, a[0], "hdry", 1e-5);
-  p.hfront = num_prop(env, a[0], "hfront", 1e-3);
-  p.dtDry = num_prop(env, a[0], "dtDry", 1.0);
-  p.limiter = (int32_t)num_prop(env, a[0], "limiter", 0);
-  p.bc[0] = (int32_t)num_prop(env, a[0], "bcWest", 0);
-  p.bc[1] = (int32_t)num_prop(env, a[0], "bcEast", 0);
-  p.bc[2] = (int32_t)num_prop(env, a[0], "bcSouth", 0);
-  p.bc[3] = (int32_t)num_prop(env, a[0], "bcNorth", 0);
-  p.lts = (int32_t)num_prop(env, a[0], "lts", 0);
-  p.rows = (int32_t)num_prop(env, a[0], "rows", 0);
-  p.untiled = (int32_t)num_prop(env, a[0], "untiled", 0);
-  p.hazard = (int32_t)num_prop(env, a[0], "hazard", 0);
-  p.implicitFriction = (int32_t)num_prop(env, a[0], "implicitFriction", 0);
-  p.stepFlux = (int32_t)num_prop(env, a[0], "stepFlux", 0);
-  if (p.nx <= 0 || p.ny <= 0) {
-    napi_throw_range_error(env, NULL, "create: nx and ny must be positive");
-    return NULL;
-  }
-  const long n = (long)p.nx * p.ny;
-  void *z, *man, *mask, *infil, *rf, *inl;
-  size_t nin = 0;
-  double ms = 0;
-  if (is_nullish(env, a[1])) {
-    napi_throw_type_error(env, NULL, "create: z is required");
-    return NULL;
-  }
-  if (typed(env, a[1], napi_float64_array, n, &z, NULL, "z")) return NULL;
-  if (typed(env, a[2], napi_float64_array, n, &man, NULL, "manning")) return NULL;
-  napi_get_value_double(env, a[3], &ms);
-  if (typed(env, a[4], napi_uint8_array, n, &mask, NULL, "mask")) return NULL;
-  if (typed(env, a[5], napi_float64_array, n, &infil, NULL, "infil")) return NULL;
-  if (typed(env, a[6], napi_float64_array, n, &rf, NULL, "rainField")) return NULL;
-  if (typed(env, a[7], napi_int32_array, -1, &inl, &nin, "inlets")) return NULL;
+  Layers L;
+  if (parse_params(env, a[0], &p) || parse_layers(env, a + 1, &p, &L)) return NULL;
   fs_handle h = NULL;
-  int rc = fs_create(&p, (const double *)z, (const double *)man, ms, (const uint8_t *)mask,
-                     (const double *)infil, (const double *)rf, (const int32_t *)inl, (int32_t)nin, &h);
+  int rc = fs_create(&p, (const double *)L.z, (const double *)L.man, L.ms, (const uint8_t *)L.mask,
+                     (const double *)L.infil, (const double *)L.rf, (const int32_t *)L.inl, (int32_t)L.nin, &h);
   if (rc) return fail(env, NULL, rc);
   Solver *s = (Solver *)calloc(1, sizeof(Solver));
+  if (!s) {
+    fs_destroy(h);
+    napi_throw_error(env, NULL, "create: out of host memory");
+    return NULL;
+  }
   s->h = h;
   s->nx = p.nx;
   s->ny = p.ny;
-  s->nInlets = (int32_t)nin;
+  s->nInlets = (int32_t)L.nin;
   napi_value ext;
   if (napi_create_external(env, s, finalize_solver, NULL, &ext) != napi_ok) {
     fs_destroy(h);
@@ -602,20 +641,7 @@ static void work_complete(napi_env env, napi_status status, void *data) {
         napi_create_object(env, &result);
         static const char *names[4] = {"h", "hu", "hv", "hmax"};
         for (int k = 0; k < 4; ++k) napi_set_named_property(env, result, names[k], make_f64(env, w->f[k], n));
-        napi_value m;
-        napi_create_object(env, &m);
-        set_num(env, m, "volume", w->mass.volume);
-        set_num(env, m, "rain", w->mass.rain);
-        set_num(env, m, "infil", w->mass.infil);
-        set_num(env, m, "giStored", w->mass.giStored);
-        set_num(env, m, "outflow", w->mass.outflow);
-        set_num(env, m, "exchanged", w->mass.exchanged);
-        set_num(env, m, "clipped", w->mass.clipped);
-        set_num(env, m, "expected", w->mass.expected);
-        set_num(env, m, "rel", w->mass.rel);
-        set_num(env, m, "relExClip", w->mass.relExClip);
-        set_num(env, m, "initial", w->mass.initial);
-        napi_set_named_property(env, result, "mass", m);
+        napi_set_named_property(env, result, "mass", mass_object(env, &w->mass));
         napi_set_named_property(env, result, "clock", clock_object(env, &w->clk));
         break;
       }
@@ -748,6 +774,26 @@ static napi_value queue_work_blob(napi_env env, Solver *s, int kind, void *blob,
   return promise;
 }
 
+/* snapshotLayout(h) -> [record, planes, barriers, soil, rainField] bytes (fs_snapshot_layout) */
+static napi_value js_snapshot_layout(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  Solver *s = get_solver(env, a[0]);
+  if (!s) return NULL;
+  uint64_t b[FS_SNAP_PARTS];
+  int rc = fs_snapshot_layout(s->h, b);
+  if (rc) return fail(env, s->h, rc);
+  napi_value arr;
+  napi_create_array_with_length(env, FS_SNAP_PARTS, &arr);
+  for (uint32_t k = 0; k < FS_SNAP_PARTS; ++k) {
+    napi_value x;
+    napi_create_double(env, (double)b[k], &x);
+    napi_set_element(env, arr, k, x);
+  }
+  return arr;
+}
+
 /* snapshot(h) -> Promise<Uint8Array> */
 static napi_value js_snapshot(napi_env env, napi_callback_info info) {
   size_t argc = 1;
@@ -840,6 +886,537 @@ static napi_value js_read_inlets(napi_env env, napi_callback_info info) {
   return queue_work(env, s, W_READ_INLETS, 0, 0, 0);
 }
 
+/* ---------- pools: batched / served instances (fs_pool_*) ----------
+ *
+ * A pool handle holds many instances (slots) advanced together by one launch sequence
+ * (floodsim_amd/js/pool-hip.js builds Pool and Server on these). No counterpart in the reference,
+ * which runs one solve at a time (src/batch.js:1-4). As with a solver handle, advance, step and
+ * readState run on libuv's thread pool and return Promises, and a pool must not be driven by two calls
+ * at once: the JS class serialises them and the addon rejects a call that finds the pool busy.
+ */
+
+typedef struct PoolH {
+  fs_pool_t p;
+  int busy;
+  int32_t cap;       /* slots described by dims */
+  int32_t *dims;     /* per slot: nx, ny (0, 0: no instance) */
+} PoolH;
+
+static void finalize_pool(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  PoolH *q = (PoolH *)data;
+  if (q->p && !q->busy) fs_pool_destroy(q->p);
+  free(q->dims);
+  free(q);
+}
+
+static PoolH *get_pool(napi_env env, napi_value v) {
+  PoolH *q = NULL;
+  if (napi_get_value_external(env, v, (void **)&q) != napi_ok || !q) {
+    napi_throw_type_error(env, NULL, "floodsim: not a pool handle");
+    return NULL;
+  }
+  if (!q->p) {
+    napi_throw_error(env, NULL, "floodsim: pool was destroyed");
+    return NULL;
+  }
+  if (q->busy) {
+    napi_throw_error(env, NULL, "floodsim: pool is busy (calls on one pool must not overlap)");
+    return NULL;
+  }
+  return q;
+}
+
+/* the slot argument, checked against the instances the pool holds */
+static int slot_arg(napi_env env, const PoolH *q, napi_value v, int32_t *slot) {
+  int32_t k = -1;
+  napi_get_value_int32(env, v, &k);
+  if (k < 0 || k >= q->cap || q->dims[2 * k] == 0) {
+    napi_throw_range_error(env, NULL, "floodsim: no instance in that pool slot");
+    return -1;
+  }
+  *slot = k;
+  return 0;
+}
+
+static napi_value pool_fail(napi_env env, const PoolH *q, int rc) {
+  char m[600];
+  const char *e = fs_pool_last_error(q ? q->p : NULL);
+  snprintf(m, sizeof m, "floodsim_hip pool error %d: %s", rc, e ? e : "");
+  napi_throw_error(env, NULL, m);
+  return NULL;
+}
+
+#define POOL_ARGS(N)                                                     \
+  size_t argc = (N);                                                     \
+  napi_value a[(N) > 0 ? (N) : 1];                                       \
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));       \
+  PoolH *q = get_pool(env, a[0]);                                        \
+  if (!q) return NULL;
+
+#define POOL_SLOT_ARGS(N)                                                \
+  POOL_ARGS(N)                                                           \
+  int32_t slot = -1;                                                     \
+  if (slot_arg(env, q, a[1], &slot)) return NULL;
+
+#define POOL_RESULT(rc) ((rc) ? pool_fail(env, q, (rc)) : undefined_value(env))
+
+/* poolCreate(hbmBudgetBytes) -> pool handle (0 = the device's free memory) */
+static napi_value js_pool_create(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  double budget = 0;
+  if (argc > 0 && !is_nullish(env, a[0])) napi_get_value_double(env, a[0], &budget);
+  if (!(budget >= 0)) {
+    napi_throw_range_error(env, NULL, "poolCreate: the budget must be >= 0 bytes");
+    return NULL;
+  }
+  fs_pool_t p = NULL;
+  int rc = fs_pool_create((size_t)budget, &p);
+  if (rc) {
+    char m[600];
+    const char *e = fs_pool_last_error(NULL);
+    snprintf(m, sizeof m, "floodsim_hip pool error %d: %s", rc, e ? e : "");
+    napi_throw_error(env, NULL, m);
+    return NULL;
+  }
+  PoolH *q = (PoolH *)calloc(1, sizeof(PoolH));
+  if (!q) {
+    fs_pool_destroy(p);
+    napi_throw_error(env, NULL, "poolCreate: out of host memory");
+    return NULL;
+  }
+  q->p = p;
+  napi_value ext;
+  if (napi_create_external(env, q, finalize_pool, NULL, &ext) != napi_ok) {
+    fs_pool_destroy(p);
+    free(q);
+    napi_throw_error(env, NULL, "poolCreate: napi_create_external failed");
+    return NULL;
+  }
+  return ext;
+}
+
+static napi_value js_pool_destroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  PoolH *q = NULL;
+  if (napi_get_value_external(env, a[0], (void **)&q) != napi_ok || !q) {
+    napi_throw_type_error(env, NULL, "poolDestroy: not a pool handle");
+    return NULL;
+  }
+  if (q->busy) {
+    napi_throw_error(env, NULL, "poolDestroy: pool is busy");
+    return NULL;
+  }
+  if (q->p) fs_pool_destroy(q->p);
+  q->p = NULL;
+  return undefined_value(env);
+}
+
+/* poolBytesNeeded(params, withInfil, withRainField) -> bytes one instance takes */
+static napi_value js_pool_bytes_needed(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(env, napi_get_cb_info(env, info, &argc, a, NULL, NULL));
+  fs_params p;
+  if (parse_params(env, a[0], &p)) return NULL;
+  bool infil = false, rf = false;
+  if (argc > 1) napi_get_value_bool(env, a[1], &infil);
+  if (argc > 2) napi_get_value_bool(env, a[2], &rf);
+  napi_value v;
+  napi_create_double(env, (double)fs_pool_bytes_needed(&p, infil ? 1 : 0, rf ? 1 : 0), &v);
+  return v;
+}
+
+static napi_value js_pool_bytes_used(napi_env env, napi_callback_info info) {
+  POOL_ARGS(1)
+  napi_value v;
+  napi_create_double(env, (double)fs_pool_bytes_used(q->p), &v);
+  return v;
+}
+
+/* poolAdd(p, params, z, manning|null, manningScalar, mask|null, infil|null, rainField|null, inlets|null)
+ * -> slot, or -1 when the pool's HBM budget has no room (FS_E_BUDGET) */
+static napi_value js_pool_add(napi_env env, napi_callback_info info) {
+  POOL_ARGS(9)
+  if (argc < 9) {
+    napi_throw_type_error(env, NULL, "poolAdd: 9 arguments expected");
+    return NULL;
+  }
+  fs_params p;
+  Layers L;
+  if (parse_params(env, a[1], &p) || parse_layers(env, a + 2, &p, &L)) return NULL;
+  int32_t slot = -1;
+  int rc = fs_pool_add(q->p, &p, (const double *)L.z, (const double *)L.man, L.ms, (const uint8_t *)L.mask,
+                       (const double *)L.infil, (const double *)L.rf, (const int32_t *)L.inl, (int32_t)L.nin, &slot);
+  napi_value v;
+  if (rc == FS_E_BUDGET) {
+    napi_create_int32(env, -1, &v);
+    return v;
+  }
+  if (rc) return pool_fail(env, q, rc);
+  if (slot >= q->cap) {
+    const int32_t cap = slot + 1 > 2 * q->cap ? slot + 1 : 2 * q->cap;
+    int32_t *d = (int32_t *)realloc(q->dims, (size_t)cap * 2 * sizeof(int32_t));
+    if (!d) {
+      fs_pool_remove(q->p, slot);
+      napi_throw_error(env, NULL, "poolAdd: out of host memory");
+      return NULL;
+    }
+    memset(d + 2 * q->cap, 0, (size_t)(cap - q->cap) * 2 * sizeof(int32_t));
+    q->dims = d;
+    q->cap = cap;
+  }
+  q->dims[2 * slot] = p.nx;
+  q->dims[2 * slot + 1] = p.ny;
+  napi_create_int32(env, slot, &v);
+  return v;
+}
+
+static napi_value js_pool_remove(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(2)
+  int rc = fs_pool_remove(q->p, slot);
+  if (!rc) q->dims[2 * slot] = q->dims[2 * slot + 1] = 0;
+  return POOL_RESULT(rc);
+}
+
+static napi_value js_pool_set_rows(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(3)
+  int32_t rows = 0;
+  napi_get_value_int32(env, a[2], &rows);
+  return POOL_RESULT(fs_pool_set_rows(q->p, slot, rows));
+}
+
+/* poolSetState(p, slot, h, hu|null, hv|null, bookInitial) */
+static napi_value js_pool_set_state(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(6)
+  const long n = (long)q->dims[2 * slot] * q->dims[2 * slot + 1];
+  void *h, *hu, *hv;
+  if (typed(env, a[2], napi_float64_array, n, &h, NULL, "h")) return NULL;
+  if (typed(env, a[3], napi_float64_array, n, &hu, NULL, "hu")) return NULL;
+  if (typed(env, a[4], napi_float64_array, n, &hv, NULL, "hv")) return NULL;
+  bool book = true;
+  if (argc > 5) napi_get_value_bool(env, a[5], &book);
+  return POOL_RESULT(fs_pool_set_state(q->p, slot, (const double *)h, (const double *)hu, (const double *)hv, book ? 1 : 0));
+}
+
+static napi_value js_pool_set_rain(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(3)
+  double r = 0;
+  napi_get_value_double(env, a[2], &r);
+  return POOL_RESULT(fs_pool_set_rain(q->p, slot, r));
+}
+
+static napi_value js_pool_set_inlet_rates(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(3)
+  void *r;
+  size_t n = 0;
+  if (typed(env, a[2], napi_float64_array, -1, &r, &n, "rates")) return NULL;
+  return POOL_RESULT(fs_pool_set_inlet_rates(q->p, slot, (const double *)r, (int32_t)n));
+}
+
+static napi_value js_pool_set_stage_cells(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(3)
+  void *c;
+  size_t n = 0;
+  if (typed(env, a[2], napi_int32_array, -1, &c, &n, "stageCells")) return NULL;
+  return POOL_RESULT(fs_pool_set_stage_cells(q->p, slot, (const int32_t *)c, (int32_t)n));
+}
+
+static napi_value js_pool_set_stage(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(3)
+  double z = 0;
+  napi_get_value_double(env, a[2], &z);
+  return POOL_RESULT(fs_pool_set_stage(q->p, slot, z));
+}
+
+/* poolSetCrests(p, slot, Int32Array i, j, axis, Float64Array z, C, ground) */
+static napi_value js_pool_set_crests(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(8)
+  void *fi, *fj, *ax, *z, *c, *g;
+  size_t n = 0, n2 = 0;
+  if (typed(env, a[2], napi_int32_array, -1, &fi, &n, "crest i")) return NULL;
+  if (typed(env, a[3], napi_int32_array, (long)n, &fj, &n2, "crest j")) return NULL;
+  if (typed(env, a[4], napi_int32_array, (long)n, &ax, &n2, "crest axis")) return NULL;
+  if (typed(env, a[5], napi_float64_array, (long)n, &z, &n2, "crest z")) return NULL;
+  if (typed(env, a[6], napi_float64_array, (long)n, &c, &n2, "crest C")) return NULL;
+  if (typed(env, a[7], napi_float64_array, (long)n, &g, &n2, "crest ground")) return NULL;
+  return POOL_RESULT(fs_pool_set_crests(q->p, slot, (const int32_t *)fi, (const int32_t *)fj, (const int32_t *)ax,
+                                        (const double *)z, (const double *)c, (const double *)g, (int32_t)n));
+}
+
+static napi_value js_pool_set_crest_levels(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(3)
+  void *z;
+  size_t n = 0;
+  if (typed(env, a[2], napi_float64_array, -1, &z, &n, "crest levels")) return NULL;
+  return POOL_RESULT(fs_pool_set_crest_levels(q->p, slot, (const double *)z));
+}
+
+/* poolReadBarriers(p, slot, nBarriers) -> Float64Array(9 n) */
+static napi_value js_pool_read_barriers(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(3)
+  int32_t n = 0;
+  napi_get_value_int32(env, a[2], &n);
+  if (n < 0) n = 0;
+  double *out = (double *)calloc((size_t)(n ? 9 * n : 1), sizeof(double));
+  if (!out) {
+    napi_throw_error(env, NULL, "poolReadBarriers: out of memory");
+    return NULL;
+  }
+  int rc = fs_pool_read_barriers(q->p, slot, out);
+  napi_value r = rc ? NULL : make_f64(env, out, (size_t)9 * n);
+  free(out);
+  return rc ? pool_fail(env, q, rc) : r;
+}
+
+/* poolSetGa(p, slot, psiD, fmax, perv) / poolSetGi(p, slot, rate, cap, drain): Float64Array(nx*ny) each */
+static napi_value pool_set_layers3(napi_env env, napi_callback_info info, int gi) {
+  POOL_SLOT_ARGS(5)
+  const long n = (long)q->dims[2 * slot] * q->dims[2 * slot + 1];
+  void *v[3];
+  static const char *nm[2][3] = {{"ga.psiD", "ga.fmax", "ga.perv"}, {"gi.rate", "gi.cap", "gi.drain"}};
+  for (int k = 0; k < 3; ++k)
+    if (typed(env, a[2 + k], napi_float64_array, n, &v[k], NULL, nm[gi][k])) return NULL;
+  int rc = gi ? fs_pool_set_gi(q->p, slot, (const double *)v[0], (const double *)v[1], (const double *)v[2])
+              : fs_pool_set_ga(q->p, slot, (const double *)v[0], (const double *)v[1], (const double *)v[2]);
+  return POOL_RESULT(rc);
+}
+static napi_value js_pool_set_ga(napi_env env, napi_callback_info info) { return pool_set_layers3(env, info, 0); }
+static napi_value js_pool_set_gi(napi_env env, napi_callback_info info) { return pool_set_layers3(env, info, 1); }
+
+static napi_value js_pool_clock(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(2)
+  fs_clock c;
+  int rc = fs_pool_clock(q->p, slot, &c);
+  return rc ? pool_fail(env, q, rc) : clock_object(env, &c);
+}
+
+static napi_value js_pool_set_timing(napi_env env, napi_callback_info info) {
+  POOL_ARGS(2)
+  bool on = false;
+  napi_get_value_bool(env, a[1], &on);
+  return POOL_RESULT(fs_pool_set_timing(q->p, on ? 1 : 0));
+}
+
+/* poolKernelMs(p) -> {stage1, stage2, launches} (timing mode; -1 when not measured) */
+static napi_value js_pool_kernel_ms(napi_env env, napi_callback_info info) {
+  POOL_ARGS(1)
+  double s1 = -1, s2 = -1;
+  int64_t n = 0;
+  int rc = fs_pool_kernel_ms(q->p, &s1, &s2, &n);
+  if (rc) return pool_fail(env, q, rc);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "stage1", s1);
+  set_num(env, o, "stage2", s2);
+  set_num(env, o, "launches", (double)n);
+  return o;
+}
+
+enum { PW_ADVANCE, PW_STEP, PW_READ_STATE };
+
+typedef struct PoolWork {
+  napi_async_work work;
+  napi_deferred deferred;
+  napi_ref keep;          /* the pool's handle object, held until the call completes */
+  PoolH *q;
+  int kind;
+  int32_t n, K, slot;
+  int32_t *slots;
+  double *targets;
+  fs_clock *clks;
+  double *f[4];           /* readState: h, hu, hv, hmax */
+  fs_mass mass;
+  fs_clock clk;
+  size_t cells;
+  int rc;
+  char err[512];
+} PoolWork;
+
+static void pool_work_free(PoolWork *w) {
+  free(w->slots);
+  free(w->targets);
+  free(w->clks);
+  for (int k = 0; k < 4; ++k) free(w->f[k]);
+  free(w);
+}
+
+static void pool_work_execute(napi_env env, void *data) {
+  (void)env;
+  PoolWork *w = (PoolWork *)data;
+  switch (w->kind) {
+    case PW_ADVANCE:
+      w->rc = fs_pool_advance(w->q->p, w->slots, w->targets, w->n, w->K, w->clks);
+      break;
+    case PW_STEP:
+      w->rc = fs_pool_step(w->q->p, w->slots, w->targets, w->n, w->K);
+      break;
+    case PW_READ_STATE:
+      w->rc = fs_pool_read_state(w->q->p, w->slot, w->f[0], w->f[1], w->f[2], w->f[3], &w->mass);
+      if (!w->rc) w->rc = fs_pool_clock(w->q->p, w->slot, &w->clk);
+      break;
+  }
+  if (w->rc) {
+    const char *e = fs_pool_last_error(w->q->p);
+    snprintf(w->err, sizeof w->err, "%s", e ? e : "");
+  }
+}
+
+static void pool_work_complete(napi_env env, napi_status status, void *data) {
+  PoolWork *w = (PoolWork *)data;
+  w->q->busy = 0;
+  if (status != napi_ok || w->rc) {
+    char m[640];
+    if (w->rc == FS_E_STALL)
+      snprintf(m, sizeof m, "GPU solver made no progress (device error?)");  /* solver-gpu.js:601 */
+    else
+      snprintf(m, sizeof m, "floodsim_hip pool error %d: %s", w->rc, w->err);
+    napi_value msg, err, code;
+    napi_create_string_utf8(env, m, NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, NULL, msg, &err);
+    napi_create_int32(env, w->rc, &code);
+    napi_set_named_property(env, err, "code", code);
+    napi_reject_deferred(env, w->deferred, err);
+  } else {
+    napi_value result = NULL;
+    switch (w->kind) {
+      case PW_ADVANCE:
+        napi_create_array_with_length(env, (size_t)w->n, &result);
+        for (int32_t k = 0; k < w->n; ++k) napi_set_element(env, result, (uint32_t)k, clock_object(env, &w->clks[k]));
+        break;
+      case PW_STEP:
+        napi_get_undefined(env, &result);
+        break;
+      case PW_READ_STATE: {
+        static const char *names[4] = {"h", "hu", "hv", "hmax"};
+        napi_create_object(env, &result);
+        for (int k = 0; k < 4; ++k) napi_set_named_property(env, result, names[k], make_f64(env, w->f[k], w->cells));
+        napi_set_named_property(env, result, "mass", mass_object(env, &w->mass));
+        napi_set_named_property(env, result, "clock", clock_object(env, &w->clk));
+        break;
+      }
+    }
+    napi_resolve_deferred(env, w->deferred, result);
+  }
+  napi_delete_async_work(env, w->work);
+  napi_delete_reference(env, w->keep);
+  pool_work_free(w);
+}
+
+static napi_value pool_queue(napi_env env, napi_value handle, PoolWork *w) {
+  napi_value promise, name;
+  napi_create_promise(env, &w->deferred, &promise);
+  napi_create_reference(env, handle, 1, &w->keep);
+  napi_create_string_utf8(env, "floodsim-pool", NAPI_AUTO_LENGTH, &name);
+  napi_create_async_work(env, NULL, name, pool_work_execute, pool_work_complete, w, &w->work);
+  w->q->busy = 1;
+  if (napi_queue_async_work(env, w->work) != napi_ok) {
+    w->q->busy = 0;
+    napi_delete_async_work(env, w->work);
+    napi_delete_reference(env, w->keep);
+    pool_work_free(w);
+    napi_throw_error(env, NULL, "floodsim: napi_queue_async_work failed");
+    return NULL;
+  }
+  return promise;
+}
+
+/* poolAdvance(p, Int32Array slots, Float64Array targets, K) -> Promise<clock[]>;
+ * poolStep(p, slots, targets, K) -> Promise<undefined> (exactly K steps, stopping at each target) */
+static napi_value pool_advance_or_step(napi_env env, napi_callback_info info, int kind) {
+  POOL_ARGS(4)
+  void *sl, *tg;
+  size_t n = 0, n2 = 0;
+  if (typed(env, a[1], napi_int32_array, -1, &sl, &n, "slots")) return NULL;
+  if (typed(env, a[2], napi_float64_array, (long)n, &tg, &n2, "targets")) return NULL;
+  for (size_t k = 0; k < n; ++k) {
+    const int32_t s = ((const int32_t *)sl)[k];
+    if (s < 0 || s >= q->cap || q->dims[2 * s] == 0) {
+      napi_throw_range_error(env, NULL, "floodsim: no instance in that pool slot");
+      return NULL;
+    }
+  }
+  int32_t K = 64;
+  if (argc > 3 && !is_nullish(env, a[3])) napi_get_value_int32(env, a[3], &K);
+  PoolWork *w = (PoolWork *)calloc(1, sizeof(PoolWork));
+  if (!w) {
+    napi_throw_error(env, NULL, "floodsim: out of host memory");
+    return NULL;
+  }
+  w->q = q;
+  w->kind = kind;
+  w->n = (int32_t)n;
+  w->K = K;
+  w->slots = (int32_t *)malloc((n ? n : 1) * sizeof(int32_t));
+  w->targets = (double *)malloc((n ? n : 1) * sizeof(double));
+  w->clks = (fs_clock *)calloc(n ? n : 1, sizeof(fs_clock));
+  if (!w->slots || !w->targets || !w->clks) {
+    pool_work_free(w);
+    napi_throw_error(env, NULL, "floodsim: out of host memory");
+    return NULL;
+  }
+  if (n) {
+    memcpy(w->slots, sl, n * sizeof(int32_t));
+    memcpy(w->targets, tg, n * sizeof(double));
+  }
+  return pool_queue(env, a[0], w);
+}
+static napi_value js_pool_advance(napi_env env, napi_callback_info info) { return pool_advance_or_step(env, info, PW_ADVANCE); }
+static napi_value js_pool_step(napi_env env, napi_callback_info info) { return pool_advance_or_step(env, info, PW_STEP); }
+
+/* poolReadState(p, slot) -> Promise<{h, hu, hv, hmax, mass, clock}> */
+static napi_value js_pool_read_state(napi_env env, napi_callback_info info) {
+  POOL_SLOT_ARGS(2)
+  PoolWork *w = (PoolWork *)calloc(1, sizeof(PoolWork));
+  if (!w) {
+    napi_throw_error(env, NULL, "floodsim: out of host memory");
+    return NULL;
+  }
+  w->q = q;
+  w->kind = PW_READ_STATE;
+  w->slot = slot;
+  w->cells = (size_t)q->dims[2 * slot] * q->dims[2 * slot + 1];
+  int oom = 0;
+  for (int k = 0; k < 4; ++k) oom |= !(w->f[k] = (double *)malloc(w->cells * sizeof(double)));
+  if (oom) {
+    pool_work_free(w);
+    napi_throw_error(env, NULL, "floodsim: out of host memory");
+    return NULL;
+  }
+  return pool_queue(env, a[0], w);
+}
+
+#define POOL_FN(name, fn) {name, NULL, fn, NULL, NULL, NULL, napi_enumerable, NULL},
+#define POOL_FUNCTIONS                                        \
+  POOL_FN("poolCreate", js_pool_create)                       \
+  POOL_FN("poolDestroy", js_pool_destroy)                     \
+  POOL_FN("poolBytesNeeded", js_pool_bytes_needed)            \
+  POOL_FN("poolBytesUsed", js_pool_bytes_used)                \
+  POOL_FN("poolAdd", js_pool_add)                             \
+  POOL_FN("poolRemove", js_pool_remove)                       \
+  POOL_FN("poolSetRows", js_pool_set_rows)                    \
+  POOL_FN("poolSetState", js_pool_set_state)                  \
+  POOL_FN("poolSetRain", js_pool_set_rain)                    \
+  POOL_FN("poolSetInletRates", js_pool_set_inlet_rates)       \
+  POOL_FN("poolSetStageCells", js_pool_set_stage_cells)       \
+  POOL_FN("poolSetStage", js_pool_set_stage)                  \
+  POOL_FN("poolSetCrests", js_pool_set_crests)                \
+  POOL_FN("poolSetCrestLevels", js_pool_set_crest_levels)     \
+  POOL_FN("poolReadBarriers", js_pool_read_barriers)          \
+  POOL_FN("poolSetGa", js_pool_set_ga)                        \
+  POOL_FN("poolSetGi", js_pool_set_gi)                        \
+  POOL_FN("poolClock", js_pool_clock)                         \
+  POOL_FN("poolSetTiming", js_pool_set_timing)                \
+  POOL_FN("poolKernelMs", js_pool_kernel_ms)                  \
+  POOL_FN("poolAdvance", js_pool_advance)                     \
+  POOL_FN("poolStep", js_pool_step)                           \
+  POOL_FN("poolReadState", js_pool_read_state)
+
 /* ---------- module ---------- */
 
 static napi_value init(napi_env env, napi_value exports) {
@@ -870,6 +1447,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"snapshot", NULL, js_snapshot, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readHazard", NULL, js_read_hazard, NULL, NULL, NULL, napi_enumerable, NULL},
       {"restore", NULL, js_restore, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"snapshotLayout", NULL, js_snapshot_layout, NULL, NULL, NULL, napi_enumerable, NULL},
+      POOL_FUNCTIONS
   };
   if (napi_define_properties(env, exports, sizeof d / sizeof d[0], d) != napi_ok) return NULL;
   return exports;

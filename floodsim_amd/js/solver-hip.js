@@ -38,20 +38,12 @@
 const path = require('path');
 
 const addon = require(path.join(__dirname, 'floodsim_napi.node'));
+const { def, f64, sidesOf, paramsOf, layersOf, crestColumns, barrierBooks } = require(path.join(__dirname, 'params.js'));
 
 const GH = 2; // the reference's ghost width, kept for idx(i, j) and zh
-const BC = { wall: 0, open: 1, outflow: 2 };
 const UNSUPPORTED = ['subgrid', 'sink', 'sediment', 'debris', 'scour', 'banks', 'underdrains', 'lines'];
-const BARRIER_BOOKS = ['plus', 'minus', 'peak', 'level', 'load', 'tPeak', 'tLevel', 'tLoad', 'tOver'];
-
-function def(v, d) { return v === undefined || v === null ? d : v; }
-
-function f64(a, n, what) {
-  if (a === undefined || a === null) return null;
-  if (typeof a === 'number') return new Float64Array(n).fill(a);
-  if (a.length !== n) throw new RangeError(what + ': length ' + a.length + ', expected ' + n);
-  return a instanceof Float64Array ? a : Float64Array.from(a);
-}
+// the named parts of a snapshot (fs_snapshot_layout), as `buf` lists them
+const SNAP_PARTS = ['record', 'planes', 'barriers', 'soil', 'rainField'];
 
 class GPUSolver {
   constructor(device, o) {
@@ -66,7 +58,7 @@ class GPUSolver {
     this.cfl = def(o.cfl, this.secondOrder ? 0.48 : 0.9);
     this.hdry = def(o.hdry, 1e-5); this.hfront = def(o.hfront, 1e-3);
     this.limiter = o.limiter === 'vanleer' ? 1 : 0;
-    this.bc = Object.assign({ west: 'wall', east: 'wall', south: 'wall', north: 'wall' }, o.bc || {});
+    this.bc = sidesOf(o.bc);
     this.dtDry = def(o.dtDry, 1.0);
     this.lts = o.lts === true ? 3 : (o.lts | 0);
     this.ltsOff = null;
@@ -76,29 +68,16 @@ class GPUSolver {
     // F26 options, default off (solver-gpu.js:43)
     this.implicitFriction = !!o.implicitFriction; this.stepFlux = !!o.stepFlux;
     const n = o.nx * o.ny;
-    const z = f64(o.z, n, 'z');
+    const params = paramsOf(o, true);
+    const layers = layersOf(o);
+    const z = layers[0], rf = layers[5];
     let zmin = Infinity;
     for (let k = 0; k < n; k++) if (z[k] < zmin) zmin = z[k];
     this.zref = zmin;
     this._z = z;
-    this._mask = o.mask ? (o.mask instanceof Uint8Array ? o.mask : Uint8Array.from(o.mask)) : null;
-    const rf = f64(o.rainField, n, 'rainField');
+    this._mask = layers[3];
     this._rf = rf;
     this.nInlets = o.inlets ? o.inlets.length : 0;
-    const params = {
-      nx: o.nx, ny: o.ny, dx: o.dx, g: this.g, cfl: this.cfl, hdry: this.hdry, hfront: this.hfront,
-      dtDry: this.dtDry, secondOrder: this.secondOrder ? 1 : 0, limiter: this.limiter,
-      bcWest: BC[this.bc.west], bcEast: BC[this.bc.east], bcSouth: BC[this.bc.south], bcNorth: BC[this.bc.north],
-      lts: this.lts, rows: o.rows | 0, untiled: this.tiled ? 0 : 1, hazard: this.hazard ? 1 : 0,
-      implicitFriction: this.implicitFriction ? 1 : 0, stepFlux: this.stepFlux ? 1 : 0,
-    };
-    for (const side of ['west', 'east', 'south', 'north']) {
-      if (BC[this.bc[side]] === undefined) throw new Error('GPUSolver (HIP): bc.' + side + ' = ' + this.bc[side]);
-    }
-    const manArr = typeof o.manning === 'number' || !o.manning ? null : f64(o.manning, n, 'manning');
-    const manS = typeof o.manning === 'number' ? o.manning : 0;
-    const infil = o.infil === undefined || o.infil === null ? null : f64(o.infil, n, 'infil');
-    const inlets = this.nInlets ? Int32Array.from(o.inlets) : null;
     const ga = o.ga ? [f64(o.ga.psiD, n, 'ga.psiD'), f64(o.ga.fmax, n, 'ga.fmax'), f64(o.ga.perv, n, 'ga.perv')] : null;
     const gi = o.gi ? [f64(o.gi.rate, n, 'gi.rate'), f64(o.gi.cap, n, 'gi.cap'), f64(o.gi.drain, n, 'gi.drain')] : null;
     this.stage = -1e9;
@@ -109,18 +88,16 @@ class GPUSolver {
     // does not throw from the constructor; `ready` rejects with it, as the reference's does when its pipelines
     // fail (solver-gpu.js:287), and every later call on the solver fails with the same error.
     this._h = null;
+    this._buf = null;
     this._createError = null;
     try {
-      this._h = addon.create(params, z, manArr, manS, this._mask, infil, rf, inlets);
+      this._h = addon.create(params, ...layers);
       if (o.stageCells && o.stageCells.length) addon.setStageCells(this._h, Int32Array.from(o.stageCells));
       // Green-Ampt soil and GI stores per cell (solver-cpu.js:94-96)
       if (ga) addon.setGa(this._h, ga[0], ga[1], ga[2]);
       if (gi) addon.setGi(this._h, gi[0], gi[1], gi[2]);
-      if (this.nBar) {
-        const col = (k, d, T) => T.from(this.crest.map((f) => (f[k] === undefined || f[k] === null ? d : f[k])));
-        addon.setCrests(this._h, col('i', 0, Int32Array), col('j', 0, Int32Array), col('axis', 0, Int32Array),
-          col('z', -Infinity, Float64Array), col('C', 1.6, Float64Array), col('ground', -Infinity, Float64Array));
-      }
+      if (this.nBar) addon.setCrests(this._h, ...crestColumns(this.crest));
+      this._layout();
     } catch (e) {
       if (this._h) addon.destroy(this._h);
       this._h = null;
@@ -226,7 +203,7 @@ class GPUSolver {
 
   setRainField(f) {
     const a = f64(f, this.nx * this.ny, 'rainField');
-    this._sync(() => addon.setRainField(this._h, a));
+    this._sync(() => { addon.setRainField(this._h, a); this._layout(); });
   }
 
   /** Water level (m, absolute) the stage cells' ghosts hold from the next step on (solver-gpu.js:554). */
@@ -244,14 +221,7 @@ class GPUSolver {
 
   /** Barrier books per face since the start (solver-gpu.js:490-504). */
   readBarriers() {
-    return this._enqueue(() => {
-      const a = addon.readBarriers(this._h, this.nBar), o = {};
-      BARRIER_BOOKS.forEach((k, q) => {
-        o[k] = new Float64Array(this.nBar);
-        for (let m = 0; m < this.nBar; m++) o[k][m] = a[9 * m + q];
-      });
-      return o;
-    });
+    return this._enqueue(() => barrierBooks(addon.readBarriers(this._h, this.nBar), this.nBar));
   }
 
   /** Depth infiltrated so far per interior cell under Green-Ampt (m per pervious area; 0 without it;
@@ -333,6 +303,71 @@ class GPUSolver {
     const meta = { t: this.t, steps: this.steps, lastDt: this.lastDt, dtAvg: this.dtAvg, vol: Object.assign({}, this.vol),
       rainRate: this._rain };
     return this._enqueue(() => addon.snapshot(this._h)).then((bytes) => ({ bytes, meta }));
+  }
+
+  /** The device state as named buffers with their sizes in bytes, {record, planes, barriers?, soil?, rainField?}
+   *  ({name: {size}}, the parts a snapshot holds; the reference's solver.buf, solver-gpu.js:173-299, which
+   *  gpuSnapshot / gpuRestore walk, checkpoint.js:96-112). */
+  get buf() {
+    if (this._createError) throw this._createError;
+    return this._buf;
+  }
+
+  _layout() {
+    const sizes = addon.snapshotLayout(this._h), b = {};
+    SNAP_PARTS.forEach((k, q) => { if (sizes[q] > 0) b[k] = { size: sizes[q] }; });
+    this._buf = b;
+    return b;
+  }
+
+  /** The named parts of a snapshot, {name: Uint8Array} in the order of `buf` (checkpoint.js:96-104). */
+  snapshotParts() {
+    this._raiseDeferred();
+    return this._enqueue(() => {
+      const buf = this._layout();
+      return addon.snapshot(this._h).then((bytes) => {
+        const out = {};
+        let off = 0;
+        for (const k of Object.keys(buf)) {
+          out[k] = bytes.slice(off, off + buf[k].size);
+          off += buf[k].size;
+        }
+        return out;
+      });
+    });
+  }
+
+  /** Put back named parts ({name: Uint8Array}), each checked against the size of the buffer of that name here
+   *  (checkpoint.js:106-112); every part this solver holds must be given. The device clock and books come back
+   *  with them; the JS fields t, steps, lastDt and dtAvg follow the device clock. */
+  restoreParts(parts) {
+    const buf = this.buf, names = Object.keys(buf);
+    let total = 0;
+    for (const k of Object.keys(parts)) {
+      const b = buf[k];
+      if (!b) continue;
+      const n = parts[k].byteLength;
+      if (b.size !== n) {
+        throw new Error('checkpoint: solver buffer ' + k + ' is ' + b.size + ' bytes here, ' + n +
+          ' in the checkpoint (a different grid or options)');
+      }
+    }
+    for (const k of names) {
+      if (!parts[k]) throw new Error('checkpoint: solver buffer ' + k + ' is not in the checkpoint');
+      total += buf[k].size;
+    }
+    const all = new Uint8Array(total);
+    let off = 0;
+    for (const k of names) {
+      const p = parts[k];
+      all.set(p instanceof Uint8Array ? p : new Uint8Array(p.buffer, p.byteOffset, p.byteLength), off);
+      off += buf[k].size;
+    }
+    this._raiseDeferred();
+    return this._enqueue(() => addon.restore(this._h, all)).then(() => {
+      const c = addon.clock(this._h);
+      this.t = c.t; this.steps = c.steps; this.lastDt = c.lastDt; this.dtAvg = c.dtAvg;
+    });
   }
 
   /** Put back a snapshot of this solver, or of one built from the same model. */

@@ -67,8 +67,9 @@ EXPORTS = [
     "fs_set_stage_cells", "fs_set_stage", "fs_pool_set_stage_cells", "fs_pool_set_stage",
     "fs_set_crests", "fs_set_crest_levels", "fs_read_barriers", "fs_pool_set_crests", "fs_pool_set_crest_levels",
     "fs_pool_read_barriers", "fs_set_ga", "fs_set_gi", "fs_pool_set_ga", "fs_pool_set_gi", "fs_pool_debug",
-    "fs_advance_noalloc",
+    "fs_advance_noalloc", "fs_snapshot_layout",
 ]
+SNAP_PARTS = ("record", "planes", "barriers", "soil", "rainField")   # fs_snapshot_layout's parts, in order
 FS_DEBUG_NO_STEPS = 1
 
 
@@ -151,6 +152,7 @@ def lib(path=None):
     L.fs_restore.argtypes = [P, P, SZ]
     L.fs_pool_snapshot.argtypes = [P, I32, P, SZ]
     L.fs_pool_restore.argtypes = [P, I32, P, SZ]
+    L.fs_snapshot_layout.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
     fp = ctypes.POINTER(ctypes.c_float)
     L.fs_read_hazard.argtypes = [P, fp, fp, fp, fp, fp, fp, u8p, u8p, u8p]
     L.fs_read_infiltration.argtypes = [P, fp]
@@ -431,6 +433,13 @@ class GPUSolver:
         """Put a snapshot of this solver (or of one built from the same model) back."""
         b = np.ascontiguousarray(np.frombuffer(buf, dtype=np.uint8) if not isinstance(buf, np.ndarray) else buf)
         self._chk(lib().fs_restore(self._h, b.ctypes.data_as(ctypes.c_void_p), b.nbytes))
+
+    def snapshot_layout(self):
+        """Bytes of each named part of a snapshot, in order ({record, planes, barriers, soil, rainField}; a part
+        the solver does not hold is 0)."""
+        b = (ctypes.c_uint64 * len(SNAP_PARTS))()
+        self._chk(lib().fs_snapshot_layout(self._h, b))
+        return dict(zip(SNAP_PARTS, (int(x) for x in b)))
 
     def levels(self):
         """LTS tile levels of the last cycle, (ty, tx) uint8."""

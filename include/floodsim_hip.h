@@ -225,6 +225,14 @@ int fs_copy_probe(size_t bytes, int32_t iters, double *gbps, double *ms);
 size_t fs_snapshot_bytes(fs_handle s);
 int fs_snapshot(fs_handle s, void *out, size_t bytes);
 int fs_restore(fs_handle s, const void *in, size_t bytes);
+/* The parts of that buffer, in order, as the named buffers gpuSnapshot returns and gpuRestore size-checks
+ * (checkpoint.js:96-112): bytes[FS_SNAP_RECORD] the clock, step record and FP64 books; bytes[FS_SNAP_PLANES]
+ * the solver's main device allocation (state, layers, LTS stores, hazard products, strip tables);
+ * bytes[FS_SNAP_BARRIERS] the barrier crests and books; bytes[FS_SNAP_SOIL] the Green-Ampt and GI planes;
+ * bytes[FS_SNAP_RAINFIELD] a rain field given after creation. A part the solver does not hold is 0 bytes.
+ * The parts add up to fs_snapshot_bytes. */
+enum { FS_SNAP_RECORD = 0, FS_SNAP_PLANES, FS_SNAP_BARRIERS, FS_SNAP_SOIL, FS_SNAP_RAINFIELD, FS_SNAP_PARTS };
+int fs_snapshot_layout(fs_handle s, uint64_t bytes[FS_SNAP_PARTS]);
 size_t fs_pool_snapshot_bytes(fs_pool_t pool, int32_t slot);
 int fs_pool_snapshot(fs_pool_t pool, int32_t slot, void *out, size_t bytes);
 int fs_pool_restore(fs_pool_t pool, int32_t slot, const void *in, size_t bytes);

@@ -22,7 +22,12 @@ pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
 ADDON_FUNCS = ["advance", "clock", "create", "debugNoSteps", "destroy", "frame", "levels", "readHazard", "readInlets", "readState",
                "readBarriers", "readInfiltration", "restore", "setCrestLevels", "setCrests", "setGa", "setGi", "setInletRates", "setRain",
                "setRainField", "setStage",
-               "setStageCells", "setState", "snapshot", "tileStats", "version"]
+               "setStageCells", "setState", "snapshot", "snapshotLayout", "tileStats", "version",
+               # the pool entries (fs_pool_*), for floodsim_amd/js/pool-hip.js
+               "poolCreate", "poolDestroy", "poolBytesNeeded", "poolBytesUsed", "poolAdd", "poolRemove", "poolSetRows",
+               "poolSetState", "poolSetRain", "poolSetInletRates", "poolSetStageCells", "poolSetStage", "poolSetCrests",
+               "poolSetCrestLevels", "poolReadBarriers", "poolSetGa", "poolSetGi", "poolClock", "poolSetTiming",
+               "poolKernelMs", "poolAdvance", "poolStep", "poolReadState"]
 
 
 def node(script, *args, timeout=120):

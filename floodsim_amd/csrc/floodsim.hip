@@ -439,7 +439,8 @@ hipEvent_t timing_event(fs_pool *P) {
 }
 
 int launch_step(fs_pool *P, const Launch &G) {
-  const int nb = (G.nwork + 3) / 4, ni = G.ni;
+  const int nb = (G.nwork + kGlobalWaves - 1) / kGlobalWaves, ni = G.ni;
+  constexpr int kT = 64 * kGlobalWaves;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   if (P->timing) {
     e0 = timing_event(P); e1 = timing_event(P); e2 = timing_event(P);
@@ -448,8 +449,8 @@ int launch_step(fs_pool *P, const Launch &G) {
   }
   // one launch per kernel variant the group's instances need (each variant skips the other kinds)
   const unsigned K = G.kinds;
-  if (K & 0x5u) sw_stage<0, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
-  if (K & 0xau) sw_stage<0, false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
+  if (K & 0x5u) sw_stage<0, false><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
+  if (K & 0xau) sw_stage<0, false, false, true><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, 0);
   for (int q = 0; q < G.nWithBarriers; ++q) {
     const int gi = G.withBarriers[q];
     sw_barrier_books<0, false><<<(G.hinst[gi].nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
@@ -457,10 +458,10 @@ int launch_step(fs_pool *P, const Launch &G) {
   if (P->timing) HIPCHK(P, hipEventRecord(e1, P->stream));
   // each instance runs its own scheme: the kernel of the other order returns at once for it
   if (G.anySecondOrder) {
-    if (K & 0x1u) sw_stage<1, false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    if (K & 0x2u) sw_stage<1, false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    if (K & 0x4u) sw_stage<1, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    if (K & 0x8u) sw_stage<1, false, true, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x1u) sw_stage<1, false><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x2u) sw_stage<1, false, false, true><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x4u) sw_stage<1, false, true><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x8u) sw_stage<1, false, true, true><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
     for (int q = 0; q < G.nWithBarriers; ++q) {
       const int gi = G.withBarriers[q];
       sw_barrier_books<1, false><<<(G.hinst[gi].nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, gi, 0);
@@ -468,10 +469,10 @@ int launch_step(fs_pool *P, const Launch &G) {
   }
   if (P->timing) P->spans.push_back({e1, nullptr, -1, -1});  // completed below with e2
   if (G.anyFirstOrder) {
-    if (K & 0x1u) sw_euler_finish<false><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    if (K & 0x2u) sw_euler_finish<false, false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    if (K & 0x4u) sw_euler_finish<false, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
-    if (K & 0x8u) sw_euler_finish<false, true, true><<<nb, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x1u) sw_euler_finish<false><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x2u) sw_euler_finish<false, false, true><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x4u) sw_euler_finish<false, true><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
+    if (K & 0x8u) sw_euler_finish<false, true, true><<<nb, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, 0);
   }
   if (P->timing) { HIPCHK(P, hipEventRecord(e2, P->stream)); P->spans.back().b = e2; }
   for (int q = 0; q < G.nWithInlets; ++q) {
@@ -526,7 +527,8 @@ int launch_cycle(fs_pool *P, const Launch &G) {
   const DevInst &d = G.hinst[0];
   const int nt = d.tx * d.ty, nu = d.wend - d.wbeg, L = d.L;
   // kLtsWaveCap wavefronts per unit at most (records of few units split them); idle blocks exit
-  const int nbp = ((kLtsWaveCap * nu + 3) / 4 + 7) / 8 * 8;
+  const int nbp = ((kLtsWaveCap * nu + kLtsWaves - 1) / kLtsWaves + 7) / 8 * 8;
+  constexpr int kT = 64 * kLtsWaves;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   if (P->timing) {
     e0 = timing_event(P); e1 = timing_event(P); e2 = timing_event(P);
@@ -564,10 +566,10 @@ int launch_cycle(fs_pool *P, const Launch &G) {
       if ((k + 1) % (1 << m)) continue;
       const bool ext = d.nbar || (d.flags & FLAG_STAGE) || d.soil || d.gi || d.limiter;
       if (ext) {
-        sw_stage<0, true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
+        sw_stage<0, true, false, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
         if (d.nbar) sw_barrier_books<0, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
       } else {
-        sw_stage<0, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
+        sw_stage<0, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p1, m);
       }
       hipEvent_t sa = nullptr, sb = nullptr;
       if (P->timing) {
@@ -579,29 +581,29 @@ int launch_cycle(fs_pool *P, const Launch &G) {
       const int mk = m | (k << 8);
       if (d.secondOrder) {
         if (ext) {
-          if (d.hzA) sw_stage<1, true, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
-          else sw_stage<1, true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          if (d.hzA) sw_stage<1, true, true, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          else sw_stage<1, true, false, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
           if (d.nbar) sw_barrier_books<1, true><<<(d.nbar + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
         } else if (d.hzA) {
-          sw_stage<1, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          sw_stage<1, true, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
         } else {
-          sw_stage<1, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          sw_stage<1, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
         }
       } else {
         if (ext) {
-          if (d.hzA) sw_euler_finish<true, true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
-          else sw_euler_finish<true, false, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          if (d.hzA) sw_euler_finish<true, true, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          else sw_euler_finish<true, false, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
         } else if (d.hzA) {
-          sw_euler_finish<true, true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          sw_euler_finish<true, true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
         } else {
-          sw_euler_finish<true><<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
+          sw_euler_finish<true><<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, G.nwork, G.p2, mk);
         }
       }
       if (P->timing) {
         HIPCHK(P, hipEventRecord(sb, P->stream));
         P->spans.push_back({sa, sb, cyc, m});
       }
-      sw_lts_fold<<<nbp, 256, 0, P->stream>>>(G.dinst, G.dwork, m);
+      sw_lts_fold<<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, m);
       if (d.nin) sw_exchange<true><<<(d.nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
       sw_lts_record_end<<<1, 1024, 0, P->stream>>>(G.dinst, G.p1, G.p2, m);
     }

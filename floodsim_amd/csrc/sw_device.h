@@ -13,6 +13,18 @@ constexpr int kLtsTile = 8;                  // LTS level tiles (solver-cpu.js:1
 constexpr int kMaxLevels = 4;                // LTS levels 1..4 (solver-cpu.js:121)
 constexpr double kLtsSafe = 0.9;             // share of its CFL step a tile may take (solver-cpu.js:15)
 constexpr int kSpeedSlots = 4096;            // LTS: slots of the per-block tile-speed maxima
+// wavefronts per block of the strip kernels (global stepping) and of the LTS record kernels: a block holds
+// its CU slots until its longest wavefront ends, so blocks of strips or parts of uneven length keep fewer
+// waves resident the more wavefronts they hold. One per block, measured at 8192^2: LTS cycle 20.43 ms with 4,
+// 19.26 with 2, 19.01 with 1; global step 2.085 / 2.102 / 2.058 ms.
+#ifndef FS_WPB_G
+#define FS_WPB_G 1
+#endif
+#ifndef FS_WPB_L
+#define FS_WPB_L 1
+#endif
+constexpr int kGlobalWaves = FS_WPB_G;
+constexpr int kLtsWaves = FS_WPB_L;
 
 enum { BC_WALL = 0, BC_OPEN = 1, BC_OUTFLOW = 2 };
 

@@ -1069,7 +1069,7 @@ __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts
   m &= 0xff;
   const float dt = I.rec->ldt[m];
   const int split = lts_split(I, m);
-  const int i = xcd_wave_index_n(I.rec->lcount[m] * split, 4);
+  const int i = xcd_wave_index_n(I.rec->lcount[m] * split, kLtsWaves);
   if (i < 0) return;
   int4 wk = {0, 0, 0, 0};
   if (dt > 0.f) {
@@ -1096,7 +1096,7 @@ template <int STAGE, bool LTS, bool HZ = false, bool EXT = false>
 __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                 int nwork, Part *__restrict__ part, int m) {
   static_assert(!LTS, "local time stepping records use the specialisations below");
-  const int wid = xcd_wave_index(4);
+  const int wid = xcd_wave_index(kGlobalWaves);
   const int lane = threadIdx.x & 63;
   if (wid >= nwork) return;
   const int4 wk = work[wid];
@@ -1226,7 +1226,7 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
   const DevInst &I = insts[0];
   const int lane = threadIdx.x & 63;
   const int split = lts_split(I, m);
-  const int i = xcd_wave_index_n(I.rec->lcount[m] * split, 4);
+  const int i = xcd_wave_index_n(I.rec->lcount[m] * split, kLtsWaves);
   const int nu = I.wend - I.wbeg;
   const bool two = I.secondOrder;
   if (i >= 0) {
@@ -1557,7 +1557,7 @@ template <bool LTS, bool HZ = false, bool EXT = false>
 __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict__ insts,
                                                        const int4 *__restrict__ work, int nwork,
                                                        Part *__restrict__ part, int m) {
-  const int wid = xcd_wave_index(4);
+  const int wid = xcd_wave_index(kGlobalWaves);
   const int lane = threadIdx.x & 63;
   if constexpr (!LTS) {
     if (wid >= nwork) return;
@@ -1581,7 +1581,7 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
     m &= 0xff;
     const float dt = I.rec->ldt[m];
     const int split = lts_split(I, m);
-    const int i = xcd_wave_index_n(I.rec->lcount[m] * split, 4);
+    const int i = xcd_wave_index_n(I.rec->lcount[m] * split, kLtsWaves);
     if (i < 0) return;
     const int u = I.ulist[m * (I.wend - I.wbeg) + i / split];
     const int4 wk = lts_rows(I, work[I.wbeg + u], m, lane, split, i % split);

@@ -780,8 +780,12 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         if (take && (lv2 == m + 1 || STAGE == 0)) {
           const int k2 = (r - 2) * pitch + X;
           float st = PRE ? tsh - tsw : gld(I.sh, k2) - gld(I.sw, k2);
-          // first-stage gather, not yet folded
-          if (STAGE == 1 && (PRE ? tgf : gld(I.gflag, k2))) st += PRE ? tg0 : gld(I.g0h, k2);
+          // first-stage gather, not yet folded (its delta loaded with the flag, not after it: one wait)
+          if (STAGE == 1) {
+            const uint8_t gf = PRE ? tgf : gld(I.gflag, k2);
+            const float g0 = PRE ? tg0 : gld(I.g0h, k2);
+            if (gf) st += g0;
+          }
           hv_ = fmaxf(0.f, h2r + st);
         }
       }
@@ -861,13 +865,19 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
           if (STAGE == 1 || LTS) {
             if (STAGE == 1) {
               if constexpr (PRE) { h0 = ph0; w0n = pw0; hu0 = phu0; hv0 = phv0; }
-              else { h0 = gld(I.h, k); w0n = gld(I.w, k); hu0 = gld(I.hu, k); hv0 = gld(I.hv, k); }
+              else {
+                h0 = gld(I.h, k); w0n = gld(I.w, k); hu0 = gld(I.hu, k); hv0 = gld(I.hv, k);
+                if constexpr (!LTS && !EXT) pso = load_src(I, k);   // (as below, under LTS)
+              }
             } else {
               h0 = hh3; w0n = qw; hu0 = hu3; hv0 = hv3;
             }
             if constexpr (LTS) {
               const float sh = PRE ? psh : gld(I.sh, k), sw = PRE ? psw : gld(I.sw, k);
               const float shu = PRE ? pshu : gld(I.shu, k), shv = PRE ? pshv : gld(I.shv, k);
+              // the sources' layers, issued after the update's operands so that one wait covers them
+              // (sources_cell would otherwise wait on each in turn)
+              if constexpr (STAGE == 1 && !PRE && !EXT) pso = load_src(I, k);
               hu0 += shu;
               hv0 += shv;
               kadd(h0, w0n, sh - sw);
@@ -901,7 +911,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             acc_out += o * ((double)dx * (double)dt * (double)wgt);
           }
           if (STAGE == 1) {
-            sources_cell<EXT, EXT || LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax, PRE ? &pso : nullptr);
+            sources_cell<EXT, EXT || LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax, EXT ? nullptr : &pso);
             if constexpr (HZ) {
               if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
             }

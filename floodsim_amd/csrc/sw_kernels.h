@@ -321,13 +321,14 @@ __device__ __forceinline__ Prim load_prim(const DevInst &I, int X, int Y, int m,
 
 // Primitive of interior cell (X, Y) for a row whose 64 lanes are all interior cells (the common
 // case): row-based global loads, no ghost logic
+// lvAhead: the cell's tile level, loaded by the caller a row ahead (the LTS second stage), else unused
 template <int STAGE, bool LTS, bool EXT = false>
-__device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, int m, float dt) {
+__device__ __forceinline__ Prim load_prim_row(const DevInst &I, int X, int Y, int m, float dt, int lvAhead = 0) {
   const int k = Y * I.pitch + X;
   Prim p;
   float hu, hv;
   p.lv = 0;
-  if constexpr (LTS) p.lv = gld(I.lv, (Y >> 3) * I.tx + (X >> 3));
+  if constexpr (LTS) p.lv = STAGE == 1 ? lvAhead : gld(I.lv, (Y >> 3) * I.tx + (X >> 3));
   bool useB = STAGE == 1;
   if constexpr (LTS) useB = STAGE == 1 && p.lv == m;
   if constexpr (LTS && STAGE == 1) {
@@ -622,10 +623,23 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   float smax = 0.f;
   bool stir = false;   // an owned cell ends with h, w, hu or hv non-zero
 
+  // LTS second stage: which state a cell of row r is read from depends on its tile level, so the levels
+  // of the next row are loaded a row ahead (one byte a lane) and the row's loads wait on nothing.
+  // The load is issued first in each pass (rows and columns clamped into the grid, so it needs no branch):
+  // waiting on the row's own loads then covers it.
+  constexpr bool LVA = LTS && STAGE == 1;
+  const int xlv = (min(max(X, 0), nx - 1)) >> 3;
+  int lvAhead = 0;
+  if constexpr (LVA) lvAhead = gld(I.lv, (min(max(y0 - kHalo, 0), ny - 1) >> 3) * I.tx + xlv);
   for (int r = y0 - kHalo; r < y1 + kHalo; ++r) {
     // ---------------- row r: primitives
+    const int lvRow = lvAhead;
+    if constexpr (LVA) {
+      lvAhead = gld(I.lv, (min(max(r + 1, 0), ny - 1) >> 3) * I.tx + xlv);
+      __builtin_amdgcn_sched_barrier(0);   // (kept ahead of the row's loads)
+    }
     Prim p;
-    if (IN || (xin && r >= 0 && r < ny)) p = load_prim_row<STAGE, LTS, EXT>(I, X, r, m, dt);
+    if (IN || (xin && r >= 0 && r < ny)) p = load_prim_row<STAGE, LTS, EXT>(I, X, r, m, dt, lvRow);
     else p = load_prim<STAGE, LTS, EXT>(I, X, r, m, dt);
 
     // ---------------- the pass's other loads, issued here in the order it consumes them: theta's store of

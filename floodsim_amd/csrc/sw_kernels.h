@@ -18,9 +18,9 @@
 #include "sw_device.h"
 
 // which plain stage kernels issue a row pass's loads together (bit 0: global second stage, bit 1: LTS first
-// stage, bit 2: LTS second stage)
+// stage, bit 2: LTS second stage, bit 3: global first stage)
 #ifndef FS_PRE
-#define FS_PRE 3
+#define FS_PRE 11
 #endif
 
 namespace fs {
@@ -573,11 +573,11 @@ template <int STAGE, bool LTS, bool HZ, bool EXT = false, bool IN = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
                                           Part *__restrict__ part, const int cells, const float tnow) {
   constexpr bool SF = EXT;
-  // the plain variants issue the pass's later loads with the row's (see below): the second stage of global
-  // stepping and the first of local time stepping (FS_PRE bit 0 and 1; bit 2, the LTS second stage, spills
-  // registers at four waves per SIMD)
+  // the plain variants issue the pass's later loads with the row's (see below): both stages of global stepping
+  // and the first of local time stepping (FS_PRE bits 0, 3 and 1; bit 2, the LTS second stage, spills registers
+  // at four waves per SIMD). Global first stage, measured at 8192^2: 0.925 ms without, 0.859 ms with.
   constexpr bool PRE = !EXT && !HZ && ((!LTS && STAGE == 1 && (FS_PRE & 1)) || (LTS && STAGE == 0 && (FS_PRE & 2)) ||
-                                       (LTS && STAGE == 1 && (FS_PRE & 4)));
+                                       (LTS && STAGE == 1 && (FS_PRE & 4)) || (!LTS && STAGE == 0 && (FS_PRE & 8)));
   const bool sf = EXT && (I.flags & FLAG_STEPF);
   const bool bar = EXT && I.crest != nullptr;
   const float rain = I.rec->rainRate;

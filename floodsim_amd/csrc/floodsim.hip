@@ -54,10 +54,12 @@ struct Slot {
   float4 *hzA = nullptr, *hzB = nullptr;   // hazard products (fs_params.hazard)
   // local time stepping
   int L = 0, tx = 0, ty = 0, nunits = 0;
-  uint8_t *lv = nullptr, *lv0 = nullptr, *gflag = nullptr, *umask = nullptr;
+  uint8_t *lv = nullptr, *lv0 = nullptr, *gflag = nullptr;
+  uint64_t *urows = nullptr;
+  int4 *plist = nullptr;
   float *tsp = nullptr, *tileRf = nullptr;
   unsigned *speedSlots = nullptr;
-  int32_t *tileOpen = nullptr, *ucells = nullptr, *ulist = nullptr;
+  int32_t *tileOpen = nullptr, *ucells = nullptr;
   float *sh = nullptr, *shu = nullptr, *shv = nullptr, *sw = nullptr;
   float *g0h = nullptr, *g0hu = nullptr, *g0hv = nullptr, *g1h = nullptr, *g1hu = nullptr, *g1hv = nullptr;
   float4 *bsx = nullptr, *bsy = nullptr;
@@ -200,12 +202,12 @@ constexpr int64_t kMax4BCells = (int64_t)1 << 30;
 constexpr int64_t kMax16BCells = (int64_t)1 << 28;
 
 // strip geometry: global stepping uses 58 owned columns (3 halo lanes each side); local time
-// stepping uses 56 (4 halo lanes on the left) and rows in multiples of 8 so that a strip holds
-// whole 8x8 level tiles
+// stepping uses 56 (4 halo lanes on the left) and units of rows in multiples of 8, at most 64, so that a
+// unit holds whole 8x8 level tiles and its rows fit one 64-bit mask
 int strip_own(const fs_params *p) { return p->lts ? 56 : kOwn; }
 int strip_rows(const fs_params *p) {
   int r = p->rows > 0 ? std::max(8, p->rows) : 64;   // at least 8 rows (strip flag capacity)
-  if (p->lts) r = (r + 7) / 8 * 8;
+  if (p->lts) r = std::min(64, (r + 7) / 8 * 8);
   return r;
 }
 int units_of(const fs_params *p) {
@@ -226,7 +228,8 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
     b += 2 * align256(nt) + align256(nt * 4) * 3;           // lv, lv0, tsp, tileOpen, tileRf
     b += align256(kSpeedSlots * 4);                          // tile-speed block maxima
     b += 10 * align256(N * 4) + align256(N) + 2 * align256(N * 16);  // store, gathers, flag, bsx, bsy
-    b += align256(nu) + 2 * align256(nu * 4 * (kMaxLevels + 1));     // umask, ucells, ulist
+    b += align256(nu * 8 * (kMaxLevels + 1)) + align256(nu * 4 * (kMaxLevels + 1));   // urows, ucells
+    b += align256(nu * kLtsUnitRuns * 16 * (kMaxLevels + 1));                         // plist
   }
   if (!p->lts) b += 3 * align256(strip_cap(p)) + align256(strip_cap(p) * 4);  // strip flags
   if (p->hazard) b += 2 * align256(N * 16);  // hazard products
@@ -327,7 +330,7 @@ DevInst devinst_of(const Slot &S, StepRec *rec, int wbeg, int wend) {
   d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
   d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.g1h = S.g1h; d.g1hu = S.g1hu; d.g1hv = S.g1hv;
   d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
-  d.umask = S.umask; d.ucells = S.ucells; d.ulist = S.ulist;
+  d.urows = S.urows; d.ucells = S.ucells; d.plist = S.plist; d.capParts = kLtsUnitRuns * S.nunits;
   d.crest = S.crest; d.bidx = S.bidx; d.barFace = S.barFace; d.barGround = S.barGround;
   d.barFlux = S.barFlux; d.barBook = S.barBook; d.nbar = S.nbar;
   d.soil = S.soil; d.gi = S.gi;
@@ -526,7 +529,7 @@ int collect_timing(fs_pool *P) {
 int launch_cycle(fs_pool *P, const Launch &G) {
   const DevInst &d = G.hinst[0];
   const int nt = d.tx * d.ty, nu = d.wend - d.wbeg, L = d.L;
-  // kLtsWaveCap wavefronts per unit at most (records of few units split them); idle blocks exit
+  // kLtsWaveCap wavefronts per unit (a record's parts beyond that share wavefronts); idle blocks exit
   const int nbp = ((kLtsWaveCap * nu + kLtsWaves - 1) / kLtsWaves + 7) / 8 * 8;
   constexpr int kT = 64 * kLtsWaves;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
@@ -540,7 +543,7 @@ int launch_cycle(fs_pool *P, const Launch &G) {
   sw_lts_level<<<(nt + 255) / 256, 256, 0, P->stream>>>(G.dinst);
   sw_lts_smooth<<<(nt + 255) / 256, 256, 0, P->stream>>>(G.dinst);
   sw_lts_units<<<(nu + 255) / 256, 256, 0, P->stream>>>(G.dinst, G.dwork);
-  sw_lts_lists<<<1, 1024, 0, P->stream>>>(G.dinst);
+  sw_lts_lists<<<kMaxLevels + 1, 1024, 0, P->stream>>>(G.dinst, G.dwork);
   const int cyc = P->nCycles;
   if (P->timing) {
     HIPCHK(P, hipEventRecord(e1, P->stream));
@@ -966,9 +969,9 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     S.g1h = (float *)take(N * 4); S.g1hu = (float *)take(N * 4); S.g1hv = (float *)take(N * 4);
     S.gflag = (uint8_t *)take(N);
     S.bsx = (float4 *)take(N * 16); S.bsy = (float4 *)take(N * 16);
-    S.umask = (uint8_t *)take((size_t)S.nunits);
+    S.urows = (uint64_t *)take((size_t)S.nunits * 8 * (kMaxLevels + 1));
     S.ucells = (int32_t *)take((size_t)S.nunits * 4 * (kMaxLevels + 1));
-    S.ulist = (int32_t *)take((size_t)S.nunits * 4 * (kMaxLevels + 1));
+    S.plist = (int4 *)take((size_t)S.nunits * kLtsUnitRuns * 16 * (kMaxLevels + 1));
   } else {
     S.sxn = (nx + strip_own(p) - 1) / strip_own(p);
     S.nstrips = units_of(p);

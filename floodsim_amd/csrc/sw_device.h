@@ -91,7 +91,7 @@ struct StepRec {
   double lrf[kMaxLevels + 1];     // sum of rain multipliers over own open cells per level
   float ldt[kMaxLevels + 1];      // record step per level: delta * 2^m (0 = no-op)
   int32_t lcells[kMaxLevels + 1]; // own open cells per level
-  int32_t lcount[kMaxLevels + 1]; // work-list length per level
+  int32_t lcount[kMaxLevels + 1]; // parts per level (the records' work lists)
   int32_t records;                // records run so far (diagnostic)
   int32_t active;                 // open cells the last step advanced (dry strips skipped)
   int32_t activeStrips;           // strips the last step advanced (dry-strip skipping on)
@@ -156,9 +156,12 @@ struct DevInst {
   uint8_t *gflag;               // 1: the cell gathered in the current record
   float4 *bsx, *bsy;            // a cell's reconstruction toward its faces at their last evaluation:
                                 // x (h west, dz west, h east, dz east), y (south, north); dz = face bed - cell bed
-  uint8_t *umask;               // per work unit: levels it takes part in
+  uint64_t *urows;              // per level and work unit (stride wend - wbeg): the unit's rows a record of the
+                                // level owns (bit q: row q of the unit), gaps of up to 2 kHalo rows filled
   int32_t *ucells;              // per work unit and level: own open cells
-  int32_t *ulist;               // per level (stride wend - wbeg): work units of the level's records
+  int4 *plist;                  // per level (stride capParts): the parts of the level's records, (0, first
+                                // column, first row, rows) as a work item
+  int capParts;                 // parts a level's list holds (8 per unit)
   // flood barriers (null: none). Per cell: crest and weir coefficient of its west face (x) and south face
   // (y), crest relative to the bed minimum (-1e30 = none); the barrier index of those faces (-1 = none).
   // Per barrier: (cell k of its right/upper side, k of the other side, axis 0 = x face / 1 = y face, 0),

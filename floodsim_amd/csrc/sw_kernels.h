@@ -571,7 +571,8 @@ struct FaceRow { float F0, F1, F2, cL, cR; };
 // no ghost cells and no domain-edge faces (the common case, compiled as a loop of its own)
 template <int STAGE, bool LTS, bool HZ, bool EXT = false, bool IN = false>
 __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
-                                          Part *__restrict__ part, const int cells, const float tnow) {
+                                          Part *__restrict__ part, const int cells, const float tnow,
+                                          const bool acc = false) {
   constexpr bool SF = EXT;
   // the plain variants issue the pass's later loads with the row's (see below): both stages of global stepping
   // and the first of local time stepping (FS_PRE bits 0, 3 and 1; bit 2, the LTS second stage, spills registers
@@ -957,7 +958,15 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
 
   const double o = wave_sum(acc_out), c = wave_sum(acc_clip), f = wave_sum(acc_inf);
   const float s = wave_max(smax);
-  if (lane == 0) *part = Part{o, c, f, s, cells};
+  if (lane == 0) {
+    if (acc) {   // a further part of the same wavefront (LTS): its books join the partial
+      Part q = *part;
+      q.out += o; q.clip += c; q.infil += f; q.smax = fmaxf(q.smax, s); q.cells += cells;
+      *part = q;
+    } else {
+      *part = Part{o, c, f, s, cells};
+    }
+  }
   return __any(stir);
 }
 
@@ -968,9 +977,10 @@ __device__ __forceinline__ bool strip_interior(const DevInst &I, const int4 wk) 
 
 template <int STAGE, bool LTS, bool HZ, bool EXT>
 __device__ __forceinline__ bool run_strip_any(const DevInst &I, const int4 wk, const int lane, const int m, const float dt,
-                                              Part *__restrict__ part, const int cells, const float tnow) {
-  if (strip_interior(I, wk)) return run_strip<STAGE, LTS, HZ, EXT, true>(I, wk, lane, m, dt, part, cells, tnow);
-  return run_strip<STAGE, LTS, HZ, EXT, false>(I, wk, lane, m, dt, part, cells, tnow);
+                                              Part *__restrict__ part, const int cells, const float tnow,
+                                              const bool acc = false) {
+  if (strip_interior(I, wk)) return run_strip<STAGE, LTS, HZ, EXT, true>(I, wk, lane, m, dt, part, cells, tnow, acc);
+  return run_strip<STAGE, LTS, HZ, EXT, false>(I, wk, lane, m, dt, part, cells, tnow, acc);
 }
 
 // ------------------------------------------------------------ dry strips (global stepping)
@@ -1008,70 +1018,82 @@ __device__ __forceinline__ bool strip_begin(const DevInst &I, const int4 wk, int
 }
 
 // ------------------------------------------------------------ LTS record work
-// A tile takes part in the record of level m when it steps at m, or waits at m + 1 beside a face
-// neighbour at m (the unit rule of sw_lts_units).
-template <bool WAITING_ONLY = false>
-__device__ __forceinline__ bool tile_relevant(const DevInst &I, int a, int b, int m) {
-  if (a < 0 || a >= I.tx || b < 0 || b >= I.ty) return false;
-  const uint8_t *lv = I.lv;
-  const int l = lv[b * I.tx + a];
-  if (l == m) return !WAITING_ONLY;
-  if (l != m + 1) return false;
-  return (a > 0 && lv[b * I.tx + a - 1] == m) || (a + 1 < I.tx && lv[b * I.tx + a + 1] == m) ||
-         (b > 0 && lv[(b - 1) * I.tx + a] == m) || (b + 1 < I.ty && lv[(b + 1) * I.tx + a] == m);
-}
+// A level-m record owns, in each work unit (56 columns x up to 64 rows, whole 8x8 tiles), the rows that
+// hold a cell it advances or a cell that gathers: every row of a tile row with an open tile at m or an
+// open tile at m + 1 whose west or east neighbour is at m (its cells next to that face gather on every
+// row), and the first or last row of an open tile at m + 1 whose south or north neighbour is at m (only
+// that row gathers). Tiles without open cells have nothing to advance or gather; a wall tile at m still
+// makes the open tiles beside it gather (its faces pass the walls' pressure). Gaps of up to 2 kHalo rows
+// between owned rows are owned too (walking them costs no more than the halo rows of a second part).
+// Each run of owned rows, cut into parts of at most FS_LTS_PART rows, is one wavefront's walk of its rows
+// plus kHalo halo rows each side (sw_lts_units, sw_lts_lists). A record that has few owned rows splits its
+// runs into parts of 1, 2, 4, 8, 16 or 32 rows, the finest that keeps the record within kLtsFineWaves
+// wavefronts, so its parts fill the chip with short walks (its latency is the longest walk). A unit holds
+// at most kLtsUnitRuns parts for any part size of 8 rows or more (runs are at least 7 rows apart).
+constexpr int kLtsWaveCap = 2;        // wavefronts a record launches per work unit (the partials it has)
+constexpr int kLtsFineWaves = 4096;   // parts a finely split record may have
+constexpr int kLtsUnitRuns = 8;       // parts a unit can hold (64 rows, runs at least 7 apart, parts >= 8 rows)
+constexpr int kLtsCand = 7;           // part sizes 1, 2, 4, 8, 16, 32 rows, then whole runs
+// the most rows a part of a record that is not split finely takes (64: whole runs; measured at 8192^2, LTS cycle
+// 16.68 ms with 64, 16.60 with 32, 19.11 with 16)
+#ifndef FS_LTS_PART
+#define FS_LTS_PART 32
+#endif
+constexpr int kLtsBigCand = FS_LTS_PART >= 64 ? kLtsCand - 1 : FS_LTS_PART >= 32 ? 5 : FS_LTS_PART >= 16 ? 4 : 3;
 
-// A record's wavefronts: every unit on its list is split into parts, one wavefront each (at most
-// kLtsWaveCap per unit of the domain, the grid of a record launch, and the partials it has room for):
-// the records of the fine levels, whose few units would otherwise each walk their rows alone, then
-// fill the chip, and every part is cut to the rows that take part. A record of at most 4096 / 16 units
-// splits them into parts of 4, 2 or 1 rows (a wavefront's walk is its rows plus 6 halo rows, so the
-// latency of a small record shrinks with its parts); larger ones into 2, 4 or 8 parts of whole tile rows.
-constexpr int kLtsWaveCap = 2;
-constexpr int kLtsFineWaves = 4096;
-__device__ __forceinline__ int lts_split(const DevInst &I, int m) {
-  const int n = I.rec->lcount[m], nu = I.wend - I.wbeg;
-  const int cap = min(kLtsFineWaves, kLtsWaveCap * nu);
-  for (int s = 64; s > 8; s >>= 1)
-    if (n * s <= cap) return s;
-  // parts of 8, 16 or 32 rows: a list of up to 1/8 of the units, up to 1/4, or more
-  return n * 8 <= nu ? 8 : n * 4 <= nu ? 4 : 2;
-}
-
-// The rows one wavefront of a level-m record walks: part `sub` of `split` parts of the unit, cut
-// to the span from its first to its last tile row holding a tile that takes part (w = 0: none);
-// WAITING_ONLY: a tile that waits and gathers (the fold of the record's gathers). Parts of at most
-// 4 rows (rp rows, rp = 1, 2 or 4, the smallest with split * rp >= the unit's rows) lie inside one
-// tile row and take part when a tile of that row does.
-template <bool WAITING_ONLY = false>
-__device__ __forceinline__ int4 lts_rows(const DevInst &I, int4 wk, int m, int lane, int split, int sub) {
-  const int tx0 = wk.y / kLtsTile, ty0 = wk.z / kLtsTile;
-  const int ntx = min(I.own / kLtsTile, I.tx - tx0);
-  const int nty = min((wk.w + kLtsTile - 1) / kLtsTile, I.ty - ty0);   // tile rows of the unit
-  const int need = (I.rows + split - 1) / split;
-  if (need <= kLtsTile / 2) {
-    const int rp = need <= 1 ? 1 : need <= 2 ? 2 : 4;
-    const int z = sub * rp, r = z / kLtsTile;
-    const int c = lane & 7;
-    const bool rel = lane < 8 && c < ntx && z < wk.w && wk.z + z < I.ny && r < nty &&
-                     tile_relevant<WAITING_ONLY>(I, tx0 + c, ty0 + r, m);
-    if (!__ballot(rel)) { wk.w = 0; return wk; }
-    wk.w = min(rp, wk.w - z);
-    wk.z += z;
-    return wk;
+// fill the gaps of at most 2 kHalo clear bits between set bits of a unit's row mask
+__device__ __forceinline__ uint64_t lts_fill_gaps(uint64_t msk) {
+  uint64_t out = msk;
+  int pos = 0;
+  while (pos < 64) {
+    const uint64_t rest = msk >> pos;
+    if (!rest) break;
+    const int s = pos + __builtin_ctzll(rest);                              // a run starts at s
+    const uint64_t from = msk >> s;
+    const int len = ~from ? __builtin_ctzll(~from) : 64 - s;                // and ends at e
+    const int e = s + len;
+    if (e >= 64) break;
+    const uint64_t after = msk >> e;
+    if (!after) break;
+    const int g = __builtin_ctzll(after);                                   // clear rows before the next run
+    if (g <= 2 * kHalo) out |= ((1ull << g) - 1ull) << e;
+    pos = e + g;
   }
-  const int per = ((wk.w + kLtsTile - 1) / kLtsTile + split - 1) / split;  // tile rows per part
-  const int r0 = sub * per, r1 = min(nty, r0 + per);
-  const int c = lane & 7, r = r0 + (lane >> 3);
-  const bool rel = c < ntx && r < r1 && tile_relevant<WAITING_ONLY>(I, tx0 + c, ty0 + r, m);
-  const uint64_t bits = __ballot(rel);
-  int first = -1, last = -1;
-  for (int q = 0; q < 8; ++q)
-    if ((bits >> (8 * q)) & 0xffull) { if (first < 0) first = q; last = q; }
-  if (first < 0) { wk.w = 0; return wk; }
-  wk.z += kLtsTile * (r0 + first);
-  wk.w = kLtsTile * (last - first + 1);
-  return wk;
+  return out;
+}
+
+// the runs of a row mask: calls f(first row, rows) for each, in row order
+template <typename F>
+__device__ __forceinline__ void lts_runs(uint64_t msk, F &&f) {
+  int pos = 0;
+  while (pos < 64) {
+    const uint64_t rest = msk >> pos;
+    if (!rest) break;
+    const int s = pos + __builtin_ctzll(rest);
+    const uint64_t from = msk >> s;
+    const int len = ~from ? __builtin_ctzll(~from) : 64 - s;
+    f(s, len);
+    pos = s + len;
+  }
+}
+
+// part size of candidate c (rows; candidate kLtsCand - 1: whole runs)
+__device__ __forceinline__ int lts_cand_rows(int c) { return c < kLtsCand - 1 ? 1 << c : 64; }
+
+// a waiting tile (m + 1, open cells) beside a tile at m: the fold's relevance test
+__device__ __forceinline__ bool tile_waits(const DevInst &I, int a, int b, int m) {
+  if (a < 0 || a >= I.tx || b < 0 || b >= I.ty) return false;
+  const int t = b * I.tx + a;
+  const uint8_t *lv = I.lv;
+  if (lv[t] != m + 1 || I.tileOpen[t] == 0) return false;
+  return (a > 0 && lv[t - 1] == m) || (a + 1 < I.tx && lv[t + 1] == m) || (b > 0 && lv[t - I.tx] == m) ||
+         (b + 1 < I.ty && lv[t + I.tx] == m);
+}
+
+// wavefronts of a level-m record: one per part, at most kLtsWaveCap per unit (the launch grid); a wavefront
+// takes parts w, w + G, w + 2 G, ... and sums their books into one partial
+__device__ __forceinline__ int lts_waves(const DevInst &I, int m) {
+  return min(I.rec->lcount[m], kLtsWaveCap * (I.wend - I.wbeg));
 }
 
 // end time of the record at base step kk of the current cycle (the last one ends the cycle exactly;
@@ -1081,30 +1103,27 @@ __device__ __forceinline__ float lts_record_end_time(const DevInst &I, int kk) {
   return (float)(kk + 1 == (1 << I.L) ? rec->cycT0 + rec->cycD : rec->cycT0 + (double)(kk + 1) * rec->base);
 }
 
-// LTS record of level m (single-instance groups): the parts of the level's units (lts_split), whose
-// number only the device knows; the grid covers kLtsWaveCap per unit and waves without a part exit.
+// LTS record of level m (single-instance groups): the parts of the level's list
 template <int STAGE, bool HZ, bool EXT = false>
-__device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
-                                               Part *__restrict__ part, int m) {
+__device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts, Part *__restrict__ part, int m) {
   const int lane = threadIdx.x & 63;
   const DevInst &I = insts[0];
   if (STAGE == 1 && !I.secondOrder) return;
   const int kk = m >> 8;   // the record's base step in its cycle (second stage: for the hazard clock)
   m &= 0xff;
   const float dt = I.rec->ldt[m];
-  const int split = lts_split(I, m);
-  const int i = xcd_wave_index_n(I.rec->lcount[m] * split, kLtsWaves);
-  if (i < 0) return;
-  int4 wk = {0, 0, 0, 0};
+  const int np = I.rec->lcount[m], G = lts_waves(I, m);
+  const int w = xcd_wave_index_n(G, kLtsWaves);
+  if (w < 0) return;
+  bool acc = false;
   if (dt > 0.f) {
-    const int u = I.ulist[m * (I.wend - I.wbeg) + i / split];
-    wk = lts_rows(I, work[I.wbeg + u], m, lane, split, i % split);
+    const float tend = lts_record_end_time(I, kk);
+    for (int i = w; i < np; i += G) {
+      run_strip_any<STAGE, true, HZ, EXT>(I, I.plist[m * I.capParts + i], lane, m, dt, part + w, 0, tend, acc);
+      acc = true;
+    }
   }
-  if (wk.w == 0) {
-    if (lane == 0) part[i] = Part{0.0, 0.0, 0.0, 0.f, 0};
-    return;
-  }
-  run_strip_any<STAGE, true, HZ, EXT>(I, wk, lane, m, dt, part + i, 0, lts_record_end_time(I, kk));
+  if (!acc && lane == 0) part[w] = Part{0.0, 0.0, 0.0, 0.f, 0};
 }
 
 // One RK stage over a work list. Global stepping: one wavefront per strip of `work`; a strip the
@@ -1152,40 +1171,40 @@ __global__ __launch_bounds__(256) void sw_stage(const DevInst *__restrict__ inst
 template <>
 __global__ __launch_bounds__(256, FS_LTS_S1_WAVES) void sw_stage<0, true, false>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
-  (void)nwork;
-  lts_stage_body<0, false>(insts, work, part, m);
+  (void)nwork; (void)work;
+  lts_stage_body<0, false>(insts, part, m);
 }
 template <>
 __global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, false>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
-  (void)nwork;
-  lts_stage_body<1, false>(insts, work, part, m);
+  (void)nwork; (void)work;
+  lts_stage_body<1, false>(insts, part, m);
 }
 // with the hazard products (fs_params.hazard)
 template <>
 __global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, true>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
-  (void)nwork;
-  lts_stage_body<1, true>(insts, work, part, m);
+  (void)nwork; (void)work;
+  lts_stage_body<1, true>(insts, part, m);
 }
 // with flood barriers (extended faces)
 template <>
 __global__ __launch_bounds__(256, FS_LTS_S1_WAVES) void sw_stage<0, true, false, true>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
-  (void)nwork;
-  lts_stage_body<0, false, true>(insts, work, part, m);
+  (void)nwork; (void)work;
+  lts_stage_body<0, false, true>(insts, part, m);
 }
 template <>
 __global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, false, true>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
-  (void)nwork;
-  lts_stage_body<1, false, true>(insts, work, part, m);
+  (void)nwork; (void)work;
+  lts_stage_body<1, false, true>(insts, part, m);
 }
 template <>
 __global__ __launch_bounds__(256, FS_LTS_S2_WAVES) void sw_stage<1, true, true, true>(
     const DevInst *__restrict__ insts, const int4 *__restrict__ work, int nwork, Part *__restrict__ part, int m) {
-  (void)nwork;
-  lts_stage_body<1, true, true>(insts, work, part, m);
+  (void)nwork; (void)work;
+  lts_stage_body<1, true, true>(insts, part, m);
 }
 
 // Barrier books after each RK stage, one thread per barrier face (the barrier kernel,
@@ -1247,53 +1266,64 @@ __global__ __launch_bounds__(64) void sw_barrier_books(const DevInst *__restrict
 // (compensated, as kadd on U0 in ltsUpdate, kernels.wgsl.js:518)
 __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
                                                    int m) {
+  (void)work;
   const DevInst &I = insts[0];
   const int lane = threadIdx.x & 63;
-  const int split = lts_split(I, m);
-  const int i = xcd_wave_index_n(I.rec->lcount[m] * split, kLtsWaves);
-  const int nu = I.wend - I.wbeg;
+  const int np = I.rec->lcount[m], G = lts_waves(I, m);
+  const int w = xcd_wave_index_n(G, kLtsWaves);
+  if (w < 0) return;
   const bool two = I.secondOrder;
-  if (i >= 0) {
-    const int4 wk = lts_rows<true>(I, work[I.wbeg + I.ulist[m * nu + i / split]], m, lane, split, i % split);
+  for (int i = w; i < np; i += G) {
+    const int4 wk = I.plist[m * I.capParts + i];
     const int X = wk.y - I.hl + lane;
-    if (lane < I.hl || lane >= I.hl + I.own || X >= I.nx) return;
-    // per tile row (8 rows): the gather flags, then every flagged cell's store and gathers, all issued before
-    // any is used, so the walk waits for memory twice per tile row rather than once per flagged row
-    const int yend = min(wk.z + wk.w, I.ny);
-    for (int yb = wk.z; yb < yend; yb += kLtsTile) {
-      uint8_t f[kLtsTile];
-      // (rows past the walk load the last row's flag, then drop it: the loads need no branch)
+    const bool mine = lane >= I.hl && lane < I.hl + I.own && X < I.nx;
+    const int tx0 = wk.y / kLtsTile, ntx = min(I.own / kLtsTile, I.tx - tx0);
+    const int yend = min(wk.z + wk.w, I.ny), tb0 = wk.z / kLtsTile, ntb = (yend - 1) / kLtsTile - tb0 + 1;
+    // only a tile that waits beside a tile at m gathers: which tile rows of the part hold one (lane = tile
+    // row * 8 + tile column; a part lies in one unit, at most 8 tile rows)
+    const bool relt = (lane & 7) < ntx && (lane >> 3) < ntb && tile_waits(I, tx0 + (lane & 7), tb0 + (lane >> 3), m);
+    const uint64_t rel = __ballot(relt);
+    if (!rel || !mine) continue;
+    // per tile row that holds one: the gather flags, then every flagged cell's store and gathers, all issued
+    // before any is used, so the walk waits for memory twice per tile row rather than once per flagged row
+    for (int yb = wk.z; yb < yend;) {
+      const int ye = min(yend, (yb / kLtsTile + 1) * kLtsTile);
+      if ((rel >> (8 * (yb / kLtsTile - tb0))) & 0xffull) {
+        uint8_t f[kLtsTile];
+        // (rows past the tile row's end load its last row's flag, then drop it: the loads need no branch)
 #pragma unroll
-      for (int q = 0; q < kLtsTile; ++q) f[q] = gld(I.gflag, min(yb + q, yend - 1) * I.pitch + X);
+        for (int q = 0; q < kLtsTile; ++q) f[q] = gld(I.gflag, min(yb + q, ye - 1) * I.pitch + X);
 #pragma unroll
-      for (int q = 0; q < kLtsTile; ++q) f[q] = yb + q < yend ? f[q] : (uint8_t)0;
-      float sh[kLtsTile], sw[kLtsTile], shu[kLtsTile], shv[kLtsTile], a[kLtsTile], au[kLtsTile], av[kLtsTile];
-      float b[kLtsTile], bu[kLtsTile], bv[kLtsTile];
+        for (int q = 0; q < kLtsTile; ++q) f[q] = yb + q < ye ? f[q] : (uint8_t)0;
+        float sh[kLtsTile], sw[kLtsTile], shu[kLtsTile], shv[kLtsTile], a[kLtsTile], au[kLtsTile], av[kLtsTile];
+        float b[kLtsTile], bu[kLtsTile], bv[kLtsTile];
 #pragma unroll
-      for (int q = 0; q < kLtsTile; ++q) {
-        sh[q] = sw[q] = shu[q] = shv[q] = a[q] = au[q] = av[q] = b[q] = bu[q] = bv[q] = 0.f;
-        if (f[q]) {
+        for (int q = 0; q < kLtsTile; ++q) {
+          sh[q] = sw[q] = shu[q] = shv[q] = a[q] = au[q] = av[q] = b[q] = bu[q] = bv[q] = 0.f;
+          if (f[q]) {
+            const int k = (yb + q) * I.pitch + X;
+            sh[q] = gld(I.sh, k); sw[q] = gld(I.sw, k); shu[q] = gld(I.shu, k); shv[q] = gld(I.shv, k);
+            a[q] = gld(I.g0h, k); au[q] = gld(I.g0hu, k); av[q] = gld(I.g0hv, k);
+            if (two) { b[q] = gld(I.g1h, k); bu[q] = gld(I.g1hu, k); bv[q] = gld(I.g1hv, k); }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kLtsTile; ++q) {
+          if (!f[q]) continue;
           const int k = (yb + q) * I.pitch + X;
-          sh[q] = gld(I.sh, k); sw[q] = gld(I.sw, k); shu[q] = gld(I.shu, k); shv[q] = gld(I.shv, k);
-          a[q] = gld(I.g0h, k); au[q] = gld(I.g0hu, k); av[q] = gld(I.g0hv, k);
-          if (two) { b[q] = gld(I.g1h, k); bu[q] = gld(I.g1hu, k); bv[q] = gld(I.g1hv, k); }
+          float h = sh[q], w = sw[q];
+          kadd(h, w, a[q]);
+          float hu = shu[q] + au[q], hv = shv[q] + av[q];
+          if (two) {
+            kadd(h, w, b[q]);
+            hu += bu[q];
+            hv += bv[q];
+          }
+          gst(I.sh, k, h); gst(I.sw, k, w); gst(I.shu, k, hu); gst(I.shv, k, hv);
+          gst(I.gflag, k, (uint8_t)0);
         }
       }
-#pragma unroll
-      for (int q = 0; q < kLtsTile; ++q) {
-        if (!f[q]) continue;
-        const int k = (yb + q) * I.pitch + X;
-        float h = sh[q], w = sw[q];
-        kadd(h, w, a[q]);
-        float hu = shu[q] + au[q], hv = shv[q] + av[q];
-        if (two) {
-          kadd(h, w, b[q]);
-          hu += bu[q];
-          hv += bv[q];
-        }
-        gst(I.sh, k, h); gst(I.sw, k, w); gst(I.shu, k, hu); gst(I.shv, k, hv);
-        gst(I.gflag, k, (uint8_t)0);
-      }
+      yb = ye;
     }
   }
 }
@@ -1402,9 +1432,8 @@ __global__ __launch_bounds__(256) void sw_lts_smooth(const DevInst *__restrict__
   I.lv[t] = (uint8_t)lv;
 }
 
-// per work unit (whole tiles): the levels whose records it takes part in -- its own tiles' levels,
-// and level l - 1 for a tile at l with a face neighbour at l - 1 (it waits and gathers) -- and its
-// own open cells per level
+// per work unit (whole tiles): the rows a record of each level owns in it (the rule at lts_fill_gaps, above
+// the LTS record work) and its own open cells per level
 __global__ __launch_bounds__(256) void sw_lts_units(const DevInst *__restrict__ insts, const int4 *__restrict__ work) {
   const DevInst &I = insts[0];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1414,65 +1443,136 @@ __global__ __launch_bounds__(256) void sw_lts_units(const DevInst *__restrict__ 
   const int tx0 = wk.y / kLtsTile, tx1 = min(I.tx, (wk.y + I.own) / kLtsTile);
   const int ty0 = wk.z / kLtsTile, ty1 = min(I.ty, (wk.z + wk.w + kLtsTile - 1) / kLtsTile);
   int cells[kMaxLevels + 1] = {0, 0, 0, 0, 0};
-  unsigned mask = 0;
+  // per level, bit q = tile row q of the unit: every row owned, its first row owned, its last row owned
+  unsigned full[kMaxLevels + 1] = {0, 0, 0, 0, 0}, first[kMaxLevels + 1] = {0, 0, 0, 0, 0};
+  unsigned last[kMaxLevels + 1] = {0, 0, 0, 0, 0};
+  const uint8_t *lv = I.lv;
   for (int b = ty0; b < ty1; ++b)
     for (int a = tx0; a < tx1; ++a) {
-      const int l = I.lv[b * I.tx + a];
-      mask |= 1u << l;
-      cells[l] += I.tileOpen[b * I.tx + a];
-      if (l > 0) {
-        const bool fine = (a > 0 && I.lv[b * I.tx + a - 1] == l - 1) || (a + 1 < I.tx && I.lv[b * I.tx + a + 1] == l - 1) ||
-                          (b > 0 && I.lv[(b - 1) * I.tx + a] == l - 1) || (b + 1 < I.ty && I.lv[(b + 1) * I.tx + a] == l - 1);
-        if (fine) mask |= 1u << (l - 1);
+      const int t = b * I.tx + a, l = lv[t], op = I.tileOpen[t], bit = 1 << (b - ty0);
+      cells[l] += op;
+      if (!op) continue;
+      full[l] |= bit;
+      if (l > 0) {   // waits beside a tile at l - 1 in that level's records
+        const int f = l - 1;
+        if ((a > 0 && lv[t - 1] == f) || (a + 1 < I.tx && lv[t + 1] == f)) full[f] |= bit;
+        if (b > 0 && lv[t - I.tx] == f) first[f] |= bit;
+        if (b + 1 < I.ty && lv[t + I.tx] == f) last[f] |= bit;
       }
     }
-  I.umask[u] = (uint8_t)mask;
-  for (int l = 0; l <= kMaxLevels; ++l) I.ucells[u * (kMaxLevels + 1) + l] = cells[l];
+  const int nr = min(wk.w, I.ny - wk.z);   // rows of the unit (at most 64)
+  const uint64_t in = nr >= 64 ? ~0ull : (1ull << nr) - 1ull;
+  for (int l = 0; l <= kMaxLevels; ++l) {
+    uint64_t msk = 0;
+    for (int q = 0; q < ty1 - ty0; ++q) {
+      if ((full[l] >> q) & 1u) msk |= 0xffull << (kLtsTile * q);
+      if ((first[l] >> q) & 1u) msk |= 1ull << (kLtsTile * q);
+      if ((last[l] >> q) & 1u) msk |= 1ull << (kLtsTile * q + kLtsTile - 1);
+    }
+    I.urows[(size_t)l * nu + u] = l <= I.L ? lts_fill_gaps(msk & in) : 0ull;
+    I.ucells[u * (kMaxLevels + 1) + l] = cells[l];
+  }
 }
 
-// per level: the list of units in unit order (a block-wide scan: deterministic), the own open
+// parts of a unit's row mask for each part size (candidates 1, 2, 4, ... 32 rows, then whole runs)
+__device__ __forceinline__ void lts_count(uint64_t msk, int cnt[kLtsCand]) {
+  lts_runs(msk, [&](int s, int len) {
+    (void)s;
+#pragma unroll
+    for (int c = 0; c < kLtsCand - 1; ++c) cnt[c] += (len + (1 << c) - 1) >> c;
+    cnt[kLtsCand - 1] += 1;
+  });
+}
+
+// per level (one block each): the parts of the level's records in unit order (a block-wide ordered scan:
+// deterministic), with the part size that keeps a record of few owned rows within kLtsFineWaves; the own open
 // cells, and (rain field) the rain multiplier summed over them
-__global__ __launch_bounds__(1024) void sw_lts_lists(const DevInst *__restrict__ insts) {
+__global__ __launch_bounds__(1024) void sw_lts_lists(const DevInst *__restrict__ insts, const int4 *__restrict__ work) {
   const DevInst &I = insts[0];
-  const int nu = I.wend - I.wbeg, T = blockDim.x, tid = threadIdx.x;
-  __shared__ int s_scan[1024];
-  __shared__ long long s_cells[kMaxLevels + 1];
-  const int chunk = (nu + T - 1) / T, u0 = tid * chunk, u1 = min(nu, u0 + chunk);
-  if (tid <= kMaxLevels) s_cells[tid] = 0;
+  const int l = blockIdx.x, nu = I.wend - I.wbeg, T = blockDim.x, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, nwv = T >> 6;
+  __shared__ double s_red[16][kLtsCand + 1];
+  __shared__ int s_off[16];
+  __shared__ int s_cand;
+  if (l > I.L) {
+    if (tid == 0) { I.rec->lcount[l] = 0; I.rec->lcells[l] = 0; }
+    return;
+  }
+  const uint64_t *um = I.urows + (size_t)l * nu;
+  // totals of the part counts of each candidate size, and of the level's own open cells
+  double acc[kLtsCand + 1];
+#pragma unroll
+  for (int c = 0; c <= kLtsCand; ++c) acc[c] = 0.0;
+  for (int u = tid; u < nu; u += T) {
+    int cnt[kLtsCand] = {0, 0, 0, 0, 0, 0, 0};
+    lts_count(um[u], cnt);
+#pragma unroll
+    for (int c = 0; c < kLtsCand; ++c) acc[c] += (double)cnt[c];
+    acc[kLtsCand] += (double)I.ucells[u * (kMaxLevels + 1) + l];
+  }
+#pragma unroll
+  for (int c = 0; c <= kLtsCand; ++c) {
+    const double v = wave_sum(acc[c]);
+    if (lane == 0) s_red[wv][c] = v;
+  }
   __syncthreads();
-  for (int l = 0; l <= I.L; ++l) {
-    int c = 0;
-    long long cl = 0;
-    for (int u = u0; u < u1; ++u) {
-      c += (I.umask[u] >> l) & 1;
-      cl += I.ucells[u * (kMaxLevels + 1) + l];
+  if (tid == 0) {
+    double tot[kLtsCand + 1];
+    for (int c = 0; c <= kLtsCand; ++c) {
+      tot[c] = 0.0;
+      for (int k = 0; k < nwv; ++k) tot[c] += s_red[k][c];
     }
-    s_scan[tid] = c;
+    const double cap = (double)min(kLtsFineWaves, I.capParts);
+    int cand = kLtsBigCand;
+    for (int c = 0; c < kLtsBigCand; ++c)
+      if (tot[c] <= cap) { cand = c; break; }
+    s_cand = cand;
+    I.rec->lcount[l] = (int)tot[cand];
+    I.rec->lcells[l] = (int)tot[kLtsCand];
+  }
+  __syncthreads();
+  const int rp = lts_cand_rows(s_cand);
+  int4 *out = I.plist + (size_t)l * I.capParts;
+  int base = 0;
+  for (int u0 = 0; u0 < nu; u0 += T) {   // chunks of T units, scanned in unit order
+    const int u = u0 + tid;
+    const uint64_t msk = u < nu ? um[u] : 0ull;
+    int n = 0;
+    lts_runs(msk, [&](int s, int len) { (void)s; n += (len + rp - 1) / rp; });
+    int x = n;   // inclusive scan over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_off[wv] = x;
     __syncthreads();
-    for (int o = 1; o < T; o <<= 1) {  // inclusive scan
-      const int v = tid >= o ? s_scan[tid - o] : 0;
-      __syncthreads();
-      s_scan[tid] += v;
-      __syncthreads();
+    int before = base, chunk = 0;
+    for (int k = 0; k < nwv; ++k) {
+      if (k < wv) before += s_off[k];
+      chunk += s_off[k];
     }
-    int pos = s_scan[tid] - c;
-    for (int u = u0; u < u1; ++u)
-      if ((I.umask[u] >> l) & 1) I.ulist[l * nu + pos++] = u;
-    atomicAdd((unsigned long long *)&s_cells[l], (unsigned long long)cl);
+    __syncthreads();   // (s_off is rewritten by the next chunk)
+    if (n) {
+      const int4 wk = work[I.wbeg + u];
+      int pos = before + x - n;
+      lts_runs(msk, [&](int s, int len) {
+        for (int r = 0; r < len; r += rp) out[pos++] = make_int4(wk.x, wk.y, wk.z + s + r, min(rp, len - r));
+      });
+    }
+    base += chunk;
+  }
+  if (I.tileRf) {   // rain multiplier of the level (only with a rain field): fixed-order sums
+    double a = 0.0;
+    for (int t = tid; t < I.tx * I.ty; t += T)
+      if (I.lv[t] == l) a += (double)I.tileRf[t];
+    a = wave_sum(a);
+    if (lane == 0) s_red[wv][0] = a;
     __syncthreads();
     if (tid == 0) {
-      I.rec->lcount[l] = s_scan[T - 1];
-      I.rec->lcells[l] = (int)s_cells[l];
+      double r = 0.0;
+      for (int k = 0; k < nwv; ++k) r += s_red[k][0];
+      I.rec->lrf[l] = r;
     }
-    __syncthreads();
-  }
-  if (tid == 0)
-    for (int l = I.L + 1; l <= kMaxLevels; ++l) { I.rec->lcount[l] = 0; I.rec->lcells[l] = 0; }
-  if (I.tileRf && tid <= I.L) {  // rain multiplier per level (only with a rain field)
-    double acc = 0.0;
-    for (int t = 0; t < I.tx * I.ty; ++t)
-      if (I.lv[t] == tid) acc += (double)I.tileRf[t];
-    I.rec->lrf[tid] = acc;
   }
 }
 
@@ -1483,7 +1583,7 @@ __global__ __launch_bounds__(1024) void sw_lts_record_end(const DevInst *__restr
   StepRec *rec = I.rec;
   const float dt = rec->ldt[m];
   if (!(dt > 0.f)) return;
-  const int n = rec->lcount[m] * lts_split(I, m);   // one partial per wavefront of the record
+  const int n = lts_waves(I, m);   // one partial per wavefront of the record
   __shared__ double s_o[16], s_c[16], s_f[16];
   double o = 0.0, c = 0.0, f = 0.0;
   // unrolled so that several partials are in flight per thread (the sums keep their order)
@@ -1550,7 +1650,7 @@ __global__ __launch_bounds__(256) void sw_speed(const DevInst *__restrict__ inst
 // whose store is then spent (solver-cpu.js:550-557).
 template <bool LTS, bool HZ, bool EXT = false>
 __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 wk, int lane, int m, float dt, Part *part,
-                                                   int cells, float tnow) {
+                                                   int cells, float tnow, bool acc = false) {
   double acc_inf = 0.0;
   float smax = 0.f;
   bool stir = false;
@@ -1573,7 +1673,15 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
   }
   const double f = wave_sum(acc_inf);
   const float s = wave_max(smax);
-  if (lane == 0) *part = Part{0.0, 0.0, f, s, cells};
+  if (lane == 0) {
+    if (acc) {   // a further part of the same wavefront (LTS)
+      Part q = *part;
+      q.infil += f; q.smax = fmaxf(q.smax, s); q.cells += cells;
+      *part = q;
+    } else {
+      *part = Part{0.0, 0.0, f, s, cells};
+    }
+  }
   return __any(stir);
 }
 
@@ -1604,12 +1712,16 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
     const int kk = m >> 8;
     m &= 0xff;
     const float dt = I.rec->ldt[m];
-    const int split = lts_split(I, m);
-    const int i = xcd_wave_index_n(I.rec->lcount[m] * split, kLtsWaves);
-    if (i < 0) return;
-    const int u = I.ulist[m * (I.wend - I.wbeg) + i / split];
-    const int4 wk = lts_rows(I, work[I.wbeg + u], m, lane, split, i % split);
-    euler_finish_strip<true, HZ, EXT>(I, wk, lane, m, wk.w > 0 ? dt : 0.f, part + i, 0, lts_record_end_time(I, kk));
+    const int np = I.rec->lcount[m], G = lts_waves(I, m);
+    const int w = xcd_wave_index_n(G, kLtsWaves);
+    if (w < 0) return;
+    bool acc = false;
+    for (int i = w; i < np; i += G) {
+      euler_finish_strip<true, HZ, EXT>(I, I.plist[m * I.capParts + i], lane, m, dt, part + w, 0,
+                                        lts_record_end_time(I, kk), acc);
+      acc = true;
+    }
+    if (!acc && lane == 0) part[w] = Part{0.0, 0.0, 0.0, 0.f, 0};
   }
 }
 

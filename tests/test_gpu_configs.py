@@ -142,7 +142,13 @@ def test_config3_full_size_properties(fsa):
         g.destroy()
         for k in ("h", "hu", "hv", "hmax"):
             np.testing.assert_array_equal(a[k], b[k])
-        assert a["mass"] == b["mass"]
+        if rows == 64:
+            assert a["mass"] == b["mass"]   # the same work decomposition: the same bits
+        else:
+            # other strip heights group the FP64 per-wavefront partials of the books differently, so a
+            # book may differ in its last bits (the cells' state above is the same bit for bit)
+            for k, v in a["mass"].items():
+                assert abs(v - b["mass"][k]) <= 1e-12 * max(abs(v), 1.0), k
         del b
 
 

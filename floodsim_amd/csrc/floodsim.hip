@@ -56,6 +56,8 @@ struct Slot {
   int L = 0, tx = 0, ty = 0, nunits = 0;
   uint8_t *lv = nullptr, *lv0 = nullptr, *gflag = nullptr;
   uint64_t *urows = nullptr;
+  double *rend = nullptr;
+  unsigned *ticket = nullptr;
   int4 *plist = nullptr;
   float *tsp = nullptr, *tileRf = nullptr;
   unsigned *speedSlots = nullptr;
@@ -230,6 +232,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
     b += 10 * align256(N * 4) + align256(N) + 2 * align256(N * 16);  // store, gathers, flag, bsx, bsy
     b += align256(nu * 8 * (kMaxLevels + 1)) + align256(nu * 4 * (kMaxLevels + 1));   // urows, ucells
     b += align256(nu * kLtsUnitRuns * 16 * (kMaxLevels + 1));                         // plist
+    b += align256(kLtsEndSlices * 3 * 8) + align256(4);                               // rend, ticket
   }
   if (!p->lts) b += 3 * align256(strip_cap(p)) + align256(strip_cap(p) * 4);  // strip flags
   if (p->hazard) b += 2 * align256(N * 16);  // hazard products
@@ -331,6 +334,7 @@ DevInst devinst_of(const Slot &S, StepRec *rec, int wbeg, int wend) {
   d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.g1h = S.g1h; d.g1hu = S.g1hu; d.g1hv = S.g1hv;
   d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
   d.urows = S.urows; d.ucells = S.ucells; d.plist = S.plist; d.capParts = kLtsUnitRuns * S.nunits;
+  d.rend = S.rend; d.ticket = S.ticket;
   d.crest = S.crest; d.bidx = S.bidx; d.barFace = S.barFace; d.barGround = S.barGround;
   d.barFlux = S.barFlux; d.barBook = S.barBook; d.nbar = S.nbar;
   d.soil = S.soil; d.gi = S.gi;
@@ -606,9 +610,8 @@ int launch_cycle(fs_pool *P, const Launch &G) {
         HIPCHK(P, hipEventRecord(sb, P->stream));
         P->spans.push_back({sa, sb, cyc, m});
       }
-      sw_lts_fold<<<nbp, kT, 0, P->stream>>>(G.dinst, G.dwork, m);
-      if (d.nin) sw_exchange<true><<<(d.nin + 63) / 64, 64, 0, P->stream>>>(G.dinst, 0, m);
-      sw_lts_record_end<<<1, 1024, 0, P->stream>>>(G.dinst, G.p1, G.p2, m);
+      // gathers folded, books, inlet exchange (one kernel)
+      sw_lts_fold<<<nbp, kT, 0, P->stream>>>(G.dinst, G.p1, G.p2, m);
     }
   sw_lts_cycle_end<<<1, 1, 0, P->stream>>>(G.dinst);
   if (P->timing) HIPCHK(P, hipEventRecord(e2, P->stream));
@@ -972,6 +975,8 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     S.urows = (uint64_t *)take((size_t)S.nunits * 8 * (kMaxLevels + 1));
     S.ucells = (int32_t *)take((size_t)S.nunits * 4 * (kMaxLevels + 1));
     S.plist = (int4 *)take((size_t)S.nunits * kLtsUnitRuns * 16 * (kMaxLevels + 1));
+    S.rend = (double *)take((size_t)kLtsEndSlices * 3 * 8);
+    S.ticket = (unsigned *)take(4);
   } else {
     S.sxn = (nx + strip_own(p) - 1) / strip_own(p);
     S.nstrips = units_of(p);
@@ -1018,6 +1023,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     HIPCHK(P, hipMemset(S.sh, 0, (size_t)((char *)S.bsx - (char *)S.sh)));      // store, gathers, flags
     HIPCHK(P, hipMemset(S.bsx, 0, N * 16));   // each plane on its own: take() pads them apart
     HIPCHK(P, hipMemset(S.bsy, 0, N * 16));
+    HIPCHK(P, hipMemset(S.ticket, 0, 4));     // (each record ends with it at 0 again)
     HIPCHK(P, hipDeviceSynchronize());
   }
   double rainArea = 0.0;

@@ -162,6 +162,8 @@ struct DevInst {
   int4 *plist;                  // per level (stride capParts): the parts of the level's records, (0, first
                                 // column, first row, rows) as a work item
   int capParts;                 // parts a level's list holds (8 per unit)
+  double *rend;                 // a record's books per slice of its partials (3 x kLtsEndSlices)
+  unsigned *ticket;             // slices summed so far in the current record (0 between records)
   // flood barriers (null: none). Per cell: crest and weir coefficient of its west face (x) and south face
   // (y), crest relative to the bed minimum (-1e30 = none); the barrier index of those faces (-1 = none).
   // Per barrier: (cell k of its right/upper side, k of the other side, axis 0 = x face / 1 = y face, 0),

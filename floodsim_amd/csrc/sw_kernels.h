@@ -1288,70 +1288,134 @@ __global__ __launch_bounds__(64) void sw_barrier_books(const DevInst *__restrict
 }
 
 // ------------------------------------------------------------ local time stepping, per record
-// after the stages of a level-m record: a waiting cell's store takes the record's gathered flux
-// (compensated, as kadd on U0 in ltsUpdate, kernels.wgsl.js:518)
-__global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ insts, const int4 *__restrict__ work,
-                                                   int m) {
-  (void)work;
+template <bool LTS>
+__device__ __forceinline__ void exchange_inlet(const DevInst &I, const int j, const int m);
+
+// blocks of the fold kernel that sum a slice of the record's partials each (kLtsEndSlices at most)
+constexpr int kLtsEndSlices = 32;
+
+// After the stages of a level-m record, one kernel ends it:
+// - a waiting cell's store takes the record's gathered flux (compensated, as kadd on U0 in ltsUpdate,
+//   kernels.wgsl.js:518);
+// - the record's books: the first blocks each sum a fixed slice of the stages' per-wavefront partials in a
+//   fixed order, and the last of them to finish (a ticket) sums the slices in slice order into the FP64
+//   books, so the result does not depend on which block ends last;
+// - that block then exchanges the inlets whose cell is at the level (exchangeL, kernels.wgsl.js:596).
+__global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
+                                                   const Part *__restrict__ p2, int m) {
   const DevInst &I = insts[0];
   const int lane = threadIdx.x & 63;
   const int np = I.rec->lcount[m], G = lts_waves(I, m);
   const int w = xcd_wave_index_n(G, kLtsWaves);
-  if (w < 0) return;
   const bool two = I.secondOrder;
-  for (int i = w; i < np; i += G) {
-    const int4 wk = I.plist[m * I.capParts + i];
-    const int X = wk.y - I.hl + lane;
-    const bool mine = lane >= I.hl && lane < I.hl + I.own && X < I.nx;
-    const int tx0 = wk.y / kLtsTile, ntx = min(I.own / kLtsTile, I.tx - tx0);
-    const int yend = min(wk.z + wk.w, I.ny), tb0 = wk.z / kLtsTile, ntb = (yend - 1) / kLtsTile - tb0 + 1;
-    // only a tile that waits beside a tile at m gathers: which tile rows of the part hold one (lane = tile
-    // row * 8 + tile column; a part lies in one unit, at most 8 tile rows)
-    const bool relt = (lane & 7) < ntx && (lane >> 3) < ntb && tile_waits(I, tx0 + (lane & 7), tb0 + (lane >> 3), m);
-    const uint64_t rel = __ballot(relt);
-    if (!rel || !mine) continue;
-    // per tile row that holds one: the gather flags, then every flagged cell's store and gathers, all issued
-    // before any is used, so the walk waits for memory twice per tile row rather than once per flagged row
-    for (int yb = wk.z; yb < yend;) {
-      const int ye = min(yend, (yb / kLtsTile + 1) * kLtsTile);
-      if ((rel >> (8 * (yb / kLtsTile - tb0))) & 0xffull) {
-        uint8_t f[kLtsTile];
-        // (rows past the tile row's end load its last row's flag, then drop it: the loads need no branch)
+  if (w >= 0) {
+    for (int i = w; i < np; i += G) {
+      const int4 wk = I.plist[m * I.capParts + i];
+      const int X = wk.y - I.hl + lane;
+      const bool mine = lane >= I.hl && lane < I.hl + I.own && X < I.nx;
+      const int tx0 = wk.y / kLtsTile, ntx = min(I.own / kLtsTile, I.tx - tx0);
+      const int yend = min(wk.z + wk.w, I.ny), tb0 = wk.z / kLtsTile, ntb = (yend - 1) / kLtsTile - tb0 + 1;
+      // only a tile that waits beside a tile at m gathers: which tile rows of the part hold one (lane = tile
+      // row * 8 + tile column; a part lies in one unit, at most 8 tile rows)
+      const bool relt = (lane & 7) < ntx && (lane >> 3) < ntb && tile_waits(I, tx0 + (lane & 7), tb0 + (lane >> 3), m);
+      const uint64_t rel = __ballot(relt);
+      if (!rel || !mine) continue;
+      // per tile row that holds one: the gather flags, then every flagged cell's store and gathers, all issued
+      // before any is used, so the walk waits for memory twice per tile row rather than once per flagged row
+      for (int yb = wk.z; yb < yend;) {
+        const int ye = min(yend, (yb / kLtsTile + 1) * kLtsTile);
+        if ((rel >> (8 * (yb / kLtsTile - tb0))) & 0xffull) {
+          uint8_t f[kLtsTile];
+          // (rows past the tile row's end load its last row's flag, then drop it: the loads need no branch)
 #pragma unroll
-        for (int q = 0; q < kLtsTile; ++q) f[q] = gld(I.gflag, min(yb + q, ye - 1) * I.pitch + X);
+          for (int q = 0; q < kLtsTile; ++q) f[q] = gld(I.gflag, min(yb + q, ye - 1) * I.pitch + X);
 #pragma unroll
-        for (int q = 0; q < kLtsTile; ++q) f[q] = yb + q < ye ? f[q] : (uint8_t)0;
-        float sh[kLtsTile], sw[kLtsTile], shu[kLtsTile], shv[kLtsTile], a[kLtsTile], au[kLtsTile], av[kLtsTile];
-        float b[kLtsTile], bu[kLtsTile], bv[kLtsTile];
+          for (int q = 0; q < kLtsTile; ++q) f[q] = yb + q < ye ? f[q] : (uint8_t)0;
+          float sh[kLtsTile], sw[kLtsTile], shu[kLtsTile], shv[kLtsTile], a[kLtsTile], au[kLtsTile], av[kLtsTile];
+          float b[kLtsTile], bu[kLtsTile], bv[kLtsTile];
 #pragma unroll
-        for (int q = 0; q < kLtsTile; ++q) {
-          sh[q] = sw[q] = shu[q] = shv[q] = a[q] = au[q] = av[q] = b[q] = bu[q] = bv[q] = 0.f;
-          if (f[q]) {
+          for (int q = 0; q < kLtsTile; ++q) {
+            sh[q] = sw[q] = shu[q] = shv[q] = a[q] = au[q] = av[q] = b[q] = bu[q] = bv[q] = 0.f;
+            if (f[q]) {
+              const int k = (yb + q) * I.pitch + X;
+              sh[q] = gld(I.sh, k); sw[q] = gld(I.sw, k); shu[q] = gld(I.shu, k); shv[q] = gld(I.shv, k);
+              a[q] = gld(I.g0h, k); au[q] = gld(I.g0hu, k); av[q] = gld(I.g0hv, k);
+              if (two) { b[q] = gld(I.g1h, k); bu[q] = gld(I.g1hu, k); bv[q] = gld(I.g1hv, k); }
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < kLtsTile; ++q) {
+            if (!f[q]) continue;
             const int k = (yb + q) * I.pitch + X;
-            sh[q] = gld(I.sh, k); sw[q] = gld(I.sw, k); shu[q] = gld(I.shu, k); shv[q] = gld(I.shv, k);
-            a[q] = gld(I.g0h, k); au[q] = gld(I.g0hu, k); av[q] = gld(I.g0hv, k);
-            if (two) { b[q] = gld(I.g1h, k); bu[q] = gld(I.g1hu, k); bv[q] = gld(I.g1hv, k); }
+            float h = sh[q], w = sw[q];
+            kadd(h, w, a[q]);
+            float hu = shu[q] + au[q], hv = shv[q] + av[q];
+            if (two) {
+              kadd(h, w, b[q]);
+              hu += bu[q];
+              hv += bv[q];
+            }
+            gst(I.sh, k, h); gst(I.sw, k, w); gst(I.shu, k, hu); gst(I.shv, k, hv);
+            gst(I.gflag, k, (uint8_t)0);
           }
         }
-#pragma unroll
-        for (int q = 0; q < kLtsTile; ++q) {
-          if (!f[q]) continue;
-          const int k = (yb + q) * I.pitch + X;
-          float h = sh[q], w = sw[q];
-          kadd(h, w, a[q]);
-          float hu = shu[q] + au[q], hv = shv[q] + av[q];
-          if (two) {
-            kadd(h, w, b[q]);
-            hu += bu[q];
-            hv += bv[q];
-          }
-          gst(I.sh, k, h); gst(I.sw, k, w); gst(I.shu, k, hu); gst(I.shv, k, hv);
-          gst(I.gflag, k, (uint8_t)0);
-        }
+        yb = ye;
       }
-      yb = ye;
     }
   }
+
+  // ---- the record's end
+  StepRec *rec = I.rec;
+  const float dt = rec->ldt[m];
+  const int S = min(kLtsEndSlices, (int)gridDim.x), b = blockIdx.x;
+  if (b >= S || !(dt > 0.f)) return;
+  const int n = G;   // one partial per wavefront of the record
+  const int k0 = (int)((long long)n * b / S), k1 = (int)((long long)n * (b + 1) / S);
+  double o = 0.0, c = 0.0, f = 0.0;
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += blockDim.x) {
+    const Part a = p1[k];
+    o += a.out;
+    c += a.clip;
+    if (two) {
+      const Part q = p2[k];
+      o += q.out;
+      c += q.clip;
+      f += q.infil;
+    } else f += p2[k].infil;
+  }
+  o = wave_sum(o); c = wave_sum(c); f = wave_sum(f);
+  __shared__ double s_r[3 * (256 / 64)];
+  __shared__ int s_last;
+  const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  if (lane == 0) { s_r[3 * wv] = o; s_r[3 * wv + 1] = c; s_r[3 * wv + 2] = f; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < nwv; ++q) { o += s_r[3 * q]; c += s_r[3 * q + 1]; f += s_r[3 * q + 2]; }
+    I.rend[3 * b] = o; I.rend[3 * b + 1] = c; I.rend[3 * b + 2] = f;
+    __threadfence();
+    s_last = atomicAdd(I.ticket, 1u) == (unsigned)(S - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double O = 0.0, C = 0.0, F = 0.0;
+    for (int q = 0; q < S; ++q) {
+      O += __hip_atomic_load(I.rend + 3 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      C += __hip_atomic_load(I.rend + 3 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      F += __hip_atomic_load(I.rend + 3 * q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    rec->outflow += O;
+    rec->clipped += C;
+    rec->infil += F;
+    if (rec->rainCells >= 0) rec->rain += (double)(rec->rainRate * dt) * (double)rec->lcells[m] * (double)I.area;
+    else rec->rain += (double)rec->rainRate * (double)dt * rec->lrf[m] * (double)I.area;
+    rec->cellUpdates += (double)rec->lcells[m];
+    rec->records += 1;
+    rec->lastDt = dt;
+    __hip_atomic_store(I.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // for the next record
+  }
+  for (int j = threadIdx.x; j < I.nin; j += blockDim.x) exchange_inlet<true>(I, j, m);
 }
 
 // ------------------------------------------------------------ local time stepping, per cycle
@@ -1602,44 +1666,6 @@ __global__ __launch_bounds__(1024) void sw_lts_lists(const DevInst *__restrict__
   }
 }
 
-// end of a level-m record: books in a fixed order (as sw_dt_final)
-__global__ __launch_bounds__(1024) void sw_lts_record_end(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
-                                                        const Part *__restrict__ p2, int m) {
-  const DevInst &I = insts[0];
-  StepRec *rec = I.rec;
-  const float dt = rec->ldt[m];
-  if (!(dt > 0.f)) return;
-  const int n = lts_waves(I, m);   // one partial per wavefront of the record
-  __shared__ double s_o[16], s_c[16], s_f[16];
-  double o = 0.0, c = 0.0, f = 0.0;
-  // unrolled so that several partials are in flight per thread (the sums keep their order)
-#pragma unroll 16
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const Part a = p1[k];
-    o += a.out;
-    c += a.clip;
-    if (I.secondOrder) {
-      const Part b = p2[k];
-      o += b.out;
-      c += b.clip;
-      f += b.infil;
-    } else f += p2[k].infil;
-  }
-  o = wave_sum(o); c = wave_sum(c); f = wave_sum(f);
-  if ((threadIdx.x & 63) == 0) { s_o[threadIdx.x >> 6] = o; s_c[threadIdx.x >> 6] = c; s_f[threadIdx.x >> 6] = f; }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) { o += s_o[k]; c += s_c[k]; f += s_f[k]; }
-  rec->outflow += o;
-  rec->clipped += c;
-  rec->infil += f;
-  if (rec->rainCells >= 0) rec->rain += (double)(rec->rainRate * dt) * (double)rec->lcells[m] * (double)I.area;
-  else rec->rain += (double)rec->rainRate * (double)dt * rec->lrf[m] * (double)I.area;
-  rec->cellUpdates += (double)rec->lcells[m];
-  rec->records += 1;
-  rec->lastDt = dt;
-}
-
 // end of a cycle: the clock (solver-cpu.js:525)
 __global__ void sw_lts_cycle_end(const DevInst *__restrict__ insts) {
   StepRec *rec = insts[0].rec;
@@ -1756,10 +1782,7 @@ __global__ __launch_bounds__(256) void sw_euler_finish(const DevInst *__restrict
 // speed of the inlet cell for the next dt (the second stage leaves inlet cells out of its maximum).
 // LTS: inlets whose cell is at the record's level, with its step (exchangeL, kernels.wgsl.js:596).
 template <bool LTS>
-__global__ void sw_exchange(const DevInst *__restrict__ insts, int inst, int m) {
-  const DevInst &I = insts[inst];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= I.nin) return;
+__device__ __forceinline__ void exchange_inlet(const DevInst &I, const int j, const int m) {
   Inlet &q = I.inl[j];
   float dt = I.rec->dt;
   const size_t k = (size_t)q.cell;
@@ -1799,6 +1822,13 @@ __global__ void sw_exchange(const DevInst *__restrict__ insts, int inst, int m) 
   if (fw > fm) { fm = fw; nb = k + I.pitch; ax = 1.f; }
   const float hn = fm > 0.f ? I.h[nb] : 0.f;
   I.iapp[j] = make_float4(fm * I.dx, (fx + fy + fz + fw - fm) * I.dx, hn, ax);
+}
+
+template <bool LTS>
+__global__ void sw_exchange(const DevInst *__restrict__ insts, int inst, int m) {
+  const DevInst &I = insts[inst];
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < I.nin) exchange_inlet<LTS>(I, j, m);
 }
 
 // End of a step, one block per instance (dtFinal, kernels.wgsl.js:783-799): fold the stage

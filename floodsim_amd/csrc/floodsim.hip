@@ -542,7 +542,7 @@ int launch_cycle(fs_pool *P, const Launch &G) {
     if (!e0 || !e1 || !e2) return fail(P, FS_E_HIP, "hipEventCreate failed");
     HIPCHK(P, hipEventRecord(e0, P->stream));
   }
-  sw_lts_speed<<<(nt + 3) / 4, 256, 0, P->stream>>>(G.dinst);
+  sw_lts_speed<<<((d.tx + 7) / 8 * d.ty + 3) / 4, 256, 0, P->stream>>>(G.dinst);
   sw_lts_cycle<<<1, 1024, 0, P->stream>>>(G.dinst);
   sw_lts_level<<<(nt + 255) / 256, 256, 0, P->stream>>>(G.dinst);
   sw_lts_smooth<<<(nt + 255) / 256, 256, 0, P->stream>>>(G.dinst);

@@ -1421,24 +1421,35 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
 // ------------------------------------------------------------ local time stepping, per cycle
 // (_cycle, solver-cpu.js:504-527; ltsCycle/ltsLevel/ltsSmooth/ltsList, kernels.wgsl.js:677-741)
 
-// max wave speed per 8x8 tile at the cycle start: one wavefront per tile
-// (and the block's maximum into one of kSpeedSlots slots by atomic max: the cycle kernel reduces
-// the slots; a max does not depend on the order of the atomics, and speeds are non-negative floats,
-// which order as their bit patterns do)
+// max wave speed per 8x8 tile at the cycle start: one wavefront per 8 tiles of a tile row (lane = column, a
+// walk down the tile row's 8 rows, then the maximum over each tile's 8 lanes), so every row's loads are
+// contiguous; and the block's maximum into one of kSpeedSlots slots by atomic max: the cycle kernel reduces
+// the slots (a max does not depend on the order of the atomics, and speeds are non-negative floats, which
+// order as their bit patterns do)
 __global__ __launch_bounds__(256) void sw_lts_speed(const DevInst *__restrict__ insts) {
   const DevInst &I = insts[0];
   const int t = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
+  const int cbs = (I.tx + 7) / 8;   // blocks of 8 tiles across
   float s = 0.f;
-  if (t < I.tx * I.ty) {
-    const int x = (t % I.tx) * kLtsTile + (lane & 7), y = (t / I.tx) * kLtsTile + (lane >> 3);
-    if (x < I.nx && y < I.ny) {
-      const size_t k = (size_t)y * I.pitch + x;
-      const float h = I.h[k];
-      if ((I.mask[k] & MASK_OPEN) && h > I.hdry) s = cell_speed(h, I.hu[k], I.hv[k], I.g);
+  if (t < cbs * I.ty) {
+    const int cb = t % cbs, tb = t / cbs;
+    const int x = cb * 64 + lane, y0 = tb * kLtsTile;
+    if (x < I.nx) {
+      const int y1 = min(y0 + kLtsTile, I.ny);
+      for (int y = y0; y < y1; ++y) {
+        const int k = y * I.pitch + x;
+        const float h = gld(I.h, k);
+        const float hu = gld(I.hu, k), hv = gld(I.hv, k);
+        if ((gld(I.mask, k) & MASK_OPEN) && h > I.hdry) s = fmaxf(s, cell_speed(h, hu, hv, I.g));
+      }
     }
+    s = fmaxf(s, __shfl_xor(s, 1, 64));
+    s = fmaxf(s, __shfl_xor(s, 2, 64));
+    s = fmaxf(s, __shfl_xor(s, 4, 64));
+    const int tc = cb * 8 + (lane >> 3);
+    if ((lane & 7) == 0 && tc < I.tx) I.tsp[tb * I.tx + tc] = s;
     s = wave_max(s);
-    if (lane == 0) I.tsp[t] = s;
   }
   __shared__ float s_m[4];
   if (lane == 0) s_m[threadIdx.x >> 6] = s;

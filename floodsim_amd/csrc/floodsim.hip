@@ -63,7 +63,7 @@ struct Slot {
   unsigned *speedSlots = nullptr;
   int32_t *tileOpen = nullptr, *ucells = nullptr;
   float *sh = nullptr, *shu = nullptr, *shv = nullptr, *sw = nullptr;
-  float *g0h = nullptr, *g0hu = nullptr, *g0hv = nullptr, *g1h = nullptr, *g1hu = nullptr, *g1hv = nullptr;
+  float *g0h = nullptr, *g0hu = nullptr, *g0hv = nullptr, *tst = nullptr;
   float4 *bsx = nullptr, *bsy = nullptr;
   // flood barriers: one separate allocation (crest, bidx per cell; faces, ground, flux, books per barrier)
   char *bmem = nullptr;
@@ -229,7 +229,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
     const size_t nt = (size_t)((p->nx + 7) / 8) * ((p->ny + 7) / 8), nu = (size_t)units_of(p);
     b += 2 * align256(nt) + align256(nt * 4) * 3;           // lv, lv0, tsp, tileOpen, tileRf
     b += align256(kSpeedSlots * 4);                          // tile-speed block maxima
-    b += 10 * align256(N * 4) + align256(N) + 2 * align256(N * 16);  // store, gathers, flag, bsx, bsy
+    b += 8 * align256(N * 4) + align256(N) + 2 * align256(N * 16);   // store, gathers, tst, flag, bsx, bsy
     b += align256(nu * 8 * (kMaxLevels + 1)) + align256(nu * 4 * (kMaxLevels + 1));   // urows, ucells
     b += align256(nu * kLtsUnitRuns * 16 * (kMaxLevels + 1));                         // plist
     b += align256(kLtsEndSlices * 3 * 8) + align256(4);                               // rend, ticket
@@ -331,7 +331,7 @@ DevInst devinst_of(const Slot &S, StepRec *rec, int wbeg, int wend) {
   d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.speedSlots = S.speedSlots; d.tileOpen = S.tileOpen;
   d.tileRf = S.rf ? S.tileRf : nullptr;
   d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
-  d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.g1h = S.g1h; d.g1hu = S.g1hu; d.g1hv = S.g1hv;
+  d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.tst = S.tst;
   d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
   d.urows = S.urows; d.ucells = S.ucells; d.plist = S.plist; d.capParts = kLtsUnitRuns * S.nunits;
   d.rend = S.rend; d.ticket = S.ticket;
@@ -969,7 +969,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     S.speedSlots = (unsigned *)take(kSpeedSlots * 4);
     S.sh = (float *)take(N * 4); S.shu = (float *)take(N * 4); S.shv = (float *)take(N * 4); S.sw = (float *)take(N * 4);
     S.g0h = (float *)take(N * 4); S.g0hu = (float *)take(N * 4); S.g0hv = (float *)take(N * 4);
-    S.g1h = (float *)take(N * 4); S.g1hu = (float *)take(N * 4); S.g1hv = (float *)take(N * 4);
+    S.tst = (float *)take(N * 4);
     S.gflag = (uint8_t *)take(N);
     S.bsx = (float4 *)take(N * 16); S.bsy = (float4 *)take(N * 16);
     S.urows = (uint64_t *)take((size_t)S.nunits * 8 * (kMaxLevels + 1));

@@ -151,9 +151,10 @@ struct DevInst {
   const int32_t *tileOpen;      // open cells per tile
   const float *tileRf;          // rain multiplier summed over the open cells of a tile (rain field only)
   float *sh, *shu, *shv, *sw;   // flux a waiting cell has gathered (Kahan residual sw)
-  float *g0h, *g0hu, *g0hv;     // a record's first-stage gather of a waiting cell (folded into sh.. after)
-  float *g1h, *g1hu, *g1hv;     // the same, second stage
-  uint8_t *gflag;               // 1: the cell gathered in the current record
+  float *g0h, *g0hu, *g0hv;     // a record's first-stage gather of a waiting cell (added to sh.. after)
+  float *tst;                   // a waiting cell's store with the record's first-stage gather: what the
+                                // second stage's theta sees while the cell's wavefront adds the gathers
+  uint8_t *gflag;               // first order: 1 = the cell gathered in the current record
   float4 *bsx, *bsy;            // a cell's reconstruction toward its faces at their last evaluation:
                                 // x (h west, dz west, h east, dz east), y (south, north); dz = face bed - cell bed
   uint64_t *urows;              // per level and work unit (stride wend - wbeg): the unit's rows a record of the

@@ -175,6 +175,17 @@ __device__ __forceinline__ void weir_flux(Face &f, float eA, float hA, float tA,
   f.F0 = F0; f.F1 = q * un + pres; f.F2 = F0 * tU;
 }
 
+// a product and a sum rounded on their own (never fused into a multiply-add with their neighbours): values
+// that earlier builds stored to memory and read back keep their bits
+__device__ __forceinline__ float mul_nc(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+__device__ __forceinline__ float add_nc(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+
 // Kahan-compensated depth: stored h with residual w, true depth h - w (kernels.wgsl.js:265-272)
 __device__ __forceinline__ void kadd(float &h, float &w, float d) {
   const float y = d - w;
@@ -618,6 +629,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
   FaceRow xf3 = {0, 0, 0, 0, 0};                          // x faces of row r-3
   FaceRow yf3 = {0, 0, 0, 0, 0}, yf4 = {0, 0, 0, 0, 0};   // y faces (r-3 | r-2) and (r-4 | r-3)
   float th3 = 1.f, th4 = 1.f;
+  float st3 = 0.f;   // LTS first stage: the store of row r-3's cell (theta's load of the previous pass)
   // bed-slope terms of row r-3 (x) and rows r-2, r-3 (y); under LTS a stepping cell's term already
   // holds, for a face whose neighbour is finer, that face's last evaluation
   float sx3 = 0, sy2 = 0, sy3 = 0;
@@ -657,17 +669,15 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     // row r-2 (LTS), then the update's operands of row r-3 (its residual, RK start state and store), then its
     // sources' layers. The memory counter drains in issue order, so loads issued at their sections would
     // each wait on their own; issued together, one wait covers them and their latency overlaps the faces.
-    float tsh = 0.f, tsw = 0.f, tg0 = 0.f;
-    uint8_t tgf = 0;
+    float tsh = 0.f, tsw = 0.f, tst = 0.f;
     float pw = 0.f, ph0 = 0.f, pw0 = 0.f, phu0 = 0.f, phv0 = 0.f, psh = 0.f, psw = 0.f, pshu = 0.f, pshv = 0.f;
     SrcOps pso = {0.f, 0.f, 0.f, 1.f, (uint8_t)0};
     if constexpr (PRE && PH == 4) {
       if constexpr (LTS) {
         if (in2 && M2r && (lv2 == m + 1 || (STAGE == 0 && lv2 == m))) {
           const int k2 = (r - 2) * pitch + X;
-          tsh = gld(I.sh, k2);
-          tsw = gld(I.sw, k2);
-          if (STAGE == 1) { tgf = gld(I.gflag, k2); tg0 = gld(I.g0h, k2); }
+          if (STAGE == 0) { tsh = gld(I.sh, k2); tsw = gld(I.sw, k2); }
+          else tst = gld(I.tst, k2);
         }
       }
       const int ry = r - kHalo;
@@ -798,6 +808,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       }
       // ---------------- theta of row r-2 (solver-cpu.js:307-318; kernels.wgsl.js:349-363)
       float th2 = 1.f;
+      float st2 = 0.f;   // LTS first stage: the store of row r-2's cell (its gather uses it in the next pass)
       {
         const float fe = xf2.F0, fw = from_left(xf2.F0), fs = yf3.F0, fn = yf2.F0;
         bool take = in2 && M2r;
@@ -807,12 +818,14 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
           // a waiting cell, or a stepping one in its first stage, holds its water plus its store
           if (take && (lv2 == m + 1 || STAGE == 0)) {
             const int k2 = (r - 2) * pitch + X;
-            float st = PRE ? tsh - tsw : gld(I.sh, k2) - gld(I.sw, k2);
-            // first-stage gather, not yet folded (its delta loaded with the flag, not after it: one wait)
-            if (STAGE == 1) {
-              const uint8_t gf = PRE ? tgf : gld(I.gflag, k2);
-              const float g0 = PRE ? tg0 : gld(I.g0h, k2);
-              if (gf) st += g0;
+            float st;
+            if (STAGE == 0) {
+              st = PRE ? tsh - tsw : gld(I.sh, k2) - gld(I.sw, k2);
+              st2 = st;
+            } else {
+              // a waiting cell in the second stage: its store with the first stage's gather, as the first
+              // stage left it (the cell's own wavefront folds the record's gathers into the store meanwhile)
+              st = PRE ? tst : gld(I.tst, k2);
             }
             hv_ = fmaxf(0.f, h2r + st);
           }
@@ -872,16 +885,29 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
           const float dhu = -inv_dx * ((Fe1 + cLe) - (w1 + cRw) + bx + Fn2 - Fs2);
           const float dhv = -inv_dx * ((Fn1 + cLn) - (Fs1 + cRs) + by + Fe2 - w2);
           if (LTS && coarse) {
-            // waiting cell: gather the RK-weighted change (ltsUpdate mode 2, kernels.wgsl.js:515-519)
-            // (into per-stage deltas: a neighbouring strip may read this cell's store for theta
-            // during this kernel; sw_lts_fold adds the deltas once the record's stages are done)
+            // waiting cell: gather the RK-weighted change (ltsUpdate mode 2, kernels.wgsl.js:515-519).
+            // First stage: the change into g0 (neighbouring strips read this cell's store for theta during the
+            // stage, so it stays), and the store the second stage's theta reads, the first stage's change
+            // included, into tst. Second stage: the cell's own wavefront adds both stages' changes to its store
+            // (compensated, as kadd on U0), which no other wavefront reads during that stage. First order: the
+            // change and a flag, added after the record (sw_lts_fold).
             if (gathers) {
               const float wd = so ? 0.5f * dt : dt;
+              const float g0 = mul_nc(wd, dh), g0u = mul_nc(wd, dhu), g0v = mul_nc(wd, dhv);
               if (STAGE == 0) {
-                gst(I.g0h, k, wd * dh); gst(I.g0hu, k, wd * dhu); gst(I.g0hv, k, wd * dhv);
-                gst(I.gflag, k, (uint8_t)1);
+                gst(I.g0h, k, g0); gst(I.g0hu, k, g0u); gst(I.g0hv, k, g0v);
+                if (so) gst(I.tst, k, add_nc(st3, g0));
+                else gst(I.gflag, k, (uint8_t)1);
               } else {
-                gst(I.g1h, k, wd * dh); gst(I.g1hu, k, wd * dhu); gst(I.g1hv, k, wd * dhv);
+                float sh = gld(I.sh, k), sw = gld(I.sw, k), shu = gld(I.shu, k), shv = gld(I.shv, k);
+                const float a0 = gld(I.g0h, k), a0u = gld(I.g0hu, k), a0v = gld(I.g0hv, k);
+                kadd(sh, sw, a0);
+                shu = add_nc(shu, a0u);
+                shv = add_nc(shv, a0v);
+                kadd(sh, sw, g0);
+                shu = add_nc(shu, g0u);
+                shv = add_nc(shv, g0v);
+                gst(I.sh, k, sh); gst(I.sw, k, sw); gst(I.shu, k, shu); gst(I.shv, k, shv);
               }
             }
           } else {
@@ -958,6 +984,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       // ---------------- x faces, theta and bed slope of row r-2 move on
       yf4 = yf3;
       th4 = th3; th3 = th2;
+      if constexpr (LTS && STAGE == 0) st3 = st2;
       xf3 = xf2;
       sx3 = sx2;
     }
@@ -1295,8 +1322,8 @@ __device__ __forceinline__ void exchange_inlet(const DevInst &I, const int j, co
 constexpr int kLtsEndSlices = 32;
 
 // After the stages of a level-m record, one kernel ends it:
-// - a waiting cell's store takes the record's gathered flux (compensated, as kadd on U0 in ltsUpdate,
-//   kernels.wgsl.js:518);
+// - first order: a waiting cell's store takes the record's gathered flux (compensated, as kadd on U0 in
+//   ltsUpdate, kernels.wgsl.js:518; second order: the second stage has added it);
 // - the record's books: the first blocks each sum a fixed slice of the stages' per-wavefront partials in a
 //   fixed order, and the last of them to finish (a ticket) sums the slices in slice order into the FP64
 //   books, so the result does not depend on which block ends last;
@@ -1308,7 +1335,7 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
   const int np = I.rec->lcount[m], G = lts_waves(I, m);
   const int w = xcd_wave_index_n(G, kLtsWaves);
   const bool two = I.secondOrder;
-  if (w >= 0) {
+  if (w >= 0 && !two) {   // (second order: the second stage has added the gathers)
     for (int i = w; i < np; i += G) {
       const int4 wk = I.plist[m * I.capParts + i];
       const int X = wk.y - I.hl + lane;
@@ -1331,16 +1358,14 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
           for (int q = 0; q < kLtsTile; ++q) f[q] = gld(I.gflag, min(yb + q, ye - 1) * I.pitch + X);
 #pragma unroll
           for (int q = 0; q < kLtsTile; ++q) f[q] = yb + q < ye ? f[q] : (uint8_t)0;
-          float sh[kLtsTile], sw[kLtsTile], shu[kLtsTile], shv[kLtsTile], a[kLtsTile], au[kLtsTile], av[kLtsTile];
-          float b[kLtsTile], bu[kLtsTile], bv[kLtsTile];
+            float sh[kLtsTile], sw[kLtsTile], shu[kLtsTile], shv[kLtsTile], a[kLtsTile], au[kLtsTile], av[kLtsTile];
 #pragma unroll
           for (int q = 0; q < kLtsTile; ++q) {
-            sh[q] = sw[q] = shu[q] = shv[q] = a[q] = au[q] = av[q] = b[q] = bu[q] = bv[q] = 0.f;
+            sh[q] = sw[q] = shu[q] = shv[q] = a[q] = au[q] = av[q] = 0.f;
             if (f[q]) {
               const int k = (yb + q) * I.pitch + X;
               sh[q] = gld(I.sh, k); sw[q] = gld(I.sw, k); shu[q] = gld(I.shu, k); shv[q] = gld(I.shv, k);
               a[q] = gld(I.g0h, k); au[q] = gld(I.g0hu, k); av[q] = gld(I.g0hv, k);
-              if (two) { b[q] = gld(I.g1h, k); bu[q] = gld(I.g1hu, k); bv[q] = gld(I.g1hv, k); }
             }
           }
 #pragma unroll
@@ -1349,12 +1374,7 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
             const int k = (yb + q) * I.pitch + X;
             float h = sh[q], w = sw[q];
             kadd(h, w, a[q]);
-            float hu = shu[q] + au[q], hv = shv[q] + av[q];
-            if (two) {
-              kadd(h, w, b[q]);
-              hu += bu[q];
-              hv += bv[q];
-            }
+            const float hu = shu[q] + au[q], hv = shv[q] + av[q];
             gst(I.sh, k, h); gst(I.sw, k, w); gst(I.shu, k, hu); gst(I.shv, k, hv);
             gst(I.gflag, k, (uint8_t)0);
           }

@@ -610,8 +610,8 @@ int launch_cycle(fs_pool *P, const Launch &G) {
         HIPCHK(P, hipEventRecord(sb, P->stream));
         P->spans.push_back({sa, sb, cyc, m});
       }
-      // gathers folded, books, inlet exchange (one kernel)
-      sw_lts_fold<<<nbp, kT, 0, P->stream>>>(G.dinst, G.p1, G.p2, m);
+      // books and inlet exchange (first order: the gathers added first, by every part's wavefront)
+      sw_lts_fold<<<d.secondOrder ? kLtsEndSlices : nbp, kT, 0, P->stream>>>(G.dinst, G.p1, G.p2, m);
     }
   sw_lts_cycle_end<<<1, 1, 0, P->stream>>>(G.dinst);
   if (P->timing) HIPCHK(P, hipEventRecord(e2, P->stream));

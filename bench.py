@@ -337,7 +337,9 @@ def main():
                      "traffic_source": tr and tr.get("source"),
                      "kernel": "fs::sw_stage<1,%s> (second RK stage + sources)" % ("true" if lts else "false"),
                      "bytes_per_cell_update": B_STAGE2, "launches": dom_n,
-                     "cells_per_launch": dom_cells / max(dom_n, 1), "avg_ms": dom_ms / max(dom_n, 1)},
+                     "cells_per_launch": dom_cells / max(dom_n, 1), "avg_ms": dom_ms / max(dom_n, 1),
+                     "copy_probe_GBps": cp_gbs, "frac_of_copy": achieved / cp_gbs if achieved and cp_gbs else None,
+                     "traffic_GBps": traffic / (dom_ms / max(dom_n, 1) * 1e-3) / 1e9 if traffic and dom_ms > 0 else None},
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,

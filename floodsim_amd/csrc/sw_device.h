@@ -109,6 +109,14 @@ struct Inlet {
   double vol;   // exchanged since the last read, m^3
 };
 
+// per wavefront strip and stage: boundary outflow (m^3), clipped (m^3), infiltration (m^3), max
+// speed, open cells advanced (second stage)
+struct Part {
+  double out, clip, infil;
+  float smax;
+  int32_t cells;
+};
+
 struct DevInst {
   float *h, *hu, *hv, *w;       // A: U^n, and U^{n+1} after the second stage (in place)
   float *h1, *hu1, *hv1, *w1;   // B: U* (first-stage result)
@@ -163,7 +171,7 @@ struct DevInst {
   int4 *plist;                  // per level (stride capParts): the parts of the level's records, (0, first
                                 // column, first row, rows) as a work item
   int capParts;                 // parts a level's list holds (8 per unit)
-  double *rend;                 // a record's books per slice of its partials (3 x kLtsEndSlices)
+  double *rend;                 // a record's books per slice of its partials (3 x the slices)
   unsigned *ticket;             // slices summed so far in the current record (0 between records)
   // flood barriers (null: none). Per cell: crest and weir coefficient of its west face (x) and south face
   // (y), crest relative to the bed minimum (-1e30 = none); the barrier index of those faces (-1 = none).
@@ -182,13 +190,6 @@ struct DevInst {
   float4 *soil, *gi;
 };
 
-// per wavefront strip and stage: boundary outflow (m^3), clipped (m^3), infiltration (m^3), max
-// speed, open cells advanced (second stage)
-struct Part {
-  double out, clip, infil;
-  float smax;
-  int32_t cells;
-};
 
 
 }  // namespace fs

@@ -1156,6 +1156,71 @@ __device__ __forceinline__ float lts_record_end_time(const DevInst &I, int kk) {
   return (float)(kk + 1 == (1 << I.L) ? rec->cycT0 + rec->cycD : rec->cycT0 + (double)(kk + 1) * rec->base);
 }
 
+template <bool LTS>
+__device__ __forceinline__ void exchange_inlet(const DevInst &I, const int j, const int m);
+
+// slices of a record's partials summed apart at its end (at most)
+constexpr int kLtsEndSlices = 64;
+
+// The end of a level-m record: its books (the stages' per-wavefront partials summed in a fixed order, so the
+// result does not depend on which block ends last) and the exchange of the inlets whose cell is at the level
+// (exchangeL, kernels.wgsl.js:596). The partials are cut into S slices, each summed in index order into the
+// slice's books; the last slice's summer (a ticket) sums the slices in slice order.
+
+// sum slice s (partials k0 .. k1-1) into rend, then take the ticket: true for the last slice (one wavefront)
+__device__ __forceinline__ bool lts_end_slice(const DevInst &I, const Part *__restrict__ p1,
+                                              const Part *__restrict__ p2, int s, int k0, int k1, int S, int lane) {
+  const bool two = I.secondOrder;
+  double o = 0.0, c = 0.0, f = 0.0;
+  for (int k = k0 + lane; k < k1; k += 64) {
+    const Part a = p1[k];
+    o += a.out;
+    c += a.clip;
+    if (two) {
+      const Part q = p2[k];
+      o += q.out;
+      c += q.clip;
+      f += q.infil;
+    } else f += p2[k].infil;
+  }
+  o = wave_sum(o); c = wave_sum(c); f = wave_sum(f);
+  unsigned t = 0;
+  if (lane == 0) {
+    I.rend[3 * s] = o; I.rend[3 * s + 1] = c; I.rend[3 * s + 2] = f;
+    __threadfence();
+    t = atomicAdd(I.ticket, 1u);
+  }
+  t = __shfl(t, 0, 64);
+  return t == (unsigned)(S - 1);
+}
+
+// the last slice's wavefront: the record's books from the S slices, the ticket reset for the next record, the
+// inlet exchange
+__device__ __forceinline__ void lts_books(const DevInst &I, int m, int S, int lane) {
+  __threadfence();
+  StepRec *rec = I.rec;
+  const float dt = rec->ldt[m];
+  double O = 0.0, C = 0.0, F = 0.0;
+  for (int q = lane; q < S; q += 64) {   // (fixed order: each lane's slices in turn, then a fixed tree)
+    O += __hip_atomic_load(I.rend + 3 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    C += __hip_atomic_load(I.rend + 3 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    F += __hip_atomic_load(I.rend + 3 * q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  O = wave_sum(O); C = wave_sum(C); F = wave_sum(F);
+  if (lane == 0) {
+    rec->outflow += O;
+    rec->clipped += C;
+    rec->infil += F;
+    if (rec->rainCells >= 0) rec->rain += (double)(rec->rainRate * dt) * (double)rec->lcells[m] * (double)I.area;
+    else rec->rain += (double)rec->rainRate * (double)dt * rec->lrf[m] * (double)I.area;
+    rec->cellUpdates += (double)rec->lcells[m];
+    rec->records += 1;
+    rec->lastDt = dt;
+  }
+  if (lane == 0) __hip_atomic_store(I.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // for the next record
+  for (int j = lane; j < I.nin; j += 64) exchange_inlet<true>(I, j, m);
+}
+
 // LTS record of level m (single-instance groups): the parts of the level's list
 template <int STAGE, bool HZ, bool EXT = false>
 __device__ __forceinline__ void lts_stage_body(const DevInst *__restrict__ insts, Part *__restrict__ part, int m) {
@@ -1315,19 +1380,12 @@ __global__ __launch_bounds__(64) void sw_barrier_books(const DevInst *__restrict
 }
 
 // ------------------------------------------------------------ local time stepping, per record
-template <bool LTS>
-__device__ __forceinline__ void exchange_inlet(const DevInst &I, const int j, const int m);
 
-// blocks of the fold kernel that sum a slice of the record's partials each (kLtsEndSlices at most)
-constexpr int kLtsEndSlices = 32;
-
-// After the stages of a level-m record, one kernel ends it:
-// - first order: a waiting cell's store takes the record's gathered flux (compensated, as kadd on U0 in
-//   ltsUpdate, kernels.wgsl.js:518; second order: the second stage has added it);
-// - the record's books: the first blocks each sum a fixed slice of the stages' per-wavefront partials in a
-//   fixed order, and the last of them to finish (a ticket) sums the slices in slice order into the FP64
-//   books, so the result does not depend on which block ends last;
-// - that block then exchanges the inlets whose cell is at the level (exchangeL, kernels.wgsl.js:596).
+// After the stages of a level-m record, one kernel ends it: under first order a waiting cell's store takes the
+// record's gathered flux (compensated, as kadd on U0 in ltsUpdate, kernels.wgsl.js:518; under second order the
+// second stage has added it), then its first S blocks sum a slice of the partials each and the last does the
+// books and the inlets. (A second stage that ended its record itself, each wavefront counting its slice done,
+// measured 1.8x slower: a device-scope release after every wavefront writes back its XCD's L2.)
 __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ insts, const Part *__restrict__ p1,
                                                    const Part *__restrict__ p2, int m) {
   const DevInst &I = insts[0];
@@ -1385,57 +1443,13 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
   }
 
   // ---- the record's end
-  StepRec *rec = I.rec;
-  const float dt = rec->ldt[m];
+  const float dt = I.rec->ldt[m];
   const int S = min(kLtsEndSlices, (int)gridDim.x), b = blockIdx.x;
   if (b >= S || !(dt > 0.f)) return;
-  const int n = G;   // one partial per wavefront of the record
-  const int k0 = (int)((long long)n * b / S), k1 = (int)((long long)n * (b + 1) / S);
-  double o = 0.0, c = 0.0, f = 0.0;
-  for (int k = k0 + (int)threadIdx.x; k < k1; k += blockDim.x) {
-    const Part a = p1[k];
-    o += a.out;
-    c += a.clip;
-    if (two) {
-      const Part q = p2[k];
-      o += q.out;
-      c += q.clip;
-      f += q.infil;
-    } else f += p2[k].infil;
+  if (threadIdx.x < 64) {   // (one wavefront per block sums its slice)
+    const int k0 = (int)((long long)G * b / S), k1 = (int)((long long)G * (b + 1) / S);
+    if (lts_end_slice(I, p1, p2, b, k0, k1, S, lane)) lts_books(I, m, S, lane);
   }
-  o = wave_sum(o); c = wave_sum(c); f = wave_sum(f);
-  __shared__ double s_r[3 * (256 / 64)];
-  __shared__ int s_last;
-  const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-  if (lane == 0) { s_r[3 * wv] = o; s_r[3 * wv + 1] = c; s_r[3 * wv + 2] = f; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 1; q < nwv; ++q) { o += s_r[3 * q]; c += s_r[3 * q + 1]; f += s_r[3 * q + 2]; }
-    I.rend[3 * b] = o; I.rend[3 * b + 1] = c; I.rend[3 * b + 2] = f;
-    __threadfence();
-    s_last = atomicAdd(I.ticket, 1u) == (unsigned)(S - 1);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  if (threadIdx.x == 0) {
-    double O = 0.0, C = 0.0, F = 0.0;
-    for (int q = 0; q < S; ++q) {
-      O += __hip_atomic_load(I.rend + 3 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      C += __hip_atomic_load(I.rend + 3 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      F += __hip_atomic_load(I.rend + 3 * q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    rec->outflow += O;
-    rec->clipped += C;
-    rec->infil += F;
-    if (rec->rainCells >= 0) rec->rain += (double)(rec->rainRate * dt) * (double)rec->lcells[m] * (double)I.area;
-    else rec->rain += (double)rec->rainRate * (double)dt * rec->lrf[m] * (double)I.area;
-    rec->cellUpdates += (double)rec->lcells[m];
-    rec->records += 1;
-    rec->lastDt = dt;
-    __hip_atomic_store(I.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // for the next record
-  }
-  for (int j = threadIdx.x; j < I.nin; j += blockDim.x) exchange_inlet<true>(I, j, m);
 }
 
 // ------------------------------------------------------------ local time stepping, per cycle

@@ -531,7 +531,7 @@ __device__ __forceinline__ float hz_arr(float d, float v) {
 }
 __device__ __forceinline__ float hz_hr(float hr) { return hr < 0.75f ? 1.f : hr < 1.25f ? 2.f : hr < 2.f ? 3.f : 4.f; }
 
-__device__ __noinline__ void hazard_cell(const DevInst &I, int k, float d, float hu, float hv, float dt,
+__device__ __forceinline__ void hazard_cell(const DevInst &I, int k, float d, float hu, float hv, float dt,
                                          float tnow) {
   if (!(d >= kHzWet)) return;
   const float v = d >= kHzVcut ? sqrtf(hu * hu + hv * hv) / d : 0.f;

@@ -45,7 +45,7 @@ typedef struct fs_params {
   int32_t limiter;     /* 0 minmod, 1 van Leer */
   int32_t bc[4];       /* west, east, south, north: FS_BC_* */
   int32_t lts;         /* local time stepping levels 0..4 (0 = global steps) */
-  int32_t rows;        /* rows per wavefront strip (0 = autotuned) */
+  int32_t rows;        /* rows per wavefront strip (0 = autotuned; local time stepping: a multiple of 8, at most 64) */
   int32_t untiled;     /* 1: advance every strip every step (the reference's tiles: false); 0: skip
                           quiet dry strips (the reference's active tiles, kernels.wgsl.js:639-673) */
   int32_t hazard;      /* 1: track the hazard products every step (hazard/hzRecord,

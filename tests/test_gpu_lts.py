@@ -94,11 +94,11 @@ def test_lts_urban_rain(fsa):
 
 
 def test_lts_record_parts_change_nothing(fsa):
-    """A record of few units splits them into parts of 4, 2 or 1 rows (as many wavefronts as the grid
-    allows, at most 4096), a larger one into parts of whole tile rows. With 8-24-row strips the urban
-    case's records of one or two units take single-row parts; with 40-row strips the last strip is cut
-    by the domain's edge. Every cell is updated by one part with the same arithmetic, so the state is
-    the 64-row run's bit for bit (and the oracle's to the parity bar), with the books closed."""
+    """A record walks the runs of rows its units own, cut into parts of at most 32 rows, or, for a record of
+    few owned rows, of 1, 2, 4, 8 or 16 rows (within 4096 wavefronts). The unit height (8-40 rows here
+    against the default 64) changes every run and part; with 40-row units the last unit is cut by the
+    domain's edge. Every cell is updated by one part with the same arithmetic, so the state is the 64-row
+    run's bit for bit (and the oracle's to the parity bar), with the books closed."""
     sc = rc.cloudburst(80)
     opts = dict(nx=sc["nx"], ny=sc["ny"], dx=sc["dx"], z=sc["z"], mask=sc["mask"], manning=sc["manning"],
                 bc=sc["bc"], infil=sc["infil"])
@@ -119,3 +119,46 @@ def test_lts_record_parts_change_nothing(fsa):
         assert same and dr == 0.0
         assert abs(s["mass"]["rel"]) <= TOL_MASS and s["mass"]["clipped"] == 0
         g.destroy()
+
+
+def _varied_case(seed, nx, ny):
+    """A bumpy basin with a deep pool in one corner, a shallow film elsewhere and building walls: the pool's
+    fast water puts its tiles on the fine levels, the film on the coarse ones, with walls at every level."""
+    rng = np.random.default_rng(seed)
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = 0.004 * I + 0.002 * J + 0.3 * np.sin(I / 7.0 + seed) * np.cos(J / 5.0)
+    mask = np.ones((ny, nx), np.uint8)
+    for _ in range(12):
+        w, h = rng.integers(3, 12, 2)
+        x0, y0 = rng.integers(0, nx - w), rng.integers(0, ny - h)
+        mask[y0:y0 + h, x0:x0 + w] = 0
+    mask[: ny // 4, : nx // 4] = 1    # the pool stays open
+    opts = dict(nx=nx, ny=ny, dx=1.0, z=z.ravel(), mask=mask.ravel(), manning=0.02,
+                bc={"east": "outflow", "north": "outflow"})
+    def init(o):
+        o.setDepth(lambda X, Y, Z: np.where((X < nx / 4) & (Y < ny / 4), 1.5, 0.02))
+    return opts, init
+
+
+@pytest.mark.parametrize("seed,nx,ny", [(1, 150, 110), (2, 97, 203)])
+def test_lts_varied_levels(fsa, seed, nx, ny):
+    """Levels 0-3 occupied in patterns that change from frame to frame (the oracle's levels at 5, 10, 20 and 30 s:
+    seed 1 [0, 45, 16, 205, 0], [46, 42, 178, 0, 0], [0, 0, 129, 137, 0], [99, 150, 17, 0, 0] tiles per level),
+    on grids that are multiples of neither the 8x8 tile nor the 56-column unit: against the oracle's LTS
+    (cycles, cell updates, depths, books) and with 8-row units against the default units bit for bit (every
+    run and part different)."""
+    opts, init = _varied_case(seed, nx, ny)
+    g, c, s, d = pair(fsa, opts, init, 40, rain=60 * rc.MMHR, lts=4, frame=10)
+    assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS
+    assert g.steps == c.steps and g.clock().cellUpdates == c.cellSteps
+    assert abs(s["mass"]["outflow"] - c.vol["outflow"]) <= 1e-3 * max(c.vol["outflow"], 1.0)
+    h = fsa.GPUSolver(**opts, lts=4, rows=8)
+    init(h)
+    h.rainRate = 60 * rc.MMHR
+    for t in np.arange(10, 40 + 1e-9, 10):
+        h.advance(float(t), 8)
+    t8 = h.readState()
+    for k in ("h", "hu", "hv", "hmax"):
+        np.testing.assert_array_equal(t8[k], s[k])
+    h.destroy()
+    g.destroy()

@@ -670,11 +670,15 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     // sources' layers. The memory counter drains in issue order, so loads issued at their sections would
     // each wait on their own; issued together, one wait covers them and their latency overlaps the faces.
     float tsh = 0.f, tsw = 0.f, tst = 0.f;
+    // LTS: row r-2's cell waits beside a cell at m. Only then does its theta scale a flux: every face of a
+    // waiting cell without a neighbour at m is inactive this record, so its store need not be read.
+    bool wb2 = false;
+    if constexpr (LTS) wb2 = lv2 == m + 1 && (from_left_i(lv2) == m || from_right_i(lv2) == m || lv1 == m || lv3 == m);
     float pw = 0.f, ph0 = 0.f, pw0 = 0.f, phu0 = 0.f, phv0 = 0.f, psh = 0.f, psw = 0.f, pshu = 0.f, pshv = 0.f;
     SrcOps pso = {0.f, 0.f, 0.f, 1.f, (uint8_t)0};
     if constexpr (PRE && PH == 4) {
       if constexpr (LTS) {
-        if (in2 && M2r && (lv2 == m + 1 || (STAGE == 0 && lv2 == m))) {
+        if (in2 && M2r && (wb2 || (STAGE == 0 && lv2 == m))) {
           const int k2 = (r - 2) * pitch + X;
           if (STAGE == 0) { tsh = gld(I.sh, k2); tsw = gld(I.sw, k2); }
           else tst = gld(I.tst, k2);
@@ -816,7 +820,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         if constexpr (LTS) {
           take = take && (lv2 == m || lv2 == m + 1);
           // a waiting cell, or a stepping one in its first stage, holds its water plus its store
-          if (take && (lv2 == m + 1 || STAGE == 0)) {
+          if (take && (lv2 == m ? STAGE == 0 : wb2)) {
             const int k2 = (r - 2) * pitch + X;
             float st;
             if (STAGE == 0) {

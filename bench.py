@@ -300,7 +300,8 @@ def main():
     except Exception as e:  # noqa: BLE001
         log("copy probe failed:", e)
         cp_gbs = None
-    traffic, tr = traffic_of(args.workload, stepping)
+    # (the committed summaries cover the plain kernel variant: none for the hazard-products variant)
+    traffic, tr = traffic_of(args.workload, stepping + ("+hazard" if args.hazard else ""))
     cpu = None
     if not args.no_cpu:
         # one core (the reference's solver-cpu.js is single-threaded), then the same sample on every core
@@ -335,7 +336,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "traffic_source": tr and tr.get("source"),
-                     "kernel": "fs::sw_stage<1,%s> (second RK stage + sources)" % ("true" if lts else "false"),
+                     "kernel": "fs::sw_stage<1,%s%s> (second RK stage + sources)" % ("true" if lts else "false",
+                                                                                ",hazard" if args.hazard else ""),
                      "bytes_per_cell_update": B_STAGE2, "launches": dom_n,
                      "cells_per_launch": dom_cells / max(dom_n, 1), "avg_ms": dom_ms / max(dom_n, 1),
                      "copy_probe_GBps": cp_gbs, "frac_of_copy": achieved / cp_gbs if achieved and cp_gbs else None,

@@ -672,8 +672,14 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
     float tsh = 0.f, tsw = 0.f, tst = 0.f;
     // LTS: row r-2's cell waits beside a cell at m. Only then does its theta scale a flux: every face of a
     // waiting cell without a neighbour at m is inactive this record, so its store need not be read.
+    // (Lane moves run on every lane, outside any condition on a lane's own data: inside such a branch a move
+    // reads the inactive lanes as zero, and the compiler may stand in the branch condition's value for the
+    // moved operand. So no lane move sits in the right operand of && or ||.)
     bool wb2 = false;
-    if constexpr (LTS) wb2 = lv2 == m + 1 && (from_left_i(lv2) == m || from_right_i(lv2) == m || lv1 == m || lv3 == m);
+    if constexpr (LTS) {
+      const int lvL = from_left_i(lv2), lvR = from_right_i(lv2);
+      wb2 = (lv2 == m + 1) & ((lvL == m) | (lvR == m) | (lv1 == m) | (lv3 == m));
+    }
     float pw = 0.f, ph0 = 0.f, pw0 = 0.f, phu0 = 0.f, phv0 = 0.f, psh = 0.f, psw = 0.f, pshu = 0.f, pshv = 0.f;
     SrcOps pso = {0.f, 0.f, 0.f, 1.f, (uint8_t)0};
     if constexpr (PRE && PH == 4) {
@@ -757,7 +763,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       const float hl = from_left(h2r), ul = from_left(u2r), vlf = from_left(v2r), zl = from_left(z2r);
       const float hr = from_right(h2r), ur = from_right(u2r), vr = from_right(v2r), zr = from_right(z2r);
       const bool Ml = from_left_i(M2r) != 0, Mr = from_right_i(M2r) != 0;
-      const bool Rr = SF ? from_right_i(R2r) != 0 : false;
+      bool Rr = false;
+      if constexpr (SF) Rr = from_right_i(R2r) != 0;
       const Slopes sxs = muscl<SF>(h2r, u2r, v2r, hl, ul, vlf, Ml, z2r - zl, hr, ur, vr, Mr, zr - z2r, so, hfront, vl,
                                    sf && R2r);
       const float xh_p = h2r + 0.5f * sxs.sh, xe_p = h2r + 0.5f * sxs.se;
@@ -817,10 +824,13 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         const float fe = xf2.F0, fw = from_left(xf2.F0), fs = yf3.F0, fn = yf2.F0;
         bool take = in2 && M2r;
         float hv_ = h2r;
+        const float out = ((fe > 0.f ? fe : 0.f) - (fw < 0.f ? fw : 0.f) + (fn > 0.f ? fn : 0.f) - (fs < 0.f ? fs : 0.f)) * dx * dt;
         if constexpr (LTS) {
           take = take && (lv2 == m || lv2 == m + 1);
-          // a waiting cell, or a stepping one in its first stage, holds its water plus its store
-          if (take && (lv2 == m ? STAGE == 0 : wb2)) {
+          // a waiting cell, or a stepping one in its first stage, holds its water plus its store (the second
+          // stage reads a waiting cell's only if it passes water out: else theta stays 1 whatever it holds; the
+          // first stage's read also makes the snapshot the cell's gather leaves for the second stage)
+          if (take && (lv2 == m ? STAGE == 0 : wb2 && (STAGE == 0 || out > 0.f))) {
             const int k2 = (r - 2) * pitch + X;
             float st;
             if (STAGE == 0) {
@@ -834,7 +844,6 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             hv_ = fmaxf(0.f, h2r + st);
           }
         }
-        const float out = ((fe > 0.f ? fe : 0.f) - (fw < 0.f ? fw : 0.f) + (fn > 0.f ? fn : 0.f) - (fs < 0.f ? fs : 0.f)) * dx * dt;
         const float vol = hv_ * areaf;
         if (take && out > vol) th2 = vol / out;
       }
@@ -873,7 +882,10 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         bool own = true, coarse = false;
         // LTS: a waiting cell gathers when one of its faces is evaluated, i.e. a face neighbour is at m
         bool gathers = false;
-        if constexpr (LTS) gathers = from_left_i(lv3) == m || from_right_i(lv3) == m || lv2 == m || lv4 == m;
+        if constexpr (LTS) {
+          const int lvL = from_left_i(lv3), lvR = from_right_i(lv3);   // (every lane: see wb2)
+          gathers = (lvL == m) | (lvR == m) | (lv2 == m) | (lv4 == m);
+        }
         if constexpr (LTS) {
           own = lv3 == m;
           coarse = lv3 == m + 1;

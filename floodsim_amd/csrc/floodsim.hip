@@ -1145,28 +1145,40 @@ int fs_copy_probe(size_t bytes, int32_t iters, double *gbps, double *ms) {
   if (bytes < 1024 || iters < 1) return fail(nullptr, FS_E_ARG, "bad copy probe size");
   const size_t n = bytes / 16;
   float4 *a = nullptr, *b = nullptr;
-  hipEvent_t e0, e1;
-  HIPCHK(nullptr, hipMalloc(&a, n * 16));
-  HIPCHK(nullptr, hipMalloc(&b, n * 16));
-  HIPCHK(nullptr, hipMemset(a, 1, n * 16));
-  HIPCHK(nullptr, hipEventCreate(&e0));
-  HIPCHK(nullptr, hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t st = nullptr;
+  // every exit releases what was made (a failed call leaves the rest unmade)
+  auto done = [&](int rc) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+  };
+  hipError_t e = hipSuccess;
+#define PROBE(call) \
+  if ((e = (call)) != hipSuccess) return done(fail(nullptr, FS_E_HIP, "%s: %s", #call, hipGetErrorString(e)))
+  PROBE(hipMalloc(&a, n * 16));
+  PROBE(hipMalloc(&b, n * 16));
+  PROBE(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  PROBE(hipMemsetAsync(a, 1, n * 16, st));
+  PROBE(hipEventCreate(&e0));
+  PROBE(hipEventCreate(&e1));
   const int blocks = 256 * 8 * 4;
-  fs_copy_kernel<<<blocks, 256>>>(a, b, n);
-  HIPCHK(nullptr, hipEventRecord(e0, 0));
-  for (int i = 0; i < iters; ++i) fs_copy_kernel<<<blocks, 256>>>(a, b, n);
-  HIPCHK(nullptr, hipEventRecord(e1, 0));
-  HIPCHK(nullptr, hipEventSynchronize(e1));
+  fs_copy_kernel<<<blocks, 256, 0, st>>>(a, b, n);
+  PROBE(hipEventRecord(e0, st));
+  for (int i = 0; i < iters; ++i) fs_copy_kernel<<<blocks, 256, 0, st>>>(a, b, n);
+  PROBE(hipEventRecord(e1, st));
+  PROBE(hipEventSynchronize(e1));
+  PROBE(hipGetLastError());
   float t = 0.f;
-  HIPCHK(nullptr, hipEventElapsedTime(&t, e0, e1));
+  PROBE(hipEventElapsedTime(&t, e0, e1));
+#undef PROBE
   const double per = t / iters;
   if (ms) *ms = per;
   if (gbps) *gbps = 2.0 * (double)(n * 16) / (per * 1e-3) / 1e9;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
-  return FS_OK;
+  return done(FS_OK);
 }
 
 int fs_pool_create(size_t hbmBudgetBytes, fs_pool_t *out) {
@@ -1187,7 +1199,13 @@ int fs_pool_create(size_t hbmBudgetBytes, fs_pool_t *out) {
   if (e != hipSuccess) { delete P; return fail(nullptr, FS_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
   for (auto &ev : P->ev) {
     e = hipEventCreate(&ev);
-    if (e != hipSuccess) { delete P; return fail(nullptr, FS_E_HIP, "hipEventCreate: %s", hipGetErrorString(e)); }
+    if (e != hipSuccess) {
+      for (auto &made : P->ev)
+        if (made) (void)hipEventDestroy(made);
+      (void)hipStreamDestroy(P->stream);
+      delete P;
+      return fail(nullptr, FS_E_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+    }
   }
   *out = P;
   return FS_OK;

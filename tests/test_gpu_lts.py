@@ -162,3 +162,22 @@ def test_lts_varied_levels(fsa, seed, nx, ny):
         np.testing.assert_array_equal(t8[k], s[k])
     h.destroy()
     g.destroy()
+
+
+@pytest.mark.parametrize("second_order", [True, False])
+def test_lts_pool_and_film(fsa, second_order):
+    """A deep pool spreading over a conical slope wetted by a 0.8 mm film, east edge free: the pool's fast water
+    takes the fine levels and the film the coarse ones, so a waiting film cell beside a finer one passes water
+    out (the draining-time limiter with its store) and every level from 0 to 4 is occupied in some frame (the
+    oracle's tiles per level at 5 and 20 s: [68, 22, 12, 12, 30] and [0, 0, 0, 20, 124]). Both orders: first
+    order adds a record's gathers after it (sw_lts_fold), second order in its second stage. Against the
+    oracle's LTS: cycles and cell updates equal, depths (1.3e-6 / 1.4e-7 m), books."""
+    nx = ny = 96
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = 0.02 * np.hypot(I - 20, J - 48) / 10
+    opts = dict(nx=nx, ny=ny, dx=1.0, z=z.ravel(), manning=0.03, bc={"east": "outflow"}, secondOrder=second_order)
+    init = lambda o: o.setDepth(lambda X, Y, Z: np.where(np.hypot(X - 20, Y - 48) < 12, 2.0, 0.0008))
+    g, c, s, d = pair(fsa, opts, init, 30, lts=4, frame=5)
+    assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS
+    assert g.steps == c.steps and g.clock().cellUpdates == c.cellSteps
+    assert abs(s["mass"]["outflow"] - c.vol["outflow"]) <= 1e-3 * max(c.vol["outflow"], 1.0)

@@ -62,7 +62,7 @@ struct Slot {
   float *tsp = nullptr, *tileRf = nullptr;
   unsigned *speedSlots = nullptr;
   int32_t *tileOpen = nullptr, *ucells = nullptr;
-  float *sh = nullptr, *shu = nullptr, *shv = nullptr, *sw = nullptr;
+  float4 *st4 = nullptr;
   float *g0h = nullptr, *g0hu = nullptr, *g0hv = nullptr, *tst = nullptr;
   float4 *bsx = nullptr, *bsy = nullptr;
   // flood barriers: one separate allocation (crest, bidx per cell; faces, ground, flux, books per barrier)
@@ -330,7 +330,7 @@ DevInst devinst_of(const Slot &S, StepRec *rec, int wbeg, int wend) {
   d.L = S.L; d.tx = S.tx; d.ty = S.ty;
   d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.speedSlots = S.speedSlots; d.tileOpen = S.tileOpen;
   d.tileRf = S.rf ? S.tileRf : nullptr;
-  d.sh = S.sh; d.shu = S.shu; d.shv = S.shv; d.sw = S.sw;
+  d.st4 = S.st4;
   d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.tst = S.tst;
   d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
   d.urows = S.urows; d.ucells = S.ucells; d.plist = S.plist; d.capParts = kLtsUnitRuns * S.nunits;
@@ -967,7 +967,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     S.lv = (uint8_t *)take(nt); S.lv0 = (uint8_t *)take(nt);
     S.tsp = (float *)take(nt * 4); S.tileOpen = (int32_t *)take(nt * 4); S.tileRf = (float *)take(nt * 4);
     S.speedSlots = (unsigned *)take(kSpeedSlots * 4);
-    S.sh = (float *)take(N * 4); S.shu = (float *)take(N * 4); S.shv = (float *)take(N * 4); S.sw = (float *)take(N * 4);
+    S.st4 = (float4 *)take(N * 16);
     S.g0h = (float *)take(N * 4); S.g0hu = (float *)take(N * 4); S.g0hv = (float *)take(N * 4);
     S.tst = (float *)take(N * 4);
     S.gflag = (uint8_t *)take(N);
@@ -1020,7 +1020,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     if (rc) return rc;
     HIPCHK(P, hipMemset(S.lv, 0, nt));
     HIPCHK(P, hipMemset(S.speedSlots, 0, kSpeedSlots * 4));
-    HIPCHK(P, hipMemset(S.sh, 0, (size_t)((char *)S.bsx - (char *)S.sh)));      // store, gathers, flags
+    HIPCHK(P, hipMemset(S.st4, 0, (size_t)((char *)S.bsx - (char *)S.st4)));    // store, gathers, flags
     HIPCHK(P, hipMemset(S.bsx, 0, N * 16));   // each plane on its own: take() pads them apart
     HIPCHK(P, hipMemset(S.bsy, 0, N * 16));
     HIPCHK(P, hipMemset(S.ticket, 0, 4));     // (each record ends with it at 0 again)

@@ -52,6 +52,15 @@ __device__ __forceinline__ int2 gld(const int2 *row, int x) {
   const fs_i2 v = *(const FS_GLOBAL fs_i2 *)((const FS_GLOBAL char *)row + (uint32_t)x * 8u);
   return make_int2(v.x, v.y);
 }
+// the first two components of a 16-byte element (an 8-byte load)
+__device__ __forceinline__ float2 gld_lo(const float4 *row, int x) {
+  const fs_v2 v = *(const FS_GLOBAL fs_v2 *)((const FS_GLOBAL char *)row + (uint32_t)x * 16u);
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void gst(float4 *row, int x, float4 v) {
+  const fs_v4 t = {v.x, v.y, v.z, v.w};
+  *(FS_GLOBAL fs_v4 *)((FS_GLOBAL char *)row + (uint32_t)x * 16u) = t;
+}
 __device__ __forceinline__ void gst(float2 *row, int x, float2 v) {
   const fs_v2 t = {v.x, v.y};
   *(FS_GLOBAL fs_v2 *)((FS_GLOBAL char *)row + (uint32_t)x * 8u) = t;
@@ -158,7 +167,7 @@ struct DevInst {
   unsigned *speedSlots;         // kSpeedSlots block maxima of the tile speeds (float bits), zero between cycles
   const int32_t *tileOpen;      // open cells per tile
   const float *tileRf;          // rain multiplier summed over the open cells of a tile (rain field only)
-  float *sh, *shu, *shv, *sw;   // flux a waiting cell has gathered (Kahan residual sw)
+  float4 *st4;                  // flux a waiting cell has gathered: (h, its Kahan residual, hu, hv), one 16-byte load
   float *g0h, *g0hu, *g0hv;     // a record's first-stage gather of a waiting cell (added to sh.. after)
   float *tst;                   // a waiting cell's store with the record's first-stage gather: what the
                                 // second stage's theta sees while the cell's wavefront adds the gathers

@@ -686,7 +686,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
       if constexpr (LTS) {
         if (in2 && M2r && (wb2 || (STAGE == 0 && lv2 == m))) {
           const int k2 = (r - 2) * pitch + X;
-          if (STAGE == 0) { tsh = gld(I.sh, k2); tsw = gld(I.sw, k2); }
+          if (STAGE == 0) { const float2 t = gld_lo(I.st4, k2); tsh = t.x; tsw = t.y; }
           else tst = gld(I.tst, k2);
         }
       }
@@ -697,7 +697,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         const int k = ry * pitch + X;
         pw = gld(Ws, k);
         if (STAGE == 1) { ph0 = gld(I.h, k); pw0 = gld(I.w, k); phu0 = gld(I.hu, k); phv0 = gld(I.hv, k); }
-        if constexpr (LTS) { psh = gld(I.sh, k); psw = gld(I.sw, k); pshu = gld(I.shu, k); pshv = gld(I.shv, k); }
+        if constexpr (LTS) { const float4 t = gld(I.st4, k); psh = t.x; psw = t.y; pshu = t.z; pshv = t.w; }
         if (STAGE == 1) pso = load_src(I, k);
       }
     }
@@ -834,7 +834,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             const int k2 = (r - 2) * pitch + X;
             float st;
             if (STAGE == 0) {
-              st = PRE ? tsh - tsw : gld(I.sh, k2) - gld(I.sw, k2);
+              if constexpr (PRE) st = tsh - tsw;
+              else { const float2 t = gld_lo(I.st4, k2); st = t.x - t.y; }
               st2 = st;
             } else {
               // a waiting cell in the second stage: its store with the first stage's gather, as the first
@@ -915,7 +916,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
                 if (so) gst(I.tst, k, add_nc(st3, g0));
                 else gst(I.gflag, k, (uint8_t)1);
               } else {
-                float sh = gld(I.sh, k), sw = gld(I.sw, k), shu = gld(I.shu, k), shv = gld(I.shv, k);
+                const float4 t = gld(I.st4, k);
+                float sh = t.x, sw = t.y, shu = t.z, shv = t.w;
                 const float a0 = gld(I.g0h, k), a0u = gld(I.g0hu, k), a0v = gld(I.g0hv, k);
                 kadd(sh, sw, a0);
                 shu = add_nc(shu, a0u);
@@ -923,7 +925,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
                 kadd(sh, sw, g0);
                 shu = add_nc(shu, g0u);
                 shv = add_nc(shv, g0v);
-                gst(I.sh, k, sh); gst(I.sw, k, sw); gst(I.shu, k, shu); gst(I.shv, k, shv);
+                gst(I.st4, k, make_float4(sh, sw, shu, shv));
               }
             }
           } else {
@@ -943,8 +945,8 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
                 h0 = hh3; w0n = qw; hu0 = hu3; hv0 = hv3;
               }
               if constexpr (LTS) {
-                const float sh = PRE ? psh : gld(I.sh, k), sw = PRE ? psw : gld(I.sw, k);
-                const float shu = PRE ? pshu : gld(I.shu, k), shv = PRE ? pshv : gld(I.shv, k);
+                float sh = psh, sw = psw, shu = pshu, shv = pshv;
+                if constexpr (!PRE) { const float4 t = gld(I.st4, k); sh = t.x; sw = t.y; shu = t.z; shv = t.w; }
                 // the sources' layers, issued after the update's operands so that one wait covers them
                 // (sources_cell would otherwise wait on each in turn)
                 if constexpr (STAGE == 1 && !PRE && !EXT) pso = load_src(I, k);
@@ -987,7 +989,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
               }
               if constexpr (LTS) {
                 if (spent) {
-                  gst(I.sh, k, 0.f); gst(I.shu, k, 0.f); gst(I.shv, k, 0.f); gst(I.sw, k, 0.f);
+                  gst(I.st4, k, make_float4(0.f, 0.f, 0.f, 0.f));
                 }
               }
             }
@@ -1438,7 +1440,8 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
             sh[q] = sw[q] = shu[q] = shv[q] = a[q] = au[q] = av[q] = 0.f;
             if (f[q]) {
               const int k = (yb + q) * I.pitch + X;
-              sh[q] = gld(I.sh, k); sw[q] = gld(I.sw, k); shu[q] = gld(I.shu, k); shv[q] = gld(I.shv, k);
+              const float4 t = gld(I.st4, k);
+              sh[q] = t.x; sw[q] = t.y; shu[q] = t.z; shv[q] = t.w;
               a[q] = gld(I.g0h, k); au[q] = gld(I.g0hu, k); av[q] = gld(I.g0hv, k);
             }
           }
@@ -1449,7 +1452,7 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
             float h = sh[q], w = sw[q];
             kadd(h, w, a[q]);
             const float hu = shu[q] + au[q], hv = shv[q] + av[q];
-            gst(I.sh, k, h); gst(I.sw, k, w); gst(I.shu, k, hu); gst(I.shv, k, hv);
+            gst(I.st4, k, make_float4(h, w, hu, hv));
             gst(I.gflag, k, (uint8_t)0);
           }
         }
@@ -1780,7 +1783,7 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
         if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
       }
       gst(I.h, k, qh); gst(I.w, k, qw); gst(I.hu, k, qhu); gst(I.hv, k, qhv);
-      if (LTS) { gst(I.sh, k, 0.f); gst(I.shu, k, 0.f); gst(I.shv, k, 0.f); gst(I.sw, k, 0.f); }
+      if (LTS) gst(I.st4, k, make_float4(0.f, 0.f, 0.f, 0.f));
       stir = stir || qh != 0.f || qw != 0.f || qhu != 0.f || qhv != 0.f;
     }
   }

@@ -63,7 +63,7 @@ struct Slot {
   unsigned *speedSlots = nullptr;
   int32_t *tileOpen = nullptr, *ucells = nullptr;
   float4 *st4 = nullptr;
-  float *g0h = nullptr, *g0hu = nullptr, *g0hv = nullptr, *tst = nullptr;
+  float4 *g4 = nullptr;
   float4 *bsx = nullptr, *bsy = nullptr;
   // flood barriers: one separate allocation (crest, bidx per cell; faces, ground, flux, books per barrier)
   char *bmem = nullptr;
@@ -331,7 +331,7 @@ DevInst devinst_of(const Slot &S, StepRec *rec, int wbeg, int wend) {
   d.lv = S.lv; d.lv0 = S.lv0; d.tsp = S.tsp; d.speedSlots = S.speedSlots; d.tileOpen = S.tileOpen;
   d.tileRf = S.rf ? S.tileRf : nullptr;
   d.st4 = S.st4;
-  d.g0h = S.g0h; d.g0hu = S.g0hu; d.g0hv = S.g0hv; d.tst = S.tst;
+  d.g4 = S.g4;
   d.gflag = S.gflag; d.bsx = S.bsx; d.bsy = S.bsy;
   d.urows = S.urows; d.ucells = S.ucells; d.plist = S.plist; d.capParts = kLtsUnitRuns * S.nunits;
   d.rend = S.rend; d.ticket = S.ticket;
@@ -968,8 +968,7 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     S.tsp = (float *)take(nt * 4); S.tileOpen = (int32_t *)take(nt * 4); S.tileRf = (float *)take(nt * 4);
     S.speedSlots = (unsigned *)take(kSpeedSlots * 4);
     S.st4 = (float4 *)take(N * 16);
-    S.g0h = (float *)take(N * 4); S.g0hu = (float *)take(N * 4); S.g0hv = (float *)take(N * 4);
-    S.tst = (float *)take(N * 4);
+    S.g4 = (float4 *)take(N * 16);
     S.gflag = (uint8_t *)take(N);
     S.bsx = (float4 *)take(N * 16); S.bsy = (float4 *)take(N * 16);
     S.urows = (uint64_t *)take((size_t)S.nunits * 8 * (kMaxLevels + 1));

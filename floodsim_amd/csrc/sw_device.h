@@ -168,9 +168,8 @@ struct DevInst {
   const int32_t *tileOpen;      // open cells per tile
   const float *tileRf;          // rain multiplier summed over the open cells of a tile (rain field only)
   float4 *st4;                  // flux a waiting cell has gathered: (h, its Kahan residual, hu, hv), one 16-byte load
-  float *g0h, *g0hu, *g0hv;     // a record's first-stage gather of a waiting cell (added to sh.. after)
-  float *tst;                   // a waiting cell's store with the record's first-stage gather: what the
-                                // second stage's theta sees while the cell's wavefront adds the gathers
+  float4 *g4;                   // a waiting cell's record: its first-stage gather (h, hu, hv; added to the store
+                                // after) and its store with that gather, the second stage's theta reads (second order)
   uint8_t *gflag;               // first order: 1 = the cell gathered in the current record
   float4 *bsx, *bsy;            // a cell's reconstruction toward its faces at their last evaluation:
                                 // x (h west, dz west, h east, dz east), y (south, north); dz = face bed - cell bed

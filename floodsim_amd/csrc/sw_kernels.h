@@ -687,7 +687,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         if (in2 && M2r && (wb2 || (STAGE == 0 && lv2 == m))) {
           const int k2 = (r - 2) * pitch + X;
           if (STAGE == 0) { const float2 t = gld_lo(I.st4, k2); tsh = t.x; tsw = t.y; }
-          else tst = gld(I.tst, k2);
+          else tst = gld(reinterpret_cast<const float *>(I.g4), 4 * k2 + 3);
         }
       }
       const int ry = r - kHalo;
@@ -840,7 +840,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
             } else {
               // a waiting cell in the second stage: its store with the first stage's gather, as the first
               // stage left it (the cell's own wavefront folds the record's gathers into the store meanwhile)
-              st = PRE ? tst : gld(I.tst, k2);
+              st = PRE ? tst : gld(reinterpret_cast<const float *>(I.g4), 4 * k2 + 3);
             }
             hv_ = fmaxf(0.f, h2r + st);
           }
@@ -912,13 +912,13 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
               const float wd = so ? 0.5f * dt : dt;
               const float g0 = mul_nc(wd, dh), g0u = mul_nc(wd, dhu), g0v = mul_nc(wd, dhv);
               if (STAGE == 0) {
-                gst(I.g0h, k, g0); gst(I.g0hu, k, g0u); gst(I.g0hv, k, g0v);
-                if (so) gst(I.tst, k, add_nc(st3, g0));
-                else gst(I.gflag, k, (uint8_t)1);
+                gst(I.g4, k, make_float4(g0, g0u, g0v, so ? add_nc(st3, g0) : 0.f));
+                if (!so) gst(I.gflag, k, (uint8_t)1);
               } else {
                 const float4 t = gld(I.st4, k);
                 float sh = t.x, sw = t.y, shu = t.z, shv = t.w;
-                const float a0 = gld(I.g0h, k), a0u = gld(I.g0hu, k), a0v = gld(I.g0hv, k);
+                const float4 ga = gld(I.g4, k);
+                const float a0 = ga.x, a0u = ga.y, a0v = ga.z;
                 kadd(sh, sw, a0);
                 shu = add_nc(shu, a0u);
                 shv = add_nc(shv, a0v);
@@ -1442,7 +1442,8 @@ __global__ __launch_bounds__(256) void sw_lts_fold(const DevInst *__restrict__ i
               const int k = (yb + q) * I.pitch + X;
               const float4 t = gld(I.st4, k);
               sh[q] = t.x; sw[q] = t.y; shu[q] = t.z; shv[q] = t.w;
-              a[q] = gld(I.g0h, k); au[q] = gld(I.g0hu, k); av[q] = gld(I.g0hv, k);
+              const float4 ga = gld(I.g4, k);
+              a[q] = ga.x; au[q] = ga.y; av[q] = ga.z;
             }
           }
 #pragma unroll

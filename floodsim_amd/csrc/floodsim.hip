@@ -229,7 +229,7 @@ size_t slot_bytes(const fs_params *p, bool infil, bool rf, int nin) {
     const size_t nt = (size_t)((p->nx + 7) / 8) * ((p->ny + 7) / 8), nu = (size_t)units_of(p);
     b += 2 * align256(nt) + align256(nt * 4) * 3;           // lv, lv0, tsp, tileOpen, tileRf
     b += align256(kSpeedSlots * 4);                          // tile-speed block maxima
-    b += 8 * align256(N * 4) + align256(N) + 2 * align256(N * 16);   // store, gathers, tst, flag, bsx, bsy
+    b += 2 * align256(N * 16) + align256(N) + 2 * align256(N * 16);  // store, gathers + tst, flag, bsx, bsy
     b += align256(nu * 8 * (kMaxLevels + 1)) + align256(nu * 4 * (kMaxLevels + 1));   // urows, ucells
     b += align256(nu * kLtsUnitRuns * 16 * (kMaxLevels + 1));                         // plist
     b += align256(kLtsEndSlices * 3 * 8) + align256(4);                               // rend, ticket
@@ -982,6 +982,12 @@ int add_slot(fs_pool *P, const fs_params *p, const double *z, const double *mann
     const size_t cap = strip_cap(p);
     S.swet = (uint8_t *)take(cap); S.sact = (uint8_t *)take(cap);
     S.srain = (uint8_t *)take(cap); S.sopen = (int32_t *)take(cap * 4);
+  }
+  const size_t carved = (size_t)(q - S.mem);
+  if (carved > need) {   // the planes carved out must fit what slot_bytes counted
+    (void)hipFree(S.mem);
+    S.mem = nullptr;
+    return fail(P, FS_E_ARG, "internal: the instance's planes take %zu bytes, %zu counted", carved, need);
   }
   S.used = true;
   P->used += need;

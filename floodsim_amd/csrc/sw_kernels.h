@@ -442,17 +442,20 @@ __device__ __forceinline__ float cell_speed(float h, float hu, float hv, float g
 // taking the test out of the plain LTS kernels slows their schedule)
 // the per-cell layers the sources read (Manning n, max depth, mask byte, infiltration rate, rain multiplier)
 struct SrcOps { float n, hmax, infil, rf; uint8_t mask; };
+// SPEED: the caller takes the cell's wave speed for the next step (global stepping; local time stepping takes
+// its tile speeds at the cycle's start instead), which needs the mask's inlet bit
+template <bool SPEED = true>
 __device__ __forceinline__ SrcOps load_src(const DevInst &I, int k) {
   SrcOps o;
   o.n = gld(I.n, k);
   o.hmax = gld(I.hmax, k);
-  o.mask = gld(I.mask, k);
+  o.mask = SPEED ? gld(I.mask, k) : (uint8_t)0;
   o.infil = (I.flags & FLAG_INFIL) ? gld(I.infil, k) : 0.f;
   o.rf = (I.flags & FLAG_RF) ? gld(I.rf, k) : 1.f;
   return o;
 }
 
-template <bool EXT = false, bool IMPF = EXT>
+template <bool EXT = false, bool IMPF = EXT, bool SPEED = true>
 __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, float rain, float areaf,
                                              float &qh, float &qw, float &qhu, float &qhv, double &acc_inf,
                                              float &smax, const SrcOps *pre = nullptr) {
@@ -510,8 +513,10 @@ __device__ __forceinline__ void sources_cell(const DevInst &I, int k, float dt, 
   if (qh <= hdry) { qhu = 0.f; qhv = 0.f; }
   const float hm = pre ? pre->hmax : gld(I.hmax, k);
   if (qh > hm) gst(I.hmax, k, qh);
-  const uint8_t mk = pre ? pre->mask : gld(I.mask, k);
-  if (qh > hdry && !(mk & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
+  if constexpr (SPEED) {
+    const uint8_t mk = pre ? pre->mask : gld(I.mask, k);
+    if (qh > hdry && !(mk & MASK_INLET)) smax = fmaxf(smax, cell_speed(qh, qhu, qhv, g));
+  }
 }
 
 // Hazard products of a cell after a step of length dt ending at tnow (hazard, hzRecord,
@@ -698,7 +703,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
         pw = gld(Ws, k);
         if (STAGE == 1) { ph0 = gld(I.h, k); pw0 = gld(I.w, k); phu0 = gld(I.hu, k); phv0 = gld(I.hv, k); }
         if constexpr (LTS) { const float4 t = gld(I.st4, k); psh = t.x; psw = t.y; pshu = t.z; pshv = t.w; }
-        if (STAGE == 1) pso = load_src(I, k);
+        if (STAGE == 1) pso = load_src<!LTS>(I, k);
       }
     }
     // barrier crests of this row: (west face crest, coefficient, south face crest, coefficient)
@@ -949,7 +954,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
                 if constexpr (!PRE) { const float4 t = gld(I.st4, k); sh = t.x; sw = t.y; shu = t.z; shv = t.w; }
                 // the sources' layers, issued after the update's operands so that one wait covers them
                 // (sources_cell would otherwise wait on each in turn)
-                if constexpr (STAGE == 1 && !PRE && !EXT) pso = load_src(I, k);
+                if constexpr (STAGE == 1 && !PRE && !EXT) pso = load_src<!LTS>(I, k);
                 hu0 += shu;
                 hv0 += shv;
                 kadd(h0, w0n, sh - sw);
@@ -983,7 +988,7 @@ __device__ __forceinline__ bool run_strip(const DevInst &I, const int4 wk, const
               acc_out += o * ((double)dx * (double)dt * (double)wgt);
             }
             if (STAGE == 1) {
-              sources_cell<EXT, EXT || LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax, EXT ? nullptr : &pso);
+              sources_cell<EXT, EXT || LTS, !LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax, EXT ? nullptr : &pso);
               if constexpr (HZ) {
                 if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
               }
@@ -1779,7 +1784,7 @@ __device__ __forceinline__ bool euler_finish_strip(const DevInst &I, const int4 
       if (!(gld(I.mask, k) & MASK_OPEN)) continue;
       if (LTS && tile_level(I, X, y) != m) continue;
       float qh = gld(I.h1, k), qw = gld(I.w1, k), qhu = gld(I.hu1, k), qhv = gld(I.hv1, k);
-      sources_cell<EXT, EXT || LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
+      sources_cell<EXT, EXT || LTS, !LTS>(I, k, dt, rain, areaf, qh, qw, qhu, qhv, acc_inf, smax);
       if constexpr (HZ) {
         if (I.hzA) hazard_cell(I, k, qh, qhu, qhv, dt, tnow);
       }

@@ -28,7 +28,7 @@ def short_circuit_lane_moves(path):
             head = s[:m.start()]
             # a move nested in the argument of another move is evaluated with it; only && / || / ?: that
             # precede the move at the statement's level make it conditional
-            if "&&" in head or "||" in head or re.search(r"\?[^:]*$", head):
+            if "&&" in head or "||" in head or "?" in head:
                 bad.append(s[:160])
                 break
     return bad
@@ -46,6 +46,7 @@ def test_check_finds_a_conditional_lane_move(tmp_path):
     p.write_text("bool w = lv2 == m + 1 && (from_left_i(lv2) == m || lv1 == m);\n"
                  "const int a = from_left_i(lv2), b = from_right_i(lv2);\n"
                  "bool v = (lv2 == m + 1) & ((a == m) | (b == m));\n"
-                 "const bool onW = from_left_i(onE ? 1 : 0) != 0;\n")
+                 "const bool onW = from_left_i(onE ? 1 : 0) != 0;\n"
+                 "float q = c ? 0.f : from_right(v);\n")
     bad = short_circuit_lane_moves(str(p))
-    assert len(bad) == 1 and bad[0].startswith("bool w")
+    assert len(bad) == 2 and bad[0].startswith("bool w") and bad[1].startswith("float q")

@@ -181,3 +181,28 @@ def test_lts_pool_and_film(fsa, second_order):
     assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS
     assert g.steps == c.steps and g.clock().cellUpdates == c.cellSteps
     assert abs(s["mass"]["outflow"] - c.vol["outflow"]) <= 1e-3 * max(c.vol["outflow"], 1.0)
+
+
+@pytest.mark.parametrize("second_order", [True, False])
+def test_lts_inlets_pool_and_film(fsa, second_order):
+    """The pool-and-film case with three drainage inlets: a sink in the fast pool (a fine level), a point inflow
+    and a sink on the slow film (coarse levels). Each inlet exchanges at the end of its cell's level records
+    (exchangeL, kernels.wgsl.js:596), at both orders (first order: after the record's gather walk). Against the
+    oracle's LTS: cycles, depths, each inlet's exchanged volume and the books."""
+    nx = ny = 96
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    z = 0.02 * np.hypot(I - 20, J - 48) / 10
+    inlets = [48 * nx + 20, 48 * nx + 60, 20 * nx + 40]
+    opts = dict(nx=nx, ny=ny, dx=1.0, z=z.ravel(), manning=0.03, bc={"east": "outflow"}, secondOrder=second_order,
+                inlets=inlets)
+
+    def init(o):
+        o.setDepth(lambda X, Y, Z: np.where(np.hypot(X - 20, Y - 48) < 12, 2.0, 0.0008))
+        o.setInletRates([0.5, -0.2, 0.05])
+    g, c, s, d = pair(fsa, opts, init, 30, lts=4, frame=5)
+    vg, vc = g.readInlets()["vol"], c.readInlets()["vol"]
+    print(f"  inlet volumes gpu {np.round(vg, 4).tolist()} oracle {np.round(vc, 4).tolist()} m3")
+    assert d <= TOL_H and abs(s["mass"]["rel"]) <= TOL_MASS
+    assert g.steps == c.steps and g.clock().cellUpdates == c.cellSteps
+    assert np.all(np.abs(vg - vc) <= 1e-3 * np.abs(vc) + 1e-6)
+    assert abs(s["mass"]["exchanged"] - c.vol["exchanged"]) <= 1e-3 * abs(c.vol["exchanged"]) + 1e-6

@@ -1106,7 +1106,9 @@ __device__ __forceinline__ bool strip_begin(const DevInst &I, const int4 wk, int
 // runs into parts of 1, 2, 4, 8, 16 or 32 rows, the finest that keeps the record within kLtsFineWaves
 // wavefronts, so its parts fill the chip with short walks (its latency is the longest walk). A unit holds
 // at most kLtsUnitRuns parts for any part size of 8 rows or more (runs are at least 7 rows apart).
-constexpr int kLtsWaveCap = 2;        // wavefronts a record launches per work unit (the partials it has)
+// wavefronts a record launches per work unit (the partials it has); measured at 8192^2, LTS cycle 14.66 / 13.72 /
+// 13.43 / 13.50 ms with 1 / 2 / 3 / 4
+constexpr int kLtsWaveCap = 3;
 constexpr int kLtsFineWaves = 4096;   // parts a finely split record may have
 constexpr int kLtsUnitRuns = 8;       // parts a unit can hold (64 rows, runs at least 7 apart, parts >= 8 rows)
 constexpr int kLtsCand = 7;           // part sizes 1, 2, 4, 8, 16, 32 rows, then whole runs

@@ -122,8 +122,8 @@ int fs_advance(fs_handle s, double tTarget, int32_t K, fs_clock *clk);     /* ad
  * device memory, such as tight serving loops and harnesses that forbid hipMalloc. The same kernels in the
  * same order as fs_advance, launched plainly (no captured graphs), with the solver's descriptor, work list
  * and per-wavefront partials in a fixed 4 MiB device array that the library holds on each device until the
- * process ends (a solver of up to ~26,000 work items: any grid up to 8192 x 8192 at the default 64 rows per
- * strip, with or without local time stepping); a larger grid returns FS_E_ARG. Results are bit for bit those
+ * process ends (a solver of up to ~52,000 work items, ~20,000 with local time stepping: any grid up to
+ * 8192 x 8192 at the default 64 rows per strip, with or without it); a larger grid returns FS_E_ARG. Results are bit for bit those
  * of fs_advance. Every call on one device shares that array: calls that overlap on one device (other
  * threads, other streams) race, so they must be serialised by the caller. Timing mode (fs_pool_set_timing)
  * must be off (FS_E_STATE), and the solver's device must be current. */

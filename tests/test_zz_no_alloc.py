@@ -77,6 +77,13 @@ def test_noalloc_errors():
     assert b"scratch" in L.fs_last_error(g._h)
     assert g.clock().steps == 0
     g.destroy()
+    # the header's bound: 8192 x 8192 fits with local time stepping too (its records hold three partials a unit)
+    n = 8192
+    print(f"no-alloc advance: {n} x {n} with local time stepping must fit", flush=True)
+    big = fsa.GPUSolver(nx=n, ny=n, dx=1.0, z=np.zeros(n * n), manning=0.03, lts=4)
+    big.setDepth(lambda X, Y, Z: np.where(X < 64, 0.5, 0.0))
+    assert L.fs_advance_noalloc(big._h, 0.5, 4, ctypes.byref(c)) == 0 and c.t == 0.5
+    big.destroy()
     s = fsa.GPUSolver(nx=64, ny=32, dx=1.0, z=np.zeros(64 * 32), manning=0.03)
     s.setDepth(lambda X, Y, Z: np.where(X < 20, 1.0, 0.0))
     s.set_timing(True)

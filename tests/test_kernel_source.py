@@ -1,6 +1,7 @@
 """CPU checks of the kernel sources for a hazard no numerical test reliably shows.
 
-A lane move (DPP, from_left / from_right) must run with every lane of the wavefront active. Placed in the
+A lane move (DPP: from_left / from_right; and the other cross-lane reads: shuffles, ballots, the wavefront
+reductions) must run with every lane of the wavefront active. Placed in the
 right operand of && or ||, it runs only on the lanes whose left operand decides nothing, and the compiler may
 then also stand in the left operand's value for the moved operand (a test of the lane's own level, say), so
 the move reads other lanes' registers as that constant or as zero. The result is a wrong neighbour value with
@@ -10,7 +11,7 @@ import os
 import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LANE_MOVE = re.compile(r"\bfrom_(left|right)(_i)?\s*\(")
+LANE_MOVE = re.compile(r"\b(from_(left|right)(_i)?|__shfl(_xor|_up|_down)?|__ballot|__any|__all|wave_(sum|max)(_i)?)\s*\(")
 
 
 def statements(path):
